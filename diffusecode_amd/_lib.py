@@ -1,0 +1,297 @@
+"""ctypes binding of libdiffuse_turbo.so (include/diffuse_turbo.h).
+
+The shared library is built in-tree (diffusecode_amd/build.py).  Importing
+this module never falls back to anything else: if the library is missing or
+cannot be loaded, ``lib()`` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdiffuse_turbo.so")
+
+DC_OK = 0
+DC_WARN_NOCELL = 1
+DC_ERR_ISTL = -3
+DC_ERR_CANCEL = -14
+DC_ERR_ZEROVEC = -132
+DC_ERR_LOTS = -162
+DC_ERR_NOATOMS = -198
+DC_ERR_ARG = -1000
+DC_ERR_DEVICE = -1001
+DC_ERR_NOMEM = -1002
+DC_ERR_NODEV = -1003
+DC_ERR_STATE = -1004
+
+PREC = {"fp32": 0, "bf16x3": 1, "fp16x2": 2}
+LOT = {"off": 1, "box": 2, "eli": 3}
+CFPKT = 9999
+
+EXPORTS = [
+    "dc_create", "dc_destroy", "dc_last_error", "dc_last_error_type", "dc_set_precision",
+    "dc_get_precision", "dc_set_gemm_tile", "dc_gemm_num_tiles", "dc_gemm_tile_shape", "dc_set_cancel_flag", "dc_version", "dc_device_info", "dc_strucf",
+    "dc_crystal_set", "dc_four_run", "dc_four_run_dev", "dc_last_stats", "dc_synchronize",
+]
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class CrystalDesc(C.Structure):
+    _fields_ = [("cr_pos", _dp), ("cr_iscat", _ip), ("natoms", C.c_int64), ("icc", C.c_int * 3),
+                ("ncatoms", C.c_int), ("nscat", C.c_int), ("dim0", C.c_double * 3), ("dim_lo", C.c_double * 3)]
+
+
+class RunDesc(C.Structure):
+    _fields_ = [("eck", C.c_double * 12), ("vi", C.c_double * 9), ("inc", C.c_int * 3), ("rten", C.c_double * 9),
+                ("cfact", _dp), ("ncfact", C.c_int), ("ilots", C.c_int), ("ls", C.c_int * 3), ("nlots", C.c_int),
+                ("lbeg", _ip), ("lperiod", C.c_int), ("fave", C.c_double), ("sel_cell", C.POINTER(C.c_ubyte)),
+                ("four_friedel", C.c_int), ("ano", C.c_int), ("diff_scale", C.c_double)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("pairs", C.c_double), ("gemm_ms", C.c_double), ("total_ms", C.c_double),
+                ("reduce_ms", C.c_double), ("gemm_launches", C.c_int64), ("slices", C.c_int64)]
+
+
+_lib = None
+
+
+def lib():
+    """Load the library (raises when it is absent: there is no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m diffusecode_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    L.dc_create.argtypes = [C.POINTER(C.c_void_p), C.c_int]
+    L.dc_destroy.argtypes = [C.c_void_p]
+    L.dc_destroy.restype = None
+    L.dc_last_error.argtypes = [C.c_void_p]
+    L.dc_last_error.restype = C.c_char_p
+    L.dc_last_error_type.argtypes = [C.c_void_p]
+    L.dc_set_precision.argtypes = [C.c_void_p, C.c_int]
+    L.dc_get_precision.argtypes = [C.c_void_p]
+    L.dc_set_gemm_tile.argtypes = [C.c_void_p, C.c_int]
+    L.dc_gemm_tile_shape.argtypes = [C.c_int, _ip, _ip, _ip]
+    L.dc_set_cancel_flag.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+    L.dc_version.restype = C.c_char_p
+    L.dc_device_info.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.POINTER(C.c_int)]
+    L.dc_strucf.argtypes = [C.c_void_p, _dp, C.c_int64, C.c_int64, _dp, _dp, _dp, _dp, _ip, _ip, _ip, _dp,
+                            C.c_int64, _ip, _dp, _dp, C.c_int, _dp]
+    L.dc_crystal_set.argtypes = [C.c_void_p, C.POINTER(CrystalDesc)]
+    L.dc_four_run.argtypes = [C.c_void_p, C.POINTER(RunDesc), _dp, _dp, _dp, _dp, _dp]
+    L.dc_four_run_dev.argtypes = [C.c_void_p, C.POINTER(RunDesc), C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_void_p]
+    L.dc_last_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
+    L.dc_synchronize.argtypes = [C.c_void_p]
+    _lib = L
+    return L
+
+
+class DCError(RuntimeError):
+    """A non-zero status of the library; ``ier_num``/``ier_typ`` follow DISCUS."""
+
+    def __init__(self, code, typ, msg):
+        super().__init__(f"status {code} (ier_typ {typ}): {msg}")
+        self.ier_num = code
+        self.ier_typ = typ
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return None if a is None else a.ctypes.data_as(_ip)
+
+
+def _f64(a, n=None):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel(order="F"))
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+def _i32(a, n=None):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.int32).ravel(order="F"))
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+class Context:
+    """One library context (one HIP stream, device buffers kept between calls)."""
+
+    def __init__(self, device: int = -1, precision: str = "fp32"):
+        L = lib()
+        h = C.c_void_p()
+        st = L.dc_create(C.byref(h), device)
+        if st != 0:
+            raise DCError(st, 99, "dc_create failed (no gfx950 device?)")
+        self.h = h
+        self._keep = []
+        self.set_precision(precision)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().dc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def check(self, st, allow_warn=True):
+        if st < 0 or (st > 0 and not allow_warn):
+            L = lib()
+            raise DCError(st, L.dc_last_error_type(self.h), L.dc_last_error(self.h).decode())
+        return st
+
+    def set_precision(self, precision):
+        p = PREC[precision] if isinstance(precision, str) else int(precision)
+        self.check(lib().dc_set_precision(self.h, p))
+        self.precision = precision
+
+    def set_gemm_tile(self, tile):
+        self.check(lib().dc_set_gemm_tile(self.h, int(tile)))
+
+    @staticmethod
+    def gemm_tiles():
+        L = lib()
+        out = []
+        for t in range(L.dc_gemm_num_tiles()):
+            r, c, n = C.c_int(), C.c_int(), C.c_int()
+            L.dc_gemm_tile_shape(t, C.byref(r), C.byref(c), C.byref(n))
+            out.append((r.value, c.value, n.value))
+        return out
+
+    def set_cancel_flag(self, flag_array):
+        """flag_array: numpy int32 array of size 1 kept alive by the caller."""
+        self._keep.append(flag_array)
+        ptr = None if flag_array is None else flag_array.ctypes.data_as(_ip)
+        self.check(lib().dc_set_cancel_flag(self.h, ptr))
+
+    def stats(self):
+        s = Stats()
+        self.check(lib().dc_last_stats(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in Stats._fields_}
+
+    def device_info(self):
+        buf = C.create_string_buffer(64)
+        n = C.c_int()
+        self.check(lib().dc_device_info(self.h, buf, 64, C.byref(n)))
+        return buf.value.decode(), n.value
+
+    def synchronize(self):
+        self.check(lib().dc_synchronize(self.h))
+
+    # -- layer call ----------------------------------------------------------
+    def strucf(self, xat, uin, vin, win, off_shift, lmn, num, fnum=None, cfact_col=None, istl=None,
+               xm=None, rten=None):
+        """Drop-in four_strucf: xat (nxat,3) positions; returns tcsf (num) complex128, Fortran order."""
+        x = np.asfortranarray(np.asarray(xat, np.float64).reshape(-1, 3))
+        n = x.shape[0]
+        num = _i32(num, 3)
+        fnum = num.copy() if fnum is None else _i32(fnum, 3)
+        lform = cfact_col is not None
+        cf = None if cfact_col is None else np.ascontiguousarray(np.asarray(cfact_col, np.complex128)).view(np.float64)
+        ncf = 0 if cf is None else cf.size // 2
+        ist = None if istl is None else _i32(istl)
+        out = np.zeros(2 * int(np.prod(num)))
+        st = lib().dc_strucf(self.h, _d(x) if n else None, n, max(n, 1), _d(_f64(uin, 3)), _d(_f64(vin, 3)),
+                             _d(_f64(win, 3)), _d(_f64(off_shift, 9)), _i(_i32(lmn, 6)), _i(num), _i(fnum),
+                             _d(cf), ncf, _i(ist), _d(None if xm is None else _f64(xm, 3)),
+                             _d(None if rten is None else _f64(rten, 9)), int(lform), _d(out))
+        self.check(st)
+        return out.view(np.complex128).reshape(tuple(num), order="F")
+
+    # -- run level -----------------------------------------------------------
+    def crystal_set(self, crystal):
+        pos = np.ascontiguousarray(np.asarray(crystal.pos, np.float64))
+        isc = np.ascontiguousarray(np.asarray(crystal.iscat, np.int32))
+        d = CrystalDesc()
+        d.cr_pos = _d(pos)
+        d.cr_iscat = _i(isc)
+        d.natoms = pos.shape[0]
+        d.icc[:] = [int(v) for v in crystal.icc]
+        d.ncatoms = int(crystal.ncatoms)
+        d.nscat = int(crystal.nscat)
+        d.dim0[:] = [float(v) for v in crystal.dim0]
+        d.dim_lo[:] = [float(v) for v in crystal.dim_lo]
+        self.check(lib().dc_crystal_set(self.h, C.byref(d)))
+        self.crystal = crystal
+
+    @staticmethod
+    def run_desc(eck, vi, inc, rten, cfact, ilots=1, ls=(1, 1, 1), lbeg=None, lperiod=True, fave=0.0,
+                 sel_cell=None, four_friedel=True, ano=False, diff_scale=1.0):
+        keep = []
+        r = RunDesc()
+        r.eck[:] = list(_f64(eck, 12))
+        r.vi[:] = list(_f64(vi, 9))
+        r.inc[:] = [int(v) for v in inc]
+        r.rten[:] = list(_f64(rten, 9))
+        cf = np.asfortranarray(np.asarray(cfact, np.complex128))
+        cff = np.ascontiguousarray(cf.ravel(order="F")).view(np.float64)
+        keep.append(cff)
+        r.cfact = _d(cff)
+        r.ncfact = cf.shape[0]
+        r.ilots = int(ilots)
+        r.ls[:] = [int(v) for v in ls]
+        if lbeg is not None:
+            lb = np.ascontiguousarray(np.asarray(lbeg, np.int32).reshape(-1, 3))
+            keep.append(lb)
+            r.lbeg = _i(lb)
+            r.nlots = lb.shape[0]
+        else:
+            r.nlots = 1
+        r.lperiod = int(bool(lperiod))
+        r.fave = float(fave)
+        if sel_cell is not None:
+            sc = np.ascontiguousarray(np.asarray(sel_cell, np.uint8).ravel(order="F"))
+            keep.append(sc)
+            r.sel_cell = sc.ctypes.data_as(C.POINTER(C.c_ubyte))
+        r.four_friedel = int(bool(four_friedel))
+        r.ano = int(bool(ano))
+        r.diff_scale = float(diff_scale)
+        return r, keep
+
+    def four_run(self, desc, want_var=False):
+        num = tuple(desc.inc)
+        ntot = int(np.prod(num))
+        csf = np.zeros(2 * ntot)
+        acsf = np.zeros(2 * ntot)
+        dsi = np.zeros(ntot)
+        fm = np.zeros(2 * ntot) if want_var else None
+        dv = np.zeros(ntot) if want_var else None
+        st = self.check(lib().dc_four_run(self.h, C.byref(desc), _d(csf), _d(acsf), _d(dsi), _d(fm), _d(dv)))
+        out = {"status": st,
+               "csf": csf.view(np.complex128).reshape(num, order="F"),
+               "acsf": acsf.view(np.complex128).reshape(num, order="F"),
+               "dsi": dsi.reshape(num, order="F")}
+        if want_var:
+            out["fmean"] = fm.view(np.complex128).reshape(num, order="F")
+            out["dvar"] = dv.reshape(num, order="F")
+        return out
+
+    def four_run_dev(self, desc, csf=None, acsf=None, dsi=None, fmean=None, dvar=None):
+        """Device-pointer variant: arguments are torch CUDA tensors (float64 /
+        view_as_real complex128) or raw integer device addresses."""
+        def p(t):
+            if t is None:
+                return None
+            return t if isinstance(t, int) else t.data_ptr()
+        return self.check(lib().dc_four_run_dev(self.h, C.byref(desc), p(csf), p(acsf), p(dsi), p(fmean), p(dvar)))

@@ -1,0 +1,966 @@
+// dc_api.hip -- C ABI of libdiffuse_turbo.so (declared in include/diffuse_turbo.h).
+//
+// Host side of the MI355X structure-factor path.  It owns device buffers
+// inside a context, restates the host bookkeeping of DISCUS's four_run
+// (grid set-up, Friedel test, lot gathering, average structure, lot loop;
+// discus/prog/fourier_sup.f90:11-323) and drives the kernels of dc_gemm.hip
+// and dc_lot.hip on one HIP stream.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/diffuse_turbo.h"
+#include "dc_internal.h"
+
+using namespace dc;
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+};
+
+constexpr int GEMM_KS = 16;
+constexpr int KCHUNK_MIN = 512;
+constexpr int KCHUNK_MAX = 16384;
+constexpr double PARTIAL_BUDGET = 6.0e9;   // bytes of slice planes per batch
+
+}  // namespace
+
+struct dc_ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    int prec = DC_PREC_FP32;
+    int tile = -1;                  // forced GEMM tile configuration, -1 = planned
+    std::string err;
+    int err_typ = DC_ER_NONE;
+    const volatile int *cancel = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    dc_stats stats{};
+    // device buffers
+    DevBuf rec, slice_beg, slice_type, lot_slice, partial, cfact, istl, idx, oid, offs, xat, w;
+    DevBuf acsf, dsi, csf, wm, wv, lat, out_tcsf, out[5];
+    // crystal
+    bool have_crystal = false;
+    DevBuf pos, iscat;
+    std::vector<double> h_pos;
+    std::vector<int> h_iscat;
+    int64_t natoms = 0;
+    int icc[3] = {1, 1, 1};
+    int ncatoms = 1, nscat = 0;
+    double dim0[3] = {0, 0, 0}, dim_lo[3] = {0, 0, 0};
+    std::vector<int> type_beg;      // per type offsets into d type list
+    DevBuf type_idx;
+};
+
+namespace {
+
+int set_err(dc_ctx *c, int code, int typ, const std::string &msg) {
+    if (c) {
+        c->err = msg;
+        c->err_typ = typ;
+    }
+    return code;
+}
+
+int hip_err(dc_ctx *c, hipError_t e, const char *where) {
+    return set_err(c, e == hipErrorOutOfMemory ? DC_ERR_NOMEM : DC_ERR_DEVICE, DC_ER_LIB,
+                   std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(call)                                   \
+    do {                                               \
+        hipError_t e_ = (call);                        \
+        if (e_ != hipSuccess) return hip_err(c, e_, #call); \
+    } while (0)
+
+int ensure(dc_ctx *c, DevBuf &b, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (b.n >= bytes) return 0;
+    if (b.p) {
+        hipError_t e = hipStreamSynchronize(c->st);
+        if (e != hipSuccess) return hip_err(c, e, "sync before realloc");
+        (void)hipFree(b.p);
+        b.p = nullptr;
+        b.n = 0;
+    }
+    size_t n = bytes + bytes / 8;
+    hipError_t e = hipMalloc(&b.p, n);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return hip_err(c, e, "hipMalloc");
+    }
+    b.n = n;
+    return 0;
+}
+
+template <class T>
+T *ptr(DevBuf &b) {
+    return reinterpret_cast<T *>(b.p);
+}
+
+template <class T>
+int upload(dc_ctx *c, DevBuf &b, const T *src, size_t count) {
+    int st = ensure(c, b, sizeof(T) * count);
+    if (st) return st;
+    if (count) HIPCHK(hipMemcpyAsync(b.p, src, sizeof(T) * count, hipMemcpyHostToDevice, c->st));
+    return 0;
+}
+
+bool cancelled(dc_ctx *c) { return c->cancel && *c->cancel; }
+
+// sin(theta)/lambda bin with the expression of four_stltab
+// (fourier_sup.f90:2663-2674); mirrors istl_point of dc_lot.hip.
+#pragma clang fp contract(off)
+int host_istl(const double *xm, const double *u, const double *v, const double *w, int i, int j, int l,
+              const double *rten) {
+    double h[3], t[3];
+    for (int k = 0; k < 3; ++k) h[k] = xm[k] + u[k] * (double)i + v[k] * (double)j + w[k] * (double)l;
+    for (int r = 0; r < 3; ++r) t[r] = rten[r] * h[0] + rten[r + 3] * h[1] + rten[r + 6] * h[2];
+    double q2 = (h[0] * t[0] + h[1] * t[1] + h[2] * t[2]) / 4.0;
+    return (int)std::llround(std::sqrt(q2) * (1.0 / CFINC));
+}
+#pragma clang fp contract(on)
+
+// s is a norm of h, so its maximum over the parallelepiped grid sits at a corner.
+int max_corner_bin(const Grid &g) {
+    int m = 0;
+    for (int ci = 0; ci < 8; ++ci) {
+        int i = (ci & 1) ? g.n[0] - 1 : 0, j = (ci & 2) ? g.n[1] - 1 : 0, l = (ci & 4) ? g.n[2] - 1 : 0;
+        m = std::max(m, host_istl(g.xm, g.uin, g.vin, g.win, i, j, l, g.rten));
+    }
+    return m;
+}
+
+// four_test_friedel (fourier_sup.f90:3645-3728): reduced sizes; returns the
+// halved axis (1..3) or 0.
+int test_friedel(const double *eck, const double *vi, const int *inc, int ano, int four_friedel, int fnum[3]) {
+    fnum[0] = inc[0];
+    fnum[1] = inc[1];
+    fnum[2] = inc[2];
+    if (!four_friedel || ano) return 0;
+    double point[3];
+    for (int r = 0; r < 3; ++r) {
+        double rut = eck[r] + vi[r] * (fnum[0] - 1) + vi[r + 3] * (fnum[1] - 1) + vi[r + 6] * (fnum[2] - 1);
+        point[r] = eck[r] + rut;
+    }
+    const double eps = (double)1.E-6f;
+    if ((fnum[0] % 2) && (fnum[1] % 2) && (fnum[2] % 2)) {
+        if (std::fabs(point[0]) < eps && std::fabs(point[2]) < eps && std::fabs(point[1]) < eps) {
+            if (fnum[0] > 1) { fnum[0] = (fnum[0] + 1) / 2; return 1; }
+            if (fnum[1] > 1) { fnum[1] = (fnum[1] + 1) / 2; return 2; }
+            if (fnum[2] > 1) { fnum[2] = (fnum[2] + 1) / 2; return 3; }
+        }
+    }
+    return 0;
+}
+
+// Zero increment vector test of fourier_lmn (fourier_lmn_mod.f90:74-93),
+// with its rounding of vi to 1e-7.
+bool zero_increment(const double *vi, const int *inc) {
+    for (int i = 0; i < 3; ++i) {
+        if (inc[i] <= 1) continue;
+        double s = 0.0;
+        for (int r = 0; r < 3; ++r) {
+            double v = (double)(long long)std::llround(vi[r + 3 * i] * 1.0e7) / 1.0e7;
+            s += v * v;
+        }
+        if (!(std::sqrt(s) > 1.0e-7)) return true;
+    }
+    return false;
+}
+
+long long padded(long long n, long long t) { return (n + t - 1) / t * t; }
+
+// GEMM plan for the computed region f of grid g.
+//   mapping 0: rows = abscissa (no wrap), cols = (ordinate, top), fast period C0
+//   mapping 1: rows = (abscissa, ordinate), fast period R0, cols = top (no wrap)
+// Fast periods are padded to >= 16 when the side wraps (the two-level operand
+// tables of dc_gemm.hip need at most one wrap per 16-group); padded pixels
+// are computed and never read.  The tile configuration minimises the padded
+// work of whole work-group tiles (ties: larger tiles); forced >= 0 overrides.
+struct Plan {
+    GemmGeom gg{};
+    int mapping = 0;
+    int tile = 0;
+};
+
+Plan plan_gemm(Grid &g, int forced) {
+    Plan best;
+    double best_cost = 1e300;
+    const int nt = gemm_num_tiles();
+    for (int mapping = 0; mapping < 2; ++mapping) {
+        if (mapping == 1 && g.f[2] <= 1) continue;
+        GemmGeom gg{};
+        long long ps[3];
+        if (mapping == 0) {
+            gg.R = g.f[0];
+            gg.R0 = g.f[0];
+            gg.C0 = g.f[2] > 1 ? std::max(16, g.f[1]) : g.f[1];
+            gg.C = gg.C0 * g.f[2];
+            ps[0] = 1; ps[1] = gg.R; ps[2] = (long long)gg.R * gg.C0;
+        } else {
+            gg.R0 = g.f[1] > 1 ? std::max(16, g.f[0]) : g.f[0];
+            gg.R = gg.R0 * g.f[1];
+            gg.C = g.f[2];
+            gg.C0 = g.f[2];
+            ps[0] = 1; ps[1] = gg.R0; ps[2] = gg.R;
+        }
+        gg.P = (long long)gg.R * gg.C;
+        for (int t = 0; t < nt; ++t) {
+            if (forced >= 0 && t != forced) continue;
+            TileInfo ti = gemm_tile_info(t);
+            double work = (double)padded(gg.R, ti.rt) * (double)padded(gg.C, ti.ct);
+            double cost = work * (1.0 + 0.5 * (16.0 / ti.rt + 16.0 / ti.ct));
+            if (cost < best_cost) {
+                best_cost = cost;
+                best.gg = gg;
+                best.mapping = mapping;
+                best.tile = t;
+                g.ps[0] = ps[0]; g.ps[1] = ps[1]; g.ps[2] = ps[2];
+                g.plane = gg.P;
+            }
+        }
+    }
+    return best;
+}
+
+int kchunk_for(const Plan &pl, long long natoms_batch) {
+    TileInfo ti = gemm_tile_info(pl.tile);
+    long long wgs = ((pl.gg.R + ti.rt - 1) / ti.rt) * ((pl.gg.C + ti.ct - 1) / ti.ct);
+    long long want = std::max<long long>(1, (1024 + wgs - 1) / wgs);
+    long long k = (natoms_batch + want - 1) / want;
+    k = std::min<long long>(std::max<long long>(k, KCHUNK_MIN), KCHUNK_MAX);
+    return (int)padded(k, GEMM_KS);
+}
+
+// Slices over a list of (type, count) segments laid out consecutively.
+struct SliceSet {
+    std::vector<int> beg;        // nslice + 1 atom offsets
+    std::vector<int> type;       // per slice
+    std::vector<int> lot;        // lot boundaries (slice ids), nlots + 1
+    std::vector<int> seg_first;  // first slice of each segment
+};
+
+void add_segments(SliceSet &s, const std::vector<std::pair<int, int>> &segs, int kchunk, int &atom_off) {
+    for (auto &sg : segs) {
+        int n = sg.second;
+        s.seg_first.push_back((int)s.type.size());
+        for (int b = 0; b < n; b += kchunk) {
+            if (s.beg.empty()) s.beg.push_back(atom_off);
+            s.type.push_back(sg.first);
+            s.beg.push_back(atom_off + std::min(n, b + kchunk));
+        }
+        atom_off += n;
+    }
+}
+
+int run_gemm(dc_ctx *c, const Plan &pl, int nslice, float2 *partial) {
+    const GemmGeom &gg = pl.gg;
+    if (nslice <= 0) return 0;
+    HIPCHK(hipEventRecord(c->ev[2], c->st));
+    // launch in chunks of slices so that the cancel flag is polled
+    const int per = 4096;
+    for (int s0 = 0; s0 < nslice; s0 += per) {
+        if (cancelled(c)) return set_err(c, DC_ERR_CANCEL, DC_ER_COMM, "cancelled");
+        int ns = std::min(per, nslice - s0);
+        HIPCHK(launch_phase_gemm(c->prec, pl.tile, ptr<AtomRec>(c->rec), ptr<int>(c->slice_beg),
+                                 partial + (long long)s0 * gg.P, gg, s0, ns, c->st));
+        c->stats.gemm_launches++;
+    }
+    HIPCHK(hipEventRecord(c->ev[3], c->st));
+    HIPCHK(hipEventSynchronize(c->ev[3]));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+    c->stats.gemm_ms += ms;
+    return 0;
+}
+
+PrepArgs base_prep(const Grid &g, int mapping) {
+    // (mapping selects which increments feed the row and column sides)
+    PrepArgs a{};
+    for (int k = 0; k < 3; ++k) {
+        a.vec[k] = g.uin[k];
+        a.vec[3 + k] = g.vin[k];
+        a.vec[6 + k] = g.win[k];
+        a.h0[k] = g.xm[k];
+    }
+    a.mode = PREP_CORNER;
+    a.mapping = mapping;
+    return a;
+}
+
+// Compute sum over atom segments (idx list on device or explicit positions)
+// into planes; used by the average structure and the lattice.
+struct GatherList {
+    std::vector<int> idx;
+    std::vector<int> oid;
+    std::vector<double> offs;
+    std::vector<std::pair<int, int>> segs;   // (type 0-based, count)
+};
+
+}  // namespace
+
+// ============================================================================
+extern "C" {
+
+const char *dc_version(void) { return "diffuse_turbo 0.1 (gfx950)"; }
+
+int dc_create(dc_ctx **out, int device) {
+    if (!out) return DC_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return DC_ERR_NODEV;
+    if (device < 0) {
+        if (hipGetDevice(&device) != hipSuccess) return DC_ERR_NODEV;
+    }
+    if (device >= ndev) return DC_ERR_NODEV;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return DC_ERR_NODEV;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return DC_ERR_NODEV;
+    dc_ctx *c = new dc_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return DC_ERR_DEVICE;
+    }
+    for (auto &e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete c;
+            return DC_ERR_DEVICE;
+        }
+    *out = c;
+    return DC_OK;
+}
+
+void dc_destroy(dc_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->st);
+    DevBuf *bufs[] = {&c->rec, &c->slice_beg, &c->slice_type, &c->lot_slice, &c->partial, &c->cfact,
+                      &c->istl, &c->idx, &c->oid, &c->offs, &c->xat, &c->w, &c->acsf, &c->dsi, &c->csf,
+                      &c->wm, &c->wv, &c->lat, &c->out_tcsf, &c->pos, &c->iscat, &c->type_idx,
+                      &c->out[0], &c->out[1], &c->out[2], &c->out[3], &c->out[4]};
+    for (DevBuf *b : bufs)
+        if (b->p) (void)hipFree(b->p);
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->st);
+    delete c;
+}
+
+const char *dc_last_error(const dc_ctx *c) { return c ? c->err.c_str() : "null context"; }
+int dc_last_error_type(const dc_ctx *c) { return c ? c->err_typ : DC_ER_LIB; }
+
+int dc_set_precision(dc_ctx *c, int p) {
+    if (!c) return DC_ERR_ARG;
+    if (p != DC_PREC_FP32 && p != DC_PREC_BF16X3 && p != DC_PREC_FP16X2)
+        return set_err(c, DC_ERR_ARG, DC_ER_LIB, "unknown precision");
+    c->prec = p;
+    return DC_OK;
+}
+int dc_get_precision(const dc_ctx *c) { return c ? c->prec : -1; }
+
+int dc_set_gemm_tile(dc_ctx *c, int tile) {
+    if (!c) return DC_ERR_ARG;
+    if (tile < -1 || tile >= gemm_num_tiles()) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "unknown tile configuration");
+    c->tile = tile;
+    return DC_OK;
+}
+
+int dc_gemm_num_tiles(void) { return gemm_num_tiles(); }
+
+int dc_gemm_tile_shape(int tile, int *rows, int *cols, int *threads) {
+    if (tile < 0 || tile >= gemm_num_tiles()) return DC_ERR_ARG;
+    TileInfo t = gemm_tile_info(tile);
+    if (rows) *rows = t.rt;
+    if (cols) *cols = t.ct;
+    if (threads) *threads = t.nt;
+    return DC_OK;
+}
+
+int dc_set_cancel_flag(dc_ctx *c, const volatile int *flag) {
+    if (!c) return DC_ERR_ARG;
+    c->cancel = flag;
+    return DC_OK;
+}
+
+int dc_device_info(const dc_ctx *c, char *name, int len, int *num_cu) {
+    if (!c) return DC_ERR_ARG;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) != hipSuccess) return DC_ERR_DEVICE;
+    if (name && len > 0) {
+        std::strncpy(name, prop.gcnArchName, len - 1);
+        name[len - 1] = 0;
+    }
+    if (num_cu) *num_cu = prop.multiProcessorCount;
+    return DC_OK;
+}
+
+int dc_last_stats(const dc_ctx *c, dc_stats *out) {
+    if (!c || !out) return DC_ERR_ARG;
+    *out = c->stats;
+    return DC_OK;
+}
+
+int dc_synchronize(dc_ctx *c) {
+    if (!c) return DC_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return DC_OK;
+}
+
+// ----------------------------------------------------------------------------
+int dc_strucf(dc_ctx *c, const double *xat, int64_t nxat, int64_t ld_xat, const double uin[3],
+              const double vin[3], const double win[3], const double off_shift[9], const int lmn[6],
+              const int num[3], const int fnum[3], const double *cfact_col, int64_t ncfact, const int *istl,
+              const double xm[3], const double rten[9], int lform, double *tcsf) {
+    if (!c) return DC_ERR_ARG;
+    c->err.clear();
+    c->err_typ = DC_ER_NONE;
+    if (!uin || !vin || !win || !off_shift || !lmn || !num || !fnum || !tcsf || nxat < 0 || (nxat > 0 && (!xat || ld_xat < nxat)))
+        return set_err(c, DC_ERR_ARG, DC_ER_LIB, "dc_strucf: null pointer or bad size");
+    for (int k = 0; k < 3; ++k)
+        if (num[k] < 1 || fnum[k] < 1 || fnum[k] > num[k]) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "dc_strucf: bad num/fnum");
+    if (lform && (!cfact_col || ncfact < 1)) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "dc_strucf: lform needs cfact");
+    if (lform && !istl && (!xm || !rten)) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "dc_strucf: need istl or xm+rten");
+    if (nxat > INT32_MAX) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "dc_strucf: too many atoms");
+    if (cancelled(c)) return set_err(c, DC_ERR_CANCEL, DC_ER_COMM, "cancelled");
+    HIPCHK(hipSetDevice(c->device));
+    std::memset(&c->stats, 0, sizeof(c->stats));
+    HIPCHK(hipEventRecord(c->ev[0], c->st));
+
+    Grid g{};
+    for (int k = 0; k < 3; ++k) {
+        g.n[k] = num[k];
+        g.f[k] = fnum[k];
+        g.uin[k] = uin[k];
+        g.vin[k] = vin[k];
+        g.win[k] = win[k];
+        g.xm[k] = xm ? xm[k] : 0.0;
+    }
+    for (int k = 0; k < 9; ++k) g.rten[k] = rten ? rten[k] : 0.0;
+    g.P = (long long)fnum[0] * fnum[1] * fnum[2];
+    const long long ntot = (long long)num[0] * num[1] * num[2];
+
+    if (lform && istl) {
+        for (int l = 0; l < fnum[2]; ++l)
+            for (int j = 0; j < fnum[1]; ++j)
+                for (int i = 0; i < fnum[0]; ++i) {
+                    int b = istl[i + (long long)num[0] * (j + (long long)num[1] * l)];
+                    if (b < 0 || b >= ncfact) return set_err(c, DC_ERR_ISTL, DC_ER_FOUR, "istl outside cfact");
+                }
+    } else if (lform) {
+        if (max_corner_bin(g) >= ncfact) return set_err(c, DC_ERR_ISTL, DC_ER_FOUR, "sin(theta)/lambda beyond table");
+    }
+
+    const Plan pl = plan_gemm(g, c->tile);
+    const GemmGeom &gg = pl.gg;
+    const int mapping = pl.mapping;
+
+    int st = 0;
+    // positions: pack xat(ld,3) columns into (n,3) Fortran layout on device
+    if ((st = ensure(c, c->xat, sizeof(double) * 3 * std::max<int64_t>(nxat, 1)))) return st;
+    for (int col = 0; col < 3 && nxat > 0; ++col)
+        HIPCHK(hipMemcpyAsync(ptr<double>(c->xat) + col * nxat, xat + col * ld_xat, sizeof(double) * nxat,
+                              hipMemcpyHostToDevice, c->st));
+    if ((st = ensure(c, c->rec, sizeof(AtomRec) * std::max<int64_t>(nxat, 1)))) return st;
+    PrepArgs pa = base_prep(g, mapping);
+    pa.mode = PREP_LMN;
+    for (int k = 0; k < 9; ++k) pa.vec[9 + k] = off_shift[k];
+    for (int k = 0; k < 6; ++k) pa.lmn[k] = lmn[k];
+    pa.src = ptr<double>(c->xat);
+    pa.src_ld = std::max<int64_t>(nxat, 1);
+    pa.n = nxat;
+    HIPCHK(launch_prep(pa, ptr<AtomRec>(c->rec), c->st));
+
+    SliceSet ss;
+    int off = 0;
+    add_segments(ss, {{0, (int)nxat}}, kchunk_for(pl, nxat), off);
+    if (ss.beg.empty()) ss.beg.push_back(0);
+    int nslice = (int)ss.type.size();
+    ss.lot = {0, nslice};
+    if ((st = upload(c, c->slice_beg, ss.beg.data(), ss.beg.size()))) return st;
+    if ((st = upload(c, c->lot_slice, ss.lot.data(), ss.lot.size()))) return st;
+    if ((st = ensure(c, c->partial, sizeof(float2) * gg.P * std::max(nslice, 1)))) return st;
+    if ((st = run_gemm(c, pl, nslice, ptr<float2>(c->partial)))) return st;
+
+    ReduceArgs ra{};
+    ra.g = g;
+    ra.partial = ptr<float2>(c->partial);
+    ra.slice0 = 0;
+    ra.lot_slice = ptr<int>(c->lot_slice);
+    ra.nlots = 1;
+    ra.lform = lform;
+    if (lform) {
+        if ((st = upload(c, c->cfact, cfact_col, 2 * (size_t)ncfact))) return st;
+        ra.cfact = ptr<double2>(c->cfact);
+        ra.ncfact = (int)ncfact;
+        if (istl) {
+            if ((st = upload(c, c->istl, istl, (size_t)ntot))) return st;
+            ra.istl = ptr<int>(c->istl);
+        }
+    }
+    if ((st = ensure(c, c->out_tcsf, sizeof(double) * 2 * ntot))) return st;
+    HIPCHK(launch_strucf_out(ra, ptr<double>(c->out_tcsf), ntot, c->st));
+    HIPCHK(hipMemcpyAsync(tcsf, c->out_tcsf.p, sizeof(double) * 2 * ntot, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipEventRecord(c->ev[1], c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    c->stats.total_ms = ms;
+    c->stats.pairs = (double)nxat * (double)ntot;
+    c->stats.slices = nslice;
+    return DC_OK;
+}
+
+// ----------------------------------------------------------------------------
+int dc_crystal_set(dc_ctx *c, const dc_crystal_desc *d) {
+    if (!c) return DC_ERR_ARG;
+    c->err.clear();
+    if (!d || d->natoms < 0 || (d->natoms > 0 && (!d->cr_pos || !d->cr_iscat)) || d->nscat < 1 || d->ncatoms < 1 ||
+        d->icc[0] < 1 || d->icc[1] < 1 || d->icc[2] < 1 || d->natoms > INT32_MAX)
+        return set_err(c, DC_ERR_ARG, DC_ER_LIB, "dc_crystal_set: bad descriptor");
+    HIPCHK(hipSetDevice(c->device));
+    c->natoms = d->natoms;
+    c->ncatoms = d->ncatoms;
+    c->nscat = d->nscat;
+    for (int k = 0; k < 3; ++k) {
+        c->icc[k] = d->icc[k];
+        c->dim0[k] = d->dim0[k];
+        c->dim_lo[k] = d->dim_lo[k];
+    }
+    c->h_pos.assign(d->cr_pos, d->cr_pos + 3 * d->natoms);
+    c->h_iscat.assign(d->cr_iscat, d->cr_iscat + d->natoms);
+    int st = 0;
+    if ((st = upload(c, c->pos, c->h_pos.data(), c->h_pos.size()))) return st;
+    // per-type atom lists in crystal order (LOT_OFF gather, four_getatm :1839-1848)
+    std::vector<std::vector<int>> lists(c->nscat);
+    for (int64_t i = 0; i < d->natoms; ++i) {
+        int t = d->cr_iscat[i];
+        if (t >= 1 && t <= c->nscat) lists[t - 1].push_back((int)i);
+    }
+    std::vector<int> flat;
+    c->type_beg.assign(c->nscat + 1, 0);
+    for (int t = 0; t < c->nscat; ++t) {
+        c->type_beg[t] = (int)flat.size();
+        flat.insert(flat.end(), lists[t].begin(), lists[t].end());
+    }
+    c->type_beg[c->nscat] = (int)flat.size();
+    if ((st = upload(c, c->type_idx, flat.data(), flat.size()))) return st;
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->have_crystal = true;
+    return DC_OK;
+}
+
+namespace {
+
+// four_getatm for box/ellipsoid lots (fourier_sup.f90:1852-1968): per type,
+// the atoms of the lot in DISCUS order with the lot's periodic offsets.
+// Offsets are per (lot, wrap pattern): oid = lot*8 + wrapbits.
+long long gather_lot(const dc_ctx *c, const dc_run_desc *r, int lot, int t1, std::vector<int> &idx,
+                     std::vector<int> &oid, std::vector<double> &offs) {
+    const int *lbeg = r->lbeg + 3 * lot;
+    double x0[3];
+    for (int k = 0; k < 3; ++k) x0[k] = (double)(float)r->ls[k] / 2.0;
+    const size_t off_base = offs.size() / 3;
+    for (int wb = 0; wb < 8; ++wb)
+        for (int k = 0; k < 3; ++k)
+            offs.push_back(c->dim0[k] + lbeg[k] - 1 - (((wb >> k) & 1) ? (double)c->icc[k] : 0.0));
+    const long long cr_end = (long long)c->ncatoms * c->icc[0] * c->icc[1] * c->icc[2];
+    long long ncell = 0;
+    for (int kk = 0; kk < r->ls[2]; ++kk) {
+        int ic2 = kk + lbeg[2], w2 = 0;
+        double xt2 = ((double)kk - x0[2] + 0.5) * ((double)kk - x0[2] + 0.5) / (x0[2] * x0[2]);
+        if (ic2 > c->icc[2]) { ic2 -= c->icc[2]; w2 = 4; }
+        for (int jj = 0; jj < r->ls[1]; ++jj) {
+            int ic1 = jj + lbeg[1], w1 = 0;
+            double xt1 = ((double)jj - x0[1] + 0.5) * ((double)jj - x0[1] + 0.5) / (x0[1] * x0[1]);
+            if (ic1 > c->icc[1]) { ic1 -= c->icc[1]; w1 = 2; }
+            for (int ii = 0; ii < r->ls[0]; ++ii) {
+                int ic0 = ii + lbeg[0], w0 = 0;
+                double xt0 = ((double)ii - x0[0] + 0.5) * ((double)ii - x0[0] + 0.5) / (x0[0] * x0[0]);
+                if (ic0 > c->icc[0]) { ic0 -= c->icc[0]; w0 = 1; }
+                if (r->ilots == DC_LOT_ELI && !(xt0 + xt1 + xt2 <= 1.0)) continue;
+                ++ncell;
+                const int o = (int)off_base + (w0 | w1 | w2);
+                const long long cell = ((long long)(ic2 - 1) * c->icc[0] * c->icc[1] + (long long)(ic1 - 1) * c->icc[0] + (ic0 - 1));
+                for (int is = 0; is < c->ncatoms; ++is) {
+                    long long ia = cell * c->ncatoms + is;
+                    if (c->h_iscat[ia] == t1) {
+                        idx.push_back((int)ia);
+                        oid.push_back(o);
+                    }
+                }
+                for (long long ir = cr_end; ir < c->natoms; ++ir) {
+                    if (c->h_iscat[ir] != t1) continue;
+                    const int ic[3] = {ic0, ic1, ic2};
+                    bool ok = true;
+                    for (int k = 0; k < 3 && ok; ++k) {
+                        int jcell = ic[k] + (int)std::llround(c->dim_lo[k]) - 1;
+                        ok = (int)c->h_pos[3 * ir + k] == jcell;
+                    }
+                    if (ok) {
+                        idx.push_back((int)ir);
+                        oid.push_back(o);
+                    }
+                }
+            }
+        }
+    }
+    return ncell;
+}
+
+// Run the phase GEMM over gathered segments and leave the slice planes in
+// c->partial.  Positions come from the resident crystal (idx/oid/offs) or,
+// when explicit != null, from an explicit (n,3) list.
+int gemm_segments(dc_ctx *c, const Grid &g, const Plan &pl, GatherList &gl,
+                  const double *explicit_pos, SliceSet &ss, int &nslice, bool use_type_lists) {
+    const GemmGeom &gg = pl.gg;
+    const int mapping = pl.mapping;
+    int st = 0;
+    long long n = 0;
+    for (auto &s : gl.segs) n += s.second;
+    PrepArgs pa = base_prep(g, mapping);
+    pa.n = n;
+    if (explicit_pos) {
+        if ((st = upload(c, c->xat, explicit_pos, 3 * (size_t)std::max<long long>(n, 1)))) return st;
+        pa.src = ptr<double>(c->xat);
+        pa.src_ld = 0;
+    } else {
+        pa.src = ptr<double>(c->pos);
+        if (use_type_lists) {
+            pa.idx = ptr<int>(c->type_idx);
+        } else {
+            if ((st = upload(c, c->idx, gl.idx.data(), gl.idx.size()))) return st;
+            pa.idx = ptr<int>(c->idx);
+            if (!gl.oid.empty()) {
+                if ((st = upload(c, c->oid, gl.oid.data(), gl.oid.size()))) return st;
+                if ((st = upload(c, c->offs, gl.offs.data(), gl.offs.size()))) return st;
+                pa.oid = ptr<int>(c->oid);
+                pa.offs = ptr<double>(c->offs);
+            }
+        }
+    }
+    if ((st = ensure(c, c->rec, sizeof(AtomRec) * std::max<long long>(n, 1)))) return st;
+    HIPCHK(launch_prep(pa, ptr<AtomRec>(c->rec), c->st));
+    int off = 0;
+    ss = SliceSet();
+    add_segments(ss, gl.segs, kchunk_for(pl, n), off);
+    if (ss.beg.empty()) ss.beg.push_back(0);
+    nslice = (int)ss.type.size();
+    if ((st = upload(c, c->slice_beg, ss.beg.data(), ss.beg.size()))) return st;
+    if ((st = upload(c, c->slice_type, ss.type.data(), std::max<size_t>(ss.type.size(), 1)))) return st;
+    if ((st = ensure(c, c->partial, sizeof(float2) * gg.P * std::max(nslice, 1)))) return st;
+    return run_gemm(c, pl, nslice, ptr<float2>(c->partial));
+}
+
+}  // namespace
+
+int dc_four_run_dev(dc_ctx *c, const dc_run_desc *r, double *d_csf, double *d_acsf, double *d_dsi,
+                    double *d_fmean, double *d_dvar) {
+    if (!c) return DC_ERR_ARG;
+    c->err.clear();
+    c->err_typ = DC_ER_NONE;
+    if (!r || !r->cfact || r->ncfact < 1) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "dc_four_run: bad descriptor");
+    if (!c->have_crystal) return set_err(c, DC_ERR_STATE, DC_ER_LIB, "dc_four_run: no crystal (dc_crystal_set)");
+    if (c->natoms < 1) return set_err(c, DC_ERR_NOATOMS, DC_ER_APPL, "No fourier will be calculated");
+    for (int k = 0; k < 3; ++k)
+        if (r->inc[k] < 1) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "dc_four_run: inc < 1");
+    if (r->ilots != DC_LOT_OFF && r->ilots != DC_LOT_BOX && r->ilots != DC_LOT_ELI)
+        return set_err(c, DC_ERR_ARG, DC_ER_LIB, "dc_four_run: bad lot shape");
+    const bool lots = r->ilots != DC_LOT_OFF;
+    if (lots) {
+        if (c->natoms != (int64_t)c->icc[0] * c->icc[1] * c->icc[2] * c->ncatoms)
+            return set_err(c, DC_ERR_LOTS, DC_ER_APPL, "Lots are on but the number of atoms differs from the crystal size");
+        if (r->nlots < 1 || !r->lbeg) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "dc_four_run: lots need lbeg");
+        for (int k = 0; k < 3; ++k)
+            if (r->ls[k] < 1 || r->ls[k] > c->icc[k]) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "dc_four_run: lot size outside crystal");
+    }
+    if (zero_increment(r->vi, r->inc)) return set_err(c, DC_ERR_ZEROVEC, DC_ER_APPL, "Zero increment vector");
+    int csize[3];
+    for (int k = 0; k < 3; ++k) csize[k] = r->lperiod ? c->icc[k] : std::max(1, c->icc[k] - (lots ? r->ls[k] : 1) + 1);
+    if (lots)
+        for (int l = 0; l < r->nlots; ++l)
+            for (int k = 0; k < 3; ++k)
+                if (r->lbeg[3 * l + k] < 1 || r->lbeg[3 * l + k] > csize[k])
+                    return set_err(c, DC_ERR_ARG, DC_ER_LIB, "dc_four_run: lot origin outside crystal");
+    if (r->fave != 0.0 && !r->sel_cell) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "dc_four_run: fave needs sel_cell");
+    if (cancelled(c)) return set_err(c, DC_ERR_CANCEL, DC_ER_COMM, "cancelled");
+    HIPCHK(hipSetDevice(c->device));
+    std::memset(&c->stats, 0, sizeof(c->stats));
+    HIPCHK(hipEventRecord(c->ev[0], c->st));
+
+    Grid g{};
+    int fnum[3];
+    int fried = test_friedel(r->eck, r->vi, r->inc, r->ano, r->four_friedel, fnum);
+    for (int k = 0; k < 3; ++k) {
+        g.n[k] = r->inc[k];
+        g.f[k] = fnum[k];
+        g.xm[k] = r->eck[k];
+        g.uin[k] = r->vi[k];
+        g.vin[k] = r->vi[3 + k];
+        g.win[k] = r->vi[6 + k];
+    }
+    for (int k = 0; k < 9; ++k) g.rten[k] = r->rten[k];
+    g.P = (long long)fnum[0] * fnum[1] * fnum[2];
+    g.friedel = fried != 0;
+    const long long ntot = (long long)g.n[0] * g.n[1] * g.n[2];
+    if (max_corner_bin(g) > std::min(CFPKT, r->ncfact - 1))
+        return set_err(c, DC_ERR_ISTL, DC_ER_FOUR, "sin(theta)/lambda beyond the form factor table");
+
+    const Plan pl = plan_gemm(g, c->tile);
+    int st = 0;
+    if ((st = upload(c, c->cfact, r->cfact, 2 * (size_t)r->ncfact * c->nscat))) return st;
+
+    // compact-region state
+    if ((st = ensure(c, c->acsf, sizeof(double) * 2 * g.P))) return st;
+    if ((st = ensure(c, c->dsi, sizeof(double) * g.P))) return st;
+    if ((st = ensure(c, c->csf, sizeof(double) * 2 * g.P))) return st;
+    const bool want_var = d_fmean || d_dvar;
+    if (want_var) {
+        if ((st = ensure(c, c->wm, sizeof(double) * 2 * g.P))) return st;
+        if ((st = ensure(c, c->wv, sizeof(double) * g.P))) return st;
+        HIPCHK(hipMemsetAsync(c->wm.p, 0, sizeof(double) * 2 * g.P, c->st));
+        HIPCHK(hipMemsetAsync(c->wv.p, 0, sizeof(double) * g.P, c->st));
+    }
+    HIPCHK(hipMemsetAsync(c->acsf.p, 0, sizeof(double) * 2 * g.P, c->st));
+    HIPCHK(hipMemsetAsync(c->dsi.p, 0, sizeof(double) * g.P, c->st));
+    HIPCHK(hipMemsetAsync(c->csf.p, 0, sizeof(double) * 2 * g.P, c->st));
+
+    double pairs = 0.0;
+    long long nslices_total = 0;
+    int warn = 0;
+    // ---- average structure <F> (four_aver, fourier_sup.f90:1194-1357) ----
+    if (r->fave != 0.0) {
+        GatherList gl;
+        // selected cells in four_aver's loop order (ii outer, kk inner, :1250-1252)
+        std::vector<long long> cells;
+        for (int ii = 1; ii <= csize[0]; ++ii)
+            for (int jj = 1; jj <= csize[1]; ++jj)
+                for (int kk = 1; kk <= csize[2]; ++kk) {
+                    if (!r->sel_cell[(ii - 1) + (long long)csize[0] * ((jj - 1) + (long long)csize[1] * (kk - 1))]) continue;
+                    cells.push_back(((long long)(kk - 1) * c->icc[0] * c->icc[1] + (long long)(jj - 1) * c->icc[0] + (ii - 1)));
+                    gl.offs.push_back((double)(ii - 1) + c->dim0[0]);
+                    gl.offs.push_back((double)(jj - 1) + c->dim0[1]);
+                    gl.offs.push_back((double)(kk - 1) + c->dim0[2]);
+                }
+        const long long nsel = (long long)cells.size();
+        for (int t = 1; t <= c->nscat; ++t) {
+            long long cnt = 0;
+            for (long long q = 0; q < nsel; ++q)
+                for (int is = 0; is < c->ncatoms; ++is) {
+                    long long ia = cells[q] * c->ncatoms + is;
+                    if (ia < c->natoms && c->h_iscat[ia] == t) {
+                        gl.idx.push_back((int)ia);
+                        gl.oid.push_back((int)q);
+                        ++cnt;
+                    }
+                }
+            gl.segs.push_back({t - 1, (int)cnt});
+        }
+        SliceSet ss;
+        int nslice = 0;
+        if ((st = gemm_segments(c, g, pl, gl, nullptr, ss, nslice, false))) return st;
+        pairs += (double)gl.idx.size() * (double)ntot;
+        nslices_total += nslice;
+        ReduceArgs ra{};
+        ra.g = g;
+        ra.partial = ptr<float2>(c->partial);
+        std::vector<int> lot_b = {0, nslice};
+        if ((st = upload(c, c->lot_slice, lot_b.data(), 2))) return st;
+        ra.lot_slice = ptr<int>(c->lot_slice);
+        ra.slice_type = ptr<int>(c->slice_type);
+        ra.nlots = 1;
+        ra.lform = 1;
+        ra.cfact = ptr<double2>(c->cfact);
+        ra.ncfact = r->ncfact;
+        HIPCHK(launch_reduce_sum(ra, ptr<double>(c->acsf), 0, c->st));
+        // lattice of the lot shape over the full grid (four_getav :2440-2510)
+        std::vector<double> lp;
+        if (!lots) {
+            for (int ii = 0; ii < c->icc[0]; ++ii)
+                for (int jj = 0; jj < c->icc[1]; ++jj)
+                    for (int kk = 0; kk < c->icc[2]; ++kk) {
+                        lp.push_back(ii + c->dim0[0]);
+                        lp.push_back(jj + c->dim0[1]);
+                        lp.push_back(kk + c->dim0[2]);
+                    }
+        } else {
+            double x0[3];
+            for (int k = 0; k < 3; ++k) x0[k] = (double)(float)r->ls[k] / 2.0;
+            for (int kk = 0; kk < r->ls[2]; ++kk) {
+                double xt2 = ((double)kk - x0[2] + 0.5) * ((double)kk - x0[2] + 0.5) / (x0[2] * x0[2]);
+                for (int jj = 0; jj < r->ls[1]; ++jj) {
+                    double xt1 = ((double)jj - x0[1] + 0.5) * ((double)jj - x0[1] + 0.5) / (x0[1] * x0[1]);
+                    for (int ii = 0; ii < r->ls[0]; ++ii) {
+                        double xt0 = ((double)ii - x0[0] + 0.5) * ((double)ii - x0[0] + 0.5) / (x0[0] * x0[0]);
+                        if (r->ilots == DC_LOT_ELI && !(xt0 + xt1 + xt2 <= 1.0)) continue;
+                        lp.push_back(ii);
+                        lp.push_back(jj);
+                        lp.push_back(kk);
+                    }
+                }
+            }
+        }
+        Grid gl_full = g;
+        for (int k = 0; k < 3; ++k) gl_full.f[k] = g.n[k];
+        gl_full.P = ntot;
+        gl_full.friedel = 0;
+        const Plan pll = plan_gemm(gl_full, c->tile);
+        GatherList gll;
+        gll.segs.push_back({0, (int)(lp.size() / 3)});
+        if ((st = gemm_segments(c, gl_full, pll, gll, lp.data(), ss, nslice, false))) return st;
+        pairs += (double)(lp.size() / 3) * (double)ntot;
+        nslices_total += nslice;
+        if ((st = ensure(c, c->lat, sizeof(double) * 2 * ntot))) return st;
+        ReduceArgs rl{};
+        rl.g = gl_full;
+        rl.partial = ptr<float2>(c->partial);
+        lot_b = {0, nslice};
+        if ((st = upload(c, c->lot_slice, lot_b.data(), 2))) return st;
+        rl.lot_slice = ptr<int>(c->lot_slice);
+        rl.nlots = 1;
+        rl.lform = 0;
+        HIPCHK(launch_reduce_sum(rl, ptr<double>(c->lat), 0, c->st));
+        double norm = nsel > 0 ? 1.0 / (double)nsel : 0.0;
+        if (nsel == 0) warn = DC_WARN_NOCELL;
+        HIPCHK(launch_aver_combine(g, ptr<double>(c->acsf), ptr<double>(c->lat), norm, c->st));
+    }
+
+    // ---- lot loop (fourier_sup.f90:140-257) ----
+    long long ncell = (long long)c->icc[0] * c->icc[1] * c->icc[2];
+    const int nlots = lots ? r->nlots : 1;
+    const double plane_bytes = (double)sizeof(float2) * (double)g.plane;
+    int lot = 0;
+    while (lot < nlots) {
+        if (cancelled(c)) return set_err(c, DC_ERR_CANCEL, DC_ER_COMM, "cancelled");
+        GatherList gl;
+        std::vector<int> lot_first_seg;
+        int l1 = lot;
+        long long n_batch = 0;
+        while (l1 < nlots) {
+            std::vector<std::pair<int, int>> segs_one;
+            std::vector<int> idx_one, oid_one;
+            std::vector<double> offs8;
+            long long nc = ncell;
+            if (!lots) {
+                for (int t = 0; t < c->nscat; ++t) segs_one.push_back({t, c->type_beg[t + 1] - c->type_beg[t]});
+            } else {
+                const int obase = (int)(gl.offs.size() / 3);
+                for (int t = 1; t <= c->nscat; ++t) {
+                    std::vector<int> idx, oid;
+                    std::vector<double> offs;
+                    nc = gather_lot(c, r, l1, t, idx, oid, offs);   // oid in 0..7, 8 offsets
+                    if (t == 1) offs8 = offs;
+                    for (auto &o : oid) o += obase;
+                    idx_one.insert(idx_one.end(), idx.begin(), idx.end());
+                    oid_one.insert(oid_one.end(), oid.begin(), oid.end());
+                    segs_one.push_back({t - 1, (int)idx.size()});
+                }
+            }
+            long long n_one = 0;
+            for (auto &sg : segs_one) n_one += sg.second;
+            const double est_slices = (double)(gl.segs.size() + segs_one.size()) +
+                                      (double)(n_batch + n_one) / (double)KCHUNK_MIN;
+            if (l1 > lot && est_slices * plane_bytes > PARTIAL_BUDGET) break;
+            lot_first_seg.push_back((int)gl.segs.size());
+            gl.offs.insert(gl.offs.end(), offs8.begin(), offs8.end());
+            gl.idx.insert(gl.idx.end(), idx_one.begin(), idx_one.end());
+            gl.oid.insert(gl.oid.end(), oid_one.begin(), oid_one.end());
+            gl.segs.insert(gl.segs.end(), segs_one.begin(), segs_one.end());
+            n_batch += n_one;
+            ncell = nc;
+            ++l1;
+            if (!lots) break;
+        }
+        lot_first_seg.push_back((int)gl.segs.size());
+        SliceSet ss;
+        int nslice = 0;
+        if ((st = gemm_segments(c, g, pl, gl, nullptr, ss, nslice, !lots))) return st;
+        pairs += (double)n_batch * (double)ntot;
+        nslices_total += nslice;
+        ss.seg_first.push_back(nslice);
+        std::vector<int> lot_sl;
+        for (int v : lot_first_seg) lot_sl.push_back(ss.seg_first[v]);
+        if ((st = upload(c, c->lot_slice, lot_sl.data(), lot_sl.size()))) return st;
+        ReduceArgs ra{};
+        ra.g = g;
+        ra.partial = ptr<float2>(c->partial);
+        ra.slice0 = 0;
+        ra.lot_slice = ptr<int>(c->lot_slice);
+        ra.slice_type = ptr<int>(c->slice_type);
+        ra.nlots = l1 - lot;
+        ra.lot0 = lot;
+        ra.lform = 1;
+        ra.cfact = ptr<double2>(c->cfact);
+        ra.ncfact = r->ncfact;
+        ra.acsf = ptr<double>(c->acsf);
+        ra.dsi = ptr<double>(c->dsi);
+        ra.csf = ptr<double>(c->csf);
+        ra.wm = want_var ? ptr<double>(c->wm) : nullptr;
+        ra.wv = want_var ? ptr<double>(c->wv) : nullptr;
+        HIPCHK(hipEventRecord(c->ev[2], c->st));
+        HIPCHK(launch_reduce_lots(ra, c->st));
+        HIPCHK(hipEventRecord(c->ev[3], c->st));
+        HIPCHK(hipEventSynchronize(c->ev[3]));
+        float rms = 0.f;
+        HIPCHK(hipEventElapsedTime(&rms, c->ev[2], c->ev[3]));
+        c->stats.reduce_ms += rms;
+        lot = l1;
+    }
+
+    // ---- normalisation, Friedel mirror, outputs (fourier_sup.f90:256-282) ----
+    const double dnorm = nlots != 1 ? (double)((long long)c->icc[0] * c->icc[1] * c->icc[2]) / (double)(ncell * nlots) : 1.0;
+    FinalArgs fa{};
+    fa.g = g;
+    fa.ntot = ntot;
+    fa.csf = ptr<double>(c->csf);
+    fa.acsf = ptr<double>(c->acsf);
+    fa.dsi = ptr<double>(c->dsi);
+    fa.wm = want_var ? ptr<double>(c->wm) : nullptr;
+    fa.wv = want_var ? ptr<double>(c->wv) : nullptr;
+    fa.csf_out = d_csf;
+    fa.acsf_out = d_acsf;
+    fa.dsi_out = d_dsi;
+    fa.fmean_out = d_fmean;
+    fa.dvar_out = d_dvar;
+    fa.sq = std::sqrt(r->diff_scale);
+    fa.dscale = dnorm * r->diff_scale;
+    fa.vscale = dnorm * r->diff_scale;
+    HIPCHK(launch_finalize(fa, c->st));
+    HIPCHK(hipEventRecord(c->ev[1], c->st));
+    HIPCHK(hipEventSynchronize(c->ev[1]));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    c->stats.total_ms = ms;
+    c->stats.pairs = pairs;
+    c->stats.slices = nslices_total;
+    return warn;
+}
+
+int dc_four_run(dc_ctx *c, const dc_run_desc *r, double *csf, double *acsf, double *dsi, double *fmean,
+                double *dvar) {
+    if (!c || !r) return DC_ERR_ARG;
+    const long long ntot = (long long)std::max(r->inc[0], 1) * std::max(r->inc[1], 1) * std::max(r->inc[2], 1);
+    double *host[5] = {csf, acsf, dsi, fmean, dvar};
+    const int width[5] = {2, 2, 1, 2, 1};
+    double *dev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    int st = 0;
+    for (int k = 0; k < 5; ++k)
+        if (host[k]) {
+            if ((st = ensure(c, c->out[k], sizeof(double) * width[k] * ntot))) return st;
+            dev[k] = ptr<double>(c->out[k]);
+        }
+    st = dc_four_run_dev(c, r, dev[0], dev[1], dev[2], dev[3], dev[4]);
+    if (st < 0) return st;
+    for (int k = 0; k < 5; ++k)
+        if (host[k]) HIPCHK(hipMemcpyAsync(host[k], dev[k], sizeof(double) * width[k] * ntot, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return st;
+}
+
+}  // extern "C"

