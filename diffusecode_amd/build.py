@@ -62,8 +62,13 @@ def build(jobs: int = 4, verbose: bool = False) -> str:
 FORTRAN_SRC = os.path.join(HERE, "fortran", "diffuse_turbo_mod.f90")
 
 
+FORTRAN_EXAMPLE = os.path.join(HERE, "fortran", "example_strucf.f90")
+FORTRAN_EXE = os.path.join(OBJ, "fortran", "example_strucf")
+
+
 def build_fortran() -> str | None:
-    """Compile the ISO_C_BINDING interface module (syntax/interface check)."""
+    """Compile the ISO_C_BINDING interface module and link the example host
+    program against libdiffuse_turbo.so (rpath to the package directory)."""
     fc = shutil.which("flang") or "/opt/rocm/llvm/bin/flang"
     if not os.path.exists(FORTRAN_SRC) or not os.path.exists(fc):
         return None
@@ -71,7 +76,9 @@ def build_fortran() -> str | None:
     os.makedirs(out, exist_ok=True)
     obj = os.path.join(out, "diffuse_turbo_mod.o")
     _run([fc, "-c", "-O2", "-fPIC", "-module-dir", out, FORTRAN_SRC, "-o", obj])
-    return obj
+    _run([fc, "-O2", "-module-dir", out, "-I", out, FORTRAN_EXAMPLE, obj, "-o", FORTRAN_EXE,
+          "-L", HERE, "-ldiffuse_turbo", "-Wl,-rpath,$ORIGIN/../..", "-Wl,-rpath," + HERE])
+    return FORTRAN_EXE
 
 
 if __name__ == "__main__":

@@ -17,12 +17,14 @@
 // tile start.  It splits exactly into
 //     [P(r0) + 16 rh s + k(16 rh) w]  +  [rl s + delta w],  delta in {0, 1},
 // so each element is the product of two table entries, TH[rh] * TL_delta[rl],
-// whose phases are exact integers turned into (cos, sin) by sincos_turn.  Per
-// 16-atom stage a work-group evaluates 16 x (64 + RT/16 + CT/16) sincos and
+// whose phases are exact integers turned into (cos, sin) by sincos_turn (the
+// low tables as short products of exact anchors).  Per 16-atom stage a
+// work-group evaluates 16 x (24 + RT/16 + CT/16) sincos and about
 // (RT + CT) x 16 complex products instead of (RT + CT) x 16 sincos.
 //
 // The complex product is embedded as a real GEMM with the atom dimension
-// interleaved (re, im):  Cr = [Ar Ai].[Br -Bi],  Ci = [Ar Ai].[Bi Br].
+// interleaved (re, im):  Cr = [Ar -Ai].[Br Bi],  Ci = [Ai Ar].[Br Bi]; the
+// consumer forms both A variants from one (re, im) load by sign/half swaps.
 //
 // Schemes (template SCHEME):
 //   SCH_FP32   v_mfma_f32_16x16x4_f32, FP32 operands: the plain drop-in.
@@ -31,9 +33,10 @@
 //   SCH_FP16X2 v_mfma_f32_16x16x32_f16, A split (ah + al), B single half,
 //              products ah*b + al*b.
 //
-// Pipeline: LDS holds two element buffers and two table buffers; iteration s
-// builds the elements of stage s+1 and the tables of stage s+2 while the
-// MFMAs consume stage s, with one barrier per stage.
+// Pipeline: LDS holds two element buffers and two table buffers.  In
+// iteration s the producer waves build the elements of stage s+1 and the
+// tables of stage s+2 while the consumer waves run the MFMAs of stage s; one
+// barrier per stage.
 //
 // Output: partial[slice * P + r + R * c] (float2), one plane per slice
 // (a slice = a contiguous atom range of one type of one lot).
@@ -46,7 +49,6 @@ constexpr int SCH_BF16X3 = 1;
 constexpr int SCH_FP16X2 = 2;
 
 constexpr int KS = 16;            // atoms per LDS stage
-constexpr int LSTR = KS + 4;      // u32 words per row of a packed plane (80 B, conflict-free b128 reads)
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
@@ -72,26 +74,40 @@ __device__ __forceinline__ f32x2 unpack2(uint32_t w, std::integral_constant<int,
 // Round the 64-bit phase to the 32-bit turn fraction used by sincos_turn.
 __device__ __forceinline__ uint32_t phase_top(uint64_t p) { return (uint32_t)((p + 0x80000000ull) >> 32); }
 
+// Work-group layout: NWR x NWC waves, each owning a (16 WR) x (16 WC) block of
+// the (16 WR NWR) x (16 WC NWC) tile.  Every wave both builds a share of the
+// operand tables/elements of the coming stages and runs its MFMAs: on gfx950
+// the VALU work of one wave only partly overlaps another wave's matrix-core
+// work, so splitting waves into producers and consumers does not pay.
 template <int WR_, int WC_, int NWR_, int NWC_>
 struct TileCfg {
     static constexpr int WR = WR_, WC = WC_, NWR = NWR_, NWC = NWC_;
-    static constexpr int RT = 16 * WR * NWR, CT = 16 * WC * NWC, NT = 64 * NWR * NWC;
+    static constexpr int NW = NWR * NWC, NT = 64 * NW, NPT = NT;
+    static constexpr int RT = 16 * WR * NWR, CT = 16 * WC * NWC;
     static constexpr int RH = RT / 16, CH = CT / 16;
-    static constexpr int FSA = RT + 16, FSB = CT + 16;   // FP32 plane strides
 };
+
+// Packed planes hold one u32 (re | im << 16) per (row, atom): 16 words per
+// row, the four 16-byte atom quads of a row XOR-swizzled by bits 2..3 of the
+// row so that 16-byte accesses of the producers (4 atom quads x 4 rows) and
+// of the MFMA operand reads (16 rows, one quad) fall on distinct banks.
+__device__ __forceinline__ int pk_off(int row, int quad) { return row * 16 + 4 * (quad ^ ((row >> 2) & 3)); }
+// FP32 planes: (row, k') with k' = 2*atom + (0 re, 1 im), 36 floats per row.
+constexpr int FST = 36;
 
 template <int SCHEME, class T>
 struct Elems {
-    uint32_t a_hi[T::RT * LSTR], a_lo[T::RT * LSTR];
-    uint32_t b_hi[T::CT * LSTR], b_lo[(SCHEME == SCH_BF16X3 ? T::CT : 1) * LSTR];
+    uint32_t a_hi[T::RT * 16], a_lo[T::RT * 16];
+    uint32_t b_hi[T::CT * 16], b_lo[(SCHEME == SCH_BF16X3 ? T::CT : 1) * 16];
 };
 template <class T>
 struct Elems<SCH_FP32, T> {
-    float a[2 * KS * T::FSA];
-    float b[2 * KS * T::FSB];
+    float a[T::RT * FST];
+    float b[T::CT * FST];
 };
 
-// Per-stage tables, [index][atom] layout.
+// Per-stage tables, rows of KS atoms: TL_d[l] (l < 16, wrap state d) and
+// TH[h] for both sides, laid out al[2][16] | ah[RH] | bl[2][16] | bh[CH].
 template <class T>
 struct Tables {
     float2 al[2][16 * KS], ah[T::RH * KS];
@@ -102,7 +118,7 @@ template <int SCHEME, class T>
 struct Smem {
     Elems<SCHEME, T> e[2];
     Tables<T> t[2];
-    int a_k16[T::RH], a_wpos[T::RH];   // wraps before group rh, position of the wrap inside it
+    int a_k16[T::RH], a_wpos[T::RH];   // wraps before group h, position of the wrap inside it
     int b_k16[T::CH], b_wpos[T::CH];
 };
 
@@ -111,175 +127,204 @@ struct Side {
     int start, a0, hi0, period;
 };
 
-// ---------------------------------------------------------------------------
-// Tables of stage [k0, k0+KS): thread t owns atom t % KS; table rows are
-// enumerated as [A lo0 (16), A lo1 (16), A hi (RH), B lo0 (16), B lo1 (16), B hi (CH)].
-template <class T>
-__device__ __forceinline__ void build_tables(Tables<T> &tb, const int *__restrict__ a_k16,
-                                             const int *__restrict__ b_k16, const AtomRec *__restrict__ rec,
-                                             int k0, int kend, const Side &sa, const Side &sb) {
-    constexpr int NROW = 64 + T::RH + T::CH;
-    const int a = threadIdx.x & (KS - 1);
-    const int k = k0 + a;
-    const bool live = k < kend;
-    const AtomRec ar = rec[live ? k : kend - 1];
-    const float w = live ? ar.w : 0.0f;
-    const uint64_t wa = ar.ra1 - (uint64_t)sa.period * ar.ra;
-    const uint64_t wb = ar.ca1 - (uint64_t)sb.period * ar.ca;
-    const uint64_t pa = (uint64_t)sa.a0 * ar.ra + (uint64_t)sa.hi0 * ar.ra1;
-    const uint64_t pb = ar.p0 + (uint64_t)sb.a0 * ar.ca + (uint64_t)sb.hi0 * ar.ca1;
-#pragma unroll
-    for (int q0 = 0; q0 < NROW; q0 += T::NT / KS) {
-        const int q = q0 + (int)(threadIdx.x >> 4);
-        if (q >= NROW) break;
-        uint64_t ph;
-        float amp = 1.0f;
-        float2 *dst;
-        if (q < 32) {
-            const int d = q >> 4, l = q & 15;
-            ph = (uint64_t)l * ar.ra + (d ? wa : 0ull);
-            dst = &tb.al[d][l * KS + a];
-        } else if (q < 32 + T::RH) {
-            const int h = q - 32;
-            ph = pa + (uint64_t)(16 * h) * ar.ra + (uint64_t)a_k16[h] * wa;
-            dst = &tb.ah[h * KS + a];
-        } else if (q < 64 + T::RH) {
-            const int qq = q - 32 - T::RH;
-            const int d = qq >> 4, l = qq & 15;
-            ph = (uint64_t)l * ar.ca + (d ? wb : 0ull);
-            dst = &tb.bl[d][l * KS + a];
-        } else {
-            const int h = q - 64 - T::RH;
-            ph = pb + (uint64_t)(16 * h) * ar.ca + (uint64_t)b_k16[h] * wb;
-            amp = w;
-            dst = &tb.bh[h * KS + a];
-        }
-        float c, s;
-        sincos_turn(phase_top(ph), c, s);
-        *dst = make_float2(c * amp, s * amp);
-    }
-}
-
 __device__ __forceinline__ float2 cmul(float2 x, float2 y) {
     return make_float2(__builtin_fmaf(x.x, y.x, -x.y * y.y), __builtin_fmaf(x.x, y.y, x.y * y.x));
 }
+__device__ __forceinline__ float2 cis(uint64_t ph) {
+    float c, s;
+    sincos_turn(phase_top(ph), c, s);
+    return make_float2(c, s);
+}
 
-// Elements of one stage from its tables.  Thread t owns atom t % KS.
-template <int SCHEME, class T>
-__device__ __forceinline__ void build_elems(Elems<SCHEME, T> &el, const Tables<T> &tb, const int *__restrict__ a_wpos,
-                                            const int *__restrict__ b_wpos) {
-    const int a = threadIdx.x & (KS - 1);
-    const int g = threadIdx.x >> 4;
-    constexpr int NG = T::NT / KS;
+// ---------------------------------------------------------------------------
+// Tables of one stage, built by all threads (pt = thread index).  Each atom
+// a = pt % 16 has NPT/16 helper threads r = pt / 16:
+//   r < 8      low tables of side r/4, entries l = 4m .. 4m+3 (m = r % 4):
+//              TL0[4m+j] = cis(4m s) cis(s)^j, TL1 = TL0 * cis(w)
+//              (3 exact anchors + 7 products instead of 8 sincos)
+//   r >= 8     high-table rows h = r-8, r-8+NPT/16-8, ... of [A (RH) | B (CH)]
+template <class T>
+__device__ __forceinline__ void build_tables(Tables<T> &tb, const int *__restrict__ a_k16,
+                                             const int *__restrict__ b_k16, const AtomRec &ar, bool live,
+                                             const Side &sa, const Side &sb, int pt) {
+    const int a = pt & (KS - 1), r = pt >> 4;
+    constexpr int NR = T::NPT / KS;           // helper threads per atom
+    if (r < 8) {
+        const bool sideb = r >= 4;
+        const int m = r & 3;
+        const uint64_t step = sideb ? ar.ca : ar.ra;
+        const uint64_t wrp = sideb ? ar.ca1 - (uint64_t)sb.period * ar.ca : ar.ra1 - (uint64_t)sa.period * ar.ra;
+        const float2 e4 = cis((uint64_t)(4 * m) * step);
+        const float2 e1 = cis(step);
+        const float2 ew = cis(wrp);
+        float2 *t0 = sideb ? tb.bl[0] : tb.al[0];
+        float2 *t1 = sideb ? tb.bl[1] : tb.al[1];
+        float2 v = e4;
 #pragma unroll
-    for (int rho = g; rho < T::RT; rho += NG) {
-        const int h = rho >> 4, l = rho & 15;
-        const float2 v = cmul(tb.ah[h * KS + a], tb.al[l >= a_wpos[h]][l * KS + a]);
-        if constexpr (SCHEME == SCH_FP32) {
-            el.a[(2 * a) * T::FSA + rho] = v.x;
-            el.a[(2 * a + 1) * T::FSA + rho] = v.y;
-        } else {
-            constexpr auto S = std::integral_constant<int, SCHEME>{};
-            const uint32_t hi = pack2(v.x, v.y, S);
-            const f32x2 hv = unpack2(hi, S);
-            el.a_hi[rho * LSTR + a] = hi;
-            el.a_lo[rho * LSTR + a] = pack2(v.x - hv.x, v.y - hv.y, S);
+        for (int j = 0; j < 4; ++j) {
+            t0[(4 * m + j) * KS + a] = v;
+            t1[(4 * m + j) * KS + a] = cmul(v, ew);
+            v = cmul(v, e1);
         }
-    }
-#pragma unroll
-    for (int gam = g; gam < T::CT; gam += NG) {
-        const int h = gam >> 4, l = gam & 15;
-        const float2 v = cmul(tb.bh[h * KS + a], tb.bl[l >= b_wpos[h]][l * KS + a]);
-        if constexpr (SCHEME == SCH_FP32) {
-            el.b[(2 * a) * T::FSB + gam] = v.x;
-            el.b[(2 * a + 1) * T::FSB + gam] = v.y;
-        } else if constexpr (SCHEME == SCH_BF16X3) {
-            constexpr auto S = std::integral_constant<int, SCHEME>{};
-            const uint32_t hi = pack2(v.x, v.y, S);
-            const f32x2 hv = unpack2(hi, S);
-            el.b_hi[gam * LSTR + a] = hi;
-            el.b_lo[gam * LSTR + a] = pack2(v.x - hv.x, v.y - hv.y, S);
-        } else {
-            el.b_hi[gam * LSTR + a] = pack2(v.x, v.y, std::integral_constant<int, SCH_FP16X2>{});
+    } else {
+        const float w = live ? ar.w : 0.0f;
+        const uint64_t wa = ar.ra1 - (uint64_t)sa.period * ar.ra;
+        const uint64_t wb = ar.ca1 - (uint64_t)sb.period * ar.ca;
+        const uint64_t pa = (uint64_t)sa.a0 * ar.ra + (uint64_t)sa.hi0 * ar.ra1;
+        const uint64_t pb = ar.p0 + (uint64_t)sb.a0 * ar.ca + (uint64_t)sb.hi0 * ar.ca1;
+        for (int q = r - 8; q < T::RH + T::CH; q += NR - 8) {
+            const bool sideb = q >= T::RH;
+            const int h = sideb ? q - T::RH : q;
+            const uint64_t step = sideb ? ar.ca : ar.ra;
+            const uint64_t wrp = sideb ? wb : wa;
+            const int k16 = sideb ? b_k16[h] : a_k16[h];
+            const uint64_t ph = (sideb ? pb : pa) + (uint64_t)(16 * h) * step + (uint64_t)(uint32_t)k16 * wrp;
+            const float amp = sideb ? w : 1.0f;
+            const float2 v = cis(ph);
+            (sideb ? tb.bh : tb.ah)[h * KS + a] = make_float2(v.x * amp, v.y * amp);
         }
     }
 }
 
-// MFMAs of one stage for this wave's (WR x WC) 16x16 tiles; dead tiles skipped.
+// Load the 4 table entries of atoms 4g..4g+3 (32 bytes).
+__device__ __forceinline__ void ld4(const float2 *p, float2 (&v)[4]) {
+    const f32x4 x = *reinterpret_cast<const f32x4 *>(p);
+    const f32x4 y = *reinterpret_cast<const f32x4 *>(p + 2);
+    v[0] = make_float2(x[0], x[1]);
+    v[1] = make_float2(x[2], x[3]);
+    v[2] = make_float2(y[0], y[1]);
+    v[3] = make_float2(y[2], y[3]);
+}
+
+// One 16-row group h of side A (isA) or B: lane (l = row low bits, g = atom
+// quad) forms the 4 elements TH[h] * TL_d[l] and stores them.
+template <int SCHEME, class T, bool isA>
+__device__ __forceinline__ void elems_group(Elems<SCHEME, T> &el, const float2 *__restrict__ th, const float2 (&tl0)[4],
+                                            const float2 (&tl1)[4], int h, int wpos, int l, int g) {
+    float2 hv[4];
+    ld4(th + h * KS + 4 * g, hv);
+    const bool d = l >= wpos;
+    float2 v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = cmul(hv[q], d ? tl1[q] : tl0[q]);
+    const int row = 16 * h + l;
+    if constexpr (SCHEME == SCH_FP32) {
+        float *dst = (isA ? el.a : el.b) + row * FST + 8 * g;
+        *reinterpret_cast<f32x4 *>(dst) = f32x4{v[0].x, v[0].y, v[1].x, v[1].y};
+        *reinterpret_cast<f32x4 *>(dst + 4) = f32x4{v[2].x, v[2].y, v[3].x, v[3].y};
+    } else {
+        constexpr auto S = std::integral_constant<int, SCHEME>{};
+        u32x4 hi, lo;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            hi[q] = pack2(v[q].x, v[q].y, S);
+            const f32x2 hvv = unpack2(hi[q], S);
+            lo[q] = pack2(v[q].x - hvv.x, v[q].y - hvv.y, S);
+        }
+        const int o = pk_off(row, g);
+        if constexpr (isA) {
+            *reinterpret_cast<u32x4 *>(&el.a_hi[o]) = hi;
+            *reinterpret_cast<u32x4 *>(&el.a_lo[o]) = lo;
+        } else {
+            *reinterpret_cast<u32x4 *>(&el.b_hi[o]) = hi;
+            if constexpr (SCHEME == SCH_BF16X3) *reinterpret_cast<u32x4 *>(&el.b_lo[o]) = lo;
+        }
+    }
+}
+
+// Elements of one stage, built by wave pw: its contiguous run of the RH + CH
+// 16-row groups.  The low tables of a side are loaded once.
+template <int SCHEME, class T>
+__device__ __forceinline__ void build_elems(Elems<SCHEME, T> &el, const Tables<T> &tb, const int *__restrict__ a_wpos,
+                                            const int *__restrict__ b_wpos, int pw) {
+    constexpr int NJOB = T::RH + T::CH;
+    const int lane = threadIdx.x & 63, g = lane & 3, l = lane >> 2;
+    const int j0 = (pw * NJOB) / T::NW, j1 = ((pw + 1) * NJOB) / T::NW;
+    float2 tl0[4], tl1[4];
+    if (j0 < T::RH) {
+        ld4(&tb.al[0][l * KS + 4 * g], tl0);
+        ld4(&tb.al[1][l * KS + 4 * g], tl1);
+        const int je = j1 < T::RH ? j1 : T::RH;
+        for (int j = j0; j < je; ++j) elems_group<SCHEME, T, true>(el, tb.ah, tl0, tl1, j, a_wpos[j], l, g);
+    }
+    if (j1 > T::RH) {
+        ld4(&tb.bl[0][l * KS + 4 * g], tl0);
+        ld4(&tb.bl[1][l * KS + 4 * g], tl1);
+        for (int j = (j0 > T::RH ? j0 : T::RH); j < j1; ++j)
+            elems_group<SCHEME, T, false>(el, tb.bh, tl0, tl1, j - T::RH, b_wpos[j - T::RH], l, g);
+    }
+}
+
+// MFMAs of one stage for this wave's (WR x WC) 16x16 tiles (waves whose
+// whole block lies outside the grid skip the stage).  A is read once as
+// (re, im) and permuted in registers into the two forms of the real
+// embedding: (re, -im) for Cr and (im, re) for Ci.
 template <int SCHEME, class T>
 __device__ __forceinline__ void mfma_stage(const Elems<SCHEME, T> &el, f32x4 (&accR)[T::WR][T::WC],
-                                           f32x4 (&accI)[T::WR][T::WC], int wr0, int wc0, int lrows, int lcols) {
+                                           f32x4 (&accI)[T::WR][T::WC], int wr0, int wc0) {
     const int lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
     if constexpr (SCHEME == SCH_FP32) {
 #pragma unroll
         for (int ks = 0; ks < KS / 2; ++ks) {
             float av[T::WR], bR[T::WC], bI[T::WC];
 #pragma unroll
-            for (int i = 0; i < T::WR; ++i) av[i] = el.a[(4 * ks + lg) * T::FSA + wr0 + 16 * i + li];
+            for (int i = 0; i < T::WR; ++i) av[i] = el.a[(wr0 + 16 * i + li) * FST + 4 * ks + lg];
             const int atom = 2 * ks + (lane >> 5);
             const bool im = (lg & 1) != 0;
 #pragma unroll
             for (int j = 0; j < T::WC; ++j) {
-                const float br = el.b[(2 * atom) * T::FSB + wc0 + 16 * j + li];
-                const float bi = el.b[(2 * atom + 1) * T::FSB + wc0 + 16 * j + li];
-                bR[j] = im ? -bi : br;
-                bI[j] = im ? br : bi;
+                const float2 b = *reinterpret_cast<const float2 *>(&el.b[(wc0 + 16 * j + li) * FST + 2 * atom]);
+                bR[j] = im ? -b.y : b.x;
+                bI[j] = im ? b.x : b.y;
             }
 #pragma unroll
-            for (int j = 0; j < T::WC; ++j) {
-                if (16 * j >= lcols) continue;
+            for (int j = 0; j < T::WC; ++j)
 #pragma unroll
                 for (int i = 0; i < T::WR; ++i) {
-                    if (16 * i >= lrows) continue;
                     accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i], bR[j], accR[i][j], 0, 0, 0);
                     accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i], bI[j], accI[i][j], 0, 0, 0);
                 }
-            }
         }
     } else {
-        u32x4 ah[T::WR], al[T::WR];
+        // operands of the whole stage in registers, then each split term sweeps
+        // all WR x WC tiles so consecutive MFMAs never share an accumulator
+        u32x4 arh[T::WR], arl[T::WR], aih[T::WR], ail[T::WR], bh[T::WC], bl[T::WC];
 #pragma unroll
         for (int i = 0; i < T::WR; ++i) {
-            const int o = (wr0 + 16 * i + li) * LSTR + 4 * lg;
-            ah[i] = *reinterpret_cast<const u32x4 *>(&el.a_hi[o]);
-            al[i] = *reinterpret_cast<const u32x4 *>(&el.a_lo[o]);
+            const int o = pk_off(wr0 + 16 * i + li, lg);
+            const u32x4 h = *reinterpret_cast<const u32x4 *>(&el.a_hi[o]);
+            const u32x4 l = *reinterpret_cast<const u32x4 *>(&el.a_lo[o]);
+            arh[i] = h ^ 0x80000000u;
+            aih[i] = (h >> 16) | (h << 16);
+            arl[i] = l ^ 0x80000000u;
+            ail[i] = (l >> 16) | (l << 16);
         }
 #pragma unroll
         for (int j = 0; j < T::WC; ++j) {
-            if (16 * j >= lcols) continue;
-            const int o = (wc0 + 16 * j + li) * LSTR + 4 * lg;
-            const u32x4 bh = *reinterpret_cast<const u32x4 *>(&el.b_hi[o]);
-            // B'_r = (re, -im) ; B'_i = (im, re)
-            const u32x4 brh = bh ^ 0x80000000u;
-            const u32x4 bih = (bh >> 16) | (bh << 16);
-            if constexpr (SCHEME == SCH_BF16X3) {
-                const u32x4 bl = *reinterpret_cast<const u32x4 *>(&el.b_lo[o]);
-                const u32x4 brl = bl ^ 0x80000000u;
-                const u32x4 bil = (bl >> 16) | (bl << 16);
+            const int o = pk_off(wc0 + 16 * j + li, lg);
+            bh[j] = *reinterpret_cast<const u32x4 *>(&el.b_hi[o]);
+            if constexpr (SCHEME == SCH_BF16X3) bl[j] = *reinterpret_cast<const u32x4 *>(&el.b_lo[o]);
+        }
+        constexpr int NTERM = SCHEME == SCH_BF16X3 ? 3 : 2;
+#pragma unroll
+        for (int t = 0; t < NTERM; ++t) {
+#pragma unroll
+            for (int j = 0; j < T::WC; ++j)
 #pragma unroll
                 for (int i = 0; i < T::WR; ++i) {
-                    if (16 * i >= lrows) continue;
-                    const bf16x8 a_h = __builtin_bit_cast(bf16x8, ah[i]);
-                    const bf16x8 a_l = __builtin_bit_cast(bf16x8, al[i]);
-                    accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, __builtin_bit_cast(bf16x8, brh), accR[i][j], 0, 0, 0);
-                    accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, __builtin_bit_cast(bf16x8, bih), accI[i][j], 0, 0, 0);
-                    accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, __builtin_bit_cast(bf16x8, brl), accR[i][j], 0, 0, 0);
-                    accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, __builtin_bit_cast(bf16x8, bil), accI[i][j], 0, 0, 0);
-                    accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_l, __builtin_bit_cast(bf16x8, brh), accR[i][j], 0, 0, 0);
-                    accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_l, __builtin_bit_cast(bf16x8, bih), accI[i][j], 0, 0, 0);
+                    // terms: (a_hi, b_hi), (a_hi, b_lo) [BF16X3 only], (a_lo, b_hi)
+                    const bool alo = (t == NTERM - 1);
+                    const u32x4 xr = alo ? arl[i] : arh[i];
+                    const u32x4 xi = alo ? ail[i] : aih[i];
+                    const u32x4 y = (SCHEME == SCH_BF16X3 && t == 1) ? bl[j] : bh[j];
+                    if constexpr (SCHEME == SCH_BF16X3) {
+                        accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, xr), __builtin_bit_cast(bf16x8, y), accR[i][j], 0, 0, 0);
+                        accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, xi), __builtin_bit_cast(bf16x8, y), accI[i][j], 0, 0, 0);
+                    } else {
+                        accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, xr), __builtin_bit_cast(f16x8, y), accR[i][j], 0, 0, 0);
+                        accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, xi), __builtin_bit_cast(f16x8, y), accI[i][j], 0, 0, 0);
+                    }
                 }
-            } else {
-#pragma unroll
-                for (int i = 0; i < T::WR; ++i) {
-                    if (16 * i >= lrows) continue;
-                    const f16x8 a_h = __builtin_bit_cast(f16x8, ah[i]);
-                    const f16x8 a_l = __builtin_bit_cast(f16x8, al[i]);
-                    accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_h, __builtin_bit_cast(f16x8, brh), accR[i][j], 0, 0, 0);
-                    accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_h, __builtin_bit_cast(f16x8, bih), accI[i][j], 0, 0, 0);
-                    accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_l, __builtin_bit_cast(f16x8, brh), accR[i][j], 0, 0, 0);
-                    accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_l, __builtin_bit_cast(f16x8, bih), accI[i][j], 0, 0, 0);
-                }
-            }
         }
     }
 }
@@ -294,20 +339,19 @@ phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice
     const int row0 = blockIdx.y * T::RT, col0 = blockIdx.x * T::CT;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wr0 = (wave % T::NWR) * 16 * T::WR, wc0 = (wave / T::NWR) * 16 * T::WC;
-    const int lrows = g.R - row0 - wr0, lcols = g.C - col0 - wc0;   // live extent of this wave's tiles
+    const bool live = g.R - row0 - wr0 > 0 && g.C - col0 - wc0 > 0;   // this wave's block touches the grid
+    const int pt = threadIdx.x;
 
     Side sa{row0, row0 % g.R0, row0 / g.R0, g.R0};
     Side sb{col0, col0 % g.C0, col0 / g.C0, g.C0};
-    // wrap bookkeeping of the 16-groups of the tile (periods >= 16 checked by the host)
+    // wrap bookkeeping of the 16-groups of the tile (periods >= 16 when a side wraps)
     for (int h = threadIdx.x; h < T::RH + T::CH; h += T::NT) {
         const bool isa = h < T::RH;
         const Side &s = isa ? sa : sb;
         const int hh = isa ? h : h - T::RH;
         const int x = s.a0 + 16 * hh;
-        const int k16 = x / s.period;
-        const int wpos = s.period - (x % s.period);   // first wrap inside the group at offset wpos
-        (isa ? sm.a_k16 : sm.b_k16)[hh] = k16;
-        (isa ? sm.a_wpos : sm.b_wpos)[hh] = wpos;
+        (isa ? sm.a_k16 : sm.b_k16)[hh] = x / s.period;
+        (isa ? sm.a_wpos : sm.b_wpos)[hh] = s.period - (x % s.period);   // first wrap inside the group
     }
 
     f32x4 accR[T::WR][T::WC], accI[T::WR][T::WC];
@@ -320,17 +364,29 @@ phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice
         }
 
     const int nst = (kend - kbeg + KS - 1) / KS;
+    // records of this thread's atom for the stages whose tables come next,
+    // loaded one stage ahead so the global-memory latency is hidden
+    const int at = pt & (KS - 1);
+    auto fetch = [&](int st) {
+        const int k = kbeg + st * KS + at;
+        return rec[k < kend ? k : kend - 1];
+    };
+    const AtomRec r0 = fetch(0);
+    const AtomRec r1 = fetch(nst > 1 ? 1 : 0);
+    AtomRec rn = fetch(nst > 2 ? 2 : 0);
     __syncthreads();
-    build_tables<T>(sm.t[0], sm.a_k16, sm.b_k16, rec, kbeg, kend, sa, sb);
+    build_tables<T>(sm.t[0], sm.a_k16, sm.b_k16, r0, kbeg + at < kend, sa, sb, pt);
     __syncthreads();
-    build_elems<SCHEME, T>(sm.e[0], sm.t[0], sm.a_wpos, sm.b_wpos);
-    if (nst > 1) build_tables<T>(sm.t[1], sm.a_k16, sm.b_k16, rec, kbeg + KS, kend, sa, sb);
+    build_elems<SCHEME, T>(sm.e[0], sm.t[0], sm.a_wpos, sm.b_wpos, wave);
+    if (nst > 1) build_tables<T>(sm.t[1], sm.a_k16, sm.b_k16, r1, kbeg + KS + at < kend, sa, sb, pt);
     __syncthreads();
     for (int s = 0; s < nst; ++s) {
         const int cur = s & 1, nxt = cur ^ 1;
-        if (s + 1 < nst) build_elems<SCHEME, T>(sm.e[nxt], sm.t[nxt], sm.a_wpos, sm.b_wpos);
-        if (s + 2 < nst) build_tables<T>(sm.t[cur], sm.a_k16, sm.b_k16, rec, kbeg + (s + 2) * KS, kend, sa, sb);
-        if (lrows > 0 && lcols > 0) mfma_stage<SCHEME, T>(sm.e[cur], accR, accI, wr0, wc0, lrows, lcols);
+        const AtomRec rt = rn;
+        if (s + 3 < nst) rn = fetch(s + 3);
+        if (s + 1 < nst) build_elems<SCHEME, T>(sm.e[nxt], sm.t[nxt], sm.a_wpos, sm.b_wpos, wave);
+        if (s + 2 < nst) build_tables<T>(sm.t[cur], sm.a_k16, sm.b_k16, rt, kbeg + (s + 2) * KS + at < kend, sa, sb, pt);
+        if (live) mfma_stage<SCHEME, T>(sm.e[cur], accR, accI, wr0, wc0);
         __syncthreads();
     }
 
@@ -354,11 +410,11 @@ phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice
 // ---------------------------------------------------------------------------
 // Tile configurations (WR x WC MFMA tiles per wave, NWR x NWC waves).
 using CfgA = TileCfg<4, 4, 2, 4>;   // 128 x 256, 8 waves
-using CfgB = TileCfg<4, 4, 4, 2>;   // 256 x 128, 8 waves
-using CfgC = TileCfg<3, 4, 2, 4>;   //  96 x 256, 8 waves
+using CfgB = TileCfg<3, 4, 2, 4>;   //  96 x 256
+using CfgC = TileCfg<2, 4, 2, 4>;   //  64 x 256
 using CfgD = TileCfg<4, 4, 2, 2>;   // 128 x 128, 4 waves
-using CfgE = TileCfg<2, 4, 2, 4>;   //  64 x 256, 8 waves
-using CfgF = TileCfg<1, 4, 1, 4>;   //  16 x 256, 4 waves (lines / few rows)
+using CfgE = TileCfg<4, 2, 2, 4>;   // 128 x 128, 8 waves
+using CfgF = TileCfg<1, 4, 1, 4>;   //  16 x 256 (lines / few rows)
 
 static const TileInfo kTiles[] = {
     {CfgA::RT, CfgA::CT, CfgA::NT}, {CfgB::RT, CfgB::CT, CfgB::NT}, {CfgC::RT, CfgC::CT, CfgC::NT},

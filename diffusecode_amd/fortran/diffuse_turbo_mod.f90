@@ -1,0 +1,173 @@
+!*******************************************************************************
+! diffuse_turbo_mod.f90 -- ISO_C_BINDING interface of libdiffuse_turbo.so
+! (include/diffuse_turbo.h) for a Fortran host such as DISCUS.
+!
+! Arrays are passed exactly as DISCUS's diffuse_mod holds them
+! (discus/prog/diffuse_mod.f90:52-78): complex(8) tcsf/csf/acsf(num(1),num(2),
+! num(3)), real(8) dsi(...), integer istl(...), real(8) xat(NMAX,3),
+! complex(8) cfact(0:CFPKT,nscat).
+!
+! dc_four_strucf is the per-type drop-in: it has the arguments of the
+! four_strucf interface (discus/prog/four_strucf_OMP_mod.f90:19) plus the
+! module arrays that four_strucf reads implicitly, and maps the library
+! status onto DISCUS's ier_num / ier_typ.
+!*******************************************************************************
+module diffuse_turbo_mod
+   use iso_c_binding
+   implicit none
+   private
+
+   integer(c_int), parameter, public :: DC_OK = 0
+   integer(c_int), parameter, public :: DC_PREC_FP32 = 0, DC_PREC_BF16X3 = 1, DC_PREC_FP16X2 = 2
+   integer(c_int), parameter, public :: DC_LOT_OFF = 1, DC_LOT_BOX = 2, DC_LOT_ELI = 3
+
+   ! dc_run_desc of include/diffuse_turbo.h
+   type, bind(C), public :: dc_run_desc
+      real(c_double)     :: eck(12)
+      real(c_double)     :: vi(9)
+      integer(c_int)     :: inc(3)
+      real(c_double)     :: rten(9)
+      type(c_ptr)        :: cfact
+      integer(c_int)     :: ncfact
+      integer(c_int)     :: ilots
+      integer(c_int)     :: ls(3)
+      integer(c_int)     :: nlots
+      type(c_ptr)        :: lbeg
+      integer(c_int)     :: lperiod
+      real(c_double)     :: fave
+      type(c_ptr)        :: sel_cell
+      integer(c_int)     :: four_friedel
+      integer(c_int)     :: ano
+      real(c_double)     :: diff_scale
+   end type dc_run_desc
+
+   type, bind(C), public :: dc_crystal_desc
+      type(c_ptr)        :: cr_pos
+      type(c_ptr)        :: cr_iscat
+      integer(c_int64_t) :: natoms
+      integer(c_int)     :: icc(3)
+      integer(c_int)     :: ncatoms
+      integer(c_int)     :: nscat
+      real(c_double)     :: dim0(3)
+      real(c_double)     :: dim_lo(3)
+   end type dc_crystal_desc
+
+   interface
+      integer(c_int) function dc_create(ctx, device) bind(C, name='dc_create')
+         import :: c_ptr, c_int
+         type(c_ptr), intent(out) :: ctx
+         integer(c_int), value :: device
+      end function dc_create
+
+      subroutine dc_destroy(ctx) bind(C, name='dc_destroy')
+         import :: c_ptr
+         type(c_ptr), value :: ctx
+      end subroutine dc_destroy
+
+      integer(c_int) function dc_set_precision(ctx, prec) bind(C, name='dc_set_precision')
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: prec
+      end function dc_set_precision
+
+      integer(c_int) function dc_last_error_type(ctx) bind(C, name='dc_last_error_type')
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx
+      end function dc_last_error_type
+
+      integer(c_int) function dc_strucf(ctx, xat, nxat, ld_xat, uin, vin, win, off_shift, lmn, num, fnum, &
+                                        cfact_col, ncfact, istl, xm, rten, lform, tcsf) bind(C, name='dc_strucf')
+         import :: c_ptr, c_int, c_int64_t, c_double, c_double_complex
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: xat(*)
+         integer(c_int64_t), value :: nxat, ld_xat
+         real(c_double), intent(in) :: uin(3), vin(3), win(3), off_shift(9)
+         integer(c_int), intent(in) :: lmn(6), num(3), fnum(3)
+         complex(c_double_complex), intent(in) :: cfact_col(*)
+         integer(c_int64_t), value :: ncfact
+         integer(c_int), intent(in) :: istl(*)
+         real(c_double), intent(in) :: xm(3), rten(9)
+         integer(c_int), value :: lform
+         complex(c_double_complex), intent(out) :: tcsf(*)
+      end function dc_strucf
+
+      integer(c_int) function dc_crystal_set(ctx, cr) bind(C, name='dc_crystal_set')
+         import :: c_ptr, c_int, dc_crystal_desc
+         type(c_ptr), value :: ctx
+         type(dc_crystal_desc), intent(in) :: cr
+      end function dc_crystal_set
+
+      integer(c_int) function dc_four_run(ctx, run, csf, acsf, dsi, fmean, dvar) bind(C, name='dc_four_run')
+         import :: c_ptr, c_int, dc_run_desc
+         type(c_ptr), value :: ctx
+         type(dc_run_desc), intent(in) :: run
+         type(c_ptr), value :: csf, acsf, dsi, fmean, dvar
+      end function dc_four_run
+   end interface
+
+   public :: dc_create, dc_destroy, dc_set_precision, dc_last_error_type, dc_strucf
+   public :: dc_crystal_set, dc_four_run
+   public :: dc_four_strucf, dc_shutdown
+
+   type(c_ptr), save :: the_ctx = c_null_ptr
+
+contains
+
+   !----------------------------------------------------------------------------
+   ! Per-type structure factor, the isotropic branch of four_strucf
+   ! (four_strucf_OMP_mod.f90:19-68 with tcsf_form_iso, fourier_form.f90:147-177).
+   ! tcsf is overwritten over the whole num grid (zero outside fnum).
+   subroutine dc_four_strucf(iscat, lform, fnum, xat, nxat, uin, vin, win, off_shift, lmn, num, &
+                             cfact, istl, tcsf, ier_num, ier_typ, precision)
+      integer, intent(in) :: iscat
+      logical, intent(in) :: lform
+      integer, dimension(3), intent(in) :: fnum
+      real(c_double), dimension(:, :), intent(in) :: xat          ! xat(NMAX,3)
+      integer, intent(in) :: nxat
+      real(c_double), dimension(3), intent(in) :: uin, vin, win
+      real(c_double), dimension(3, 3), intent(in) :: off_shift
+      integer, dimension(6), intent(in) :: lmn
+      integer, dimension(3), intent(in) :: num
+      complex(c_double_complex), dimension(0:, :), intent(in) :: cfact   ! cfact(0:CFPKT, nscat)
+      integer, dimension(:, :, :), intent(in) :: istl
+      complex(c_double_complex), dimension(:, :, :), intent(inout) :: tcsf
+      integer, intent(inout) :: ier_num, ier_typ
+      integer, intent(in), optional :: precision
+
+      integer(c_int) :: st, prec
+      integer(c_int) :: c_lmn(6), c_num(3), c_fnum(3)
+      real(c_double) :: xm_dummy(3), rten_dummy(9)
+      integer(c_int64_t) :: ncf
+
+      if (.not. c_associated(the_ctx)) then
+         st = dc_create(the_ctx, -1_c_int)
+         if (st /= DC_OK) then
+            ier_num = st
+            ier_typ = 99
+            return
+         end if
+      end if
+      prec = DC_PREC_FP32
+      if (present(precision)) prec = int(precision, c_int)
+      st = dc_set_precision(the_ctx, prec)
+      c_lmn = int(lmn, c_int)
+      c_num = int(num, c_int)
+      c_fnum = int(fnum, c_int)
+      xm_dummy = 0.0_c_double
+      rten_dummy = 0.0_c_double
+      ncf = int(size(cfact, 1), c_int64_t)
+      st = dc_strucf(the_ctx, xat, int(nxat, c_int64_t), int(size(xat, 1), c_int64_t), uin, vin, win, &
+                     reshape(off_shift, [9]), c_lmn, c_num, c_fnum, cfact(:, max(iscat, 1)), ncf, istl, &
+                     xm_dummy, rten_dummy, merge(1_c_int, 0_c_int, lform), tcsf)
+      if (st < 0) then
+         ier_num = st
+         ier_typ = dc_last_error_type(the_ctx)
+      end if
+   end subroutine dc_four_strucf
+
+   subroutine dc_shutdown()
+      if (c_associated(the_ctx)) call dc_destroy(the_ctx)
+      the_ctx = c_null_ptr
+   end subroutine dc_shutdown
+
+end module diffuse_turbo_mod

@@ -1,0 +1,75 @@
+"""C-ABI checks that need no GPU: the library loads, exports every entry
+point declared in include/diffuse_turbo.h, and its argument checks return
+the documented status codes without touching a device."""
+import ctypes as C
+import os
+import re
+import shutil
+import subprocess
+
+import pytest
+
+from diffusecode_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "diffuse_turbo.h")
+
+
+def declared():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(dc_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = declared()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(_lib.EXPORTS) <= set(names)
+
+
+def test_version_and_tiles():
+    L = _lib.lib()
+    L.dc_version.restype = C.c_char_p
+    assert b"gfx950" in L.dc_version()
+    tiles = _lib.Context.gemm_tiles()
+    assert len(tiles) >= 4
+    for rt, ct, nt in tiles:
+        assert rt % 16 == 0 and ct % 16 == 0 and nt % 64 == 0
+    r = C.c_int()
+    assert L.dc_gemm_tile_shape(len(tiles), C.byref(r), None, None) == _lib.DC_ERR_ARG
+
+
+def test_null_context_arguments():
+    L = _lib.lib()
+    assert L.dc_set_precision(None, 0) == _lib.DC_ERR_ARG
+    assert L.dc_set_gemm_tile(None, 0) == _lib.DC_ERR_ARG
+    assert L.dc_crystal_set(None, None) == _lib.DC_ERR_ARG
+    assert L.dc_four_run(None, None, None, None, None, None, None) == _lib.DC_ERR_ARG
+    assert L.dc_strucf(None, None, 0, 0, None, None, None, None, None, None, None, None, 0, None, None, None, 0,
+                       None) == _lib.DC_ERR_ARG
+    assert L.dc_create(None, 0) == _lib.DC_ERR_ARG
+
+
+def _gpu_present():
+    return os.path.exists("/dev/kfd") and bool(shutil.which("rocminfo")) and \
+        subprocess.run(["rocminfo"], capture_output=True).returncode == 0
+
+
+@pytest.mark.skipif(_gpu_present(), reason="checks the no-device path")
+def test_create_without_device_reports_nodev():
+    L = _lib.lib()
+    h = C.c_void_p()
+    assert L.dc_create(C.byref(h), 0) == _lib.DC_ERR_NODEV
+
+
+def test_fortran_interface_module_compiles(tmp_path):
+    fc = shutil.which("flang") or "/opt/rocm/llvm/bin/flang"
+    src = os.path.join(ROOT, "diffusecode_amd", "fortran", "diffuse_turbo_mod.f90")
+    if not os.path.exists(fc) or not os.path.exists(src):
+        pytest.skip("flang or the Fortran module not present")
+    p = subprocess.run([fc, "-c", "-module-dir", str(tmp_path), src, "-o", str(tmp_path / "m.o")],
+                       capture_output=True, text=True)
+    assert p.returncode == 0, p.stderr

@@ -171,3 +171,18 @@ def test_four_run_ellipsoid_3d(ctx):
     err = float(np.abs(gpu["dsi"] - ref["dsi"]).max() / ref["dsi"].max())
     print(f"  dsi max rel {err:.3e}", flush=True)
     assert err <= 1e-6
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_fortran_host_dropin(prec):
+    """The Fortran host program (diffuse_turbo_mod, ISO_C_BINDING) calls the
+    library like DISCUS's four_strucf and checks a closed form."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "diffusecode_amd", "_build", "fortran", "example_strucf")
+    assert os.path.exists(exe), "build the Fortran example first (__graft_entry__.build)"
+    print(f"case fortran prec={prec}", flush=True)
+    p = subprocess.run([exe, prec], capture_output=True, text=True, timeout=120)
+    print(p.stdout, p.stderr, flush=True)
+    assert p.returncode == 0 and "PASS" in p.stdout

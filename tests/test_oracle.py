@@ -1,0 +1,326 @@
+"""Pins the CPU oracle (oracle/dc_oracle.c) against what the mathematics
+fixes: closed forms, invariants, exact rational arithmetic and brute force.
+Each test is chosen so that a plausible slip (dropped term, wrong sign or
+index, transposed operand) fails it."""
+import cmath
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from diffusecode_amd import crystal as X
+from diffusecode_amd import fourier as FR
+from oracle import oracle as O
+
+
+def grid_h(xm, u, v, w, num):
+    i = np.arange(num[0])[:, None, None]
+    j = np.arange(num[1])[None, :, None]
+    k = np.arange(num[2])[None, None, :]
+    return [xm[a] + i * u[a] + j * v[a] + k * w[a] for a in range(3)]
+
+
+def test_dirichlet_lattice_closed_form():
+    """Perfect N1xN2xN3 lattice: F = prod_a exp(i pi h_a (N_a-1)) sin(pi N_a h_a)/sin(pi h_a)."""
+    n = (5, 4, 3)
+    pos = np.array([[i, j, k] for k in range(n[2]) for j in range(n[1]) for i in range(n[0])], float)
+    xm, u, v, w = [-1.03, -0.47, 0.2], [0.037, 0, 0], [0, 0.051, 0], [0, 0, 0.06]
+    num = [31, 21, 11]
+    F = O.strucf_exact(pos, xm, u, v, w, num)
+    hh = grid_h(xm, u, v, w, num)
+    D = np.ones(num, complex)
+    for a in range(3):
+        h = hh[a]
+        D = D * np.exp(1j * np.pi * h * (n[a] - 1)) * np.sin(np.pi * n[a] * h) / np.sin(np.pi * h)
+    assert np.abs(F - D).max() < 1e-11 * np.abs(D).max()
+
+
+def test_single_atom_origin_is_form_factor():
+    c = X.rocksalt(n=2)
+    cf = O.formtab(c.scat, c.occ, c.dw)
+    eck, vi, inc = FR.grid_block((-3, -2, 0), (3, 2, 0), (13, 9, 1))
+    st, istl = O.stltab(eck[:, 0], vi[:, 0], vi[:, 1], vi[:, 2], inc, c.rten())
+    F = O.strucf_exact(np.zeros((1, 3)), eck[:, 0], vi[:, 0], vi[:, 1], vi[:, 2], inc, cfact_col=cf[:, 0], istl=istl)
+    np.testing.assert_allclose(F, cf[istl, 0], rtol=1e-14, atol=1e-14)
+
+
+def test_rocksalt_integer_hkl_selection_rules():
+    """Rocksalt at integer hkl: 4(fA+fB) all even, 4(fA-fB) all odd, 0 mixed (per cell)."""
+    n = 3
+    pos, types = X.supercell(X.ROCKSALT_SITES, X.ROCKSALT_TYPES, (n, n, n))
+    Fa = O.strucf_exact(pos[types == 1], [-2, -2, -2], [1, 0, 0], [0, 1, 0], [0, 0, 1], [5, 5, 5])
+    Fb = O.strucf_exact(pos[types == 2], [-2, -2, -2], [1, 0, 0], [0, 1, 0], [0, 0, 1], [5, 5, 5])
+    fa, fb = 1.7, 0.6
+    F = fa * Fa + fb * Fb
+    ncell = n ** 3
+    for i in range(5):
+        for j in range(5):
+            for k in range(5):
+                par = [(x - 2) % 2 for x in (i, j, k)]
+                if sum(par) == 0:
+                    want = 4 * (fa + fb) * ncell
+                elif sum(par) == 3:
+                    want = 4 * (fa - fb) * ncell
+                else:
+                    want = 0.0
+                assert abs(F[i, j, k] - want) < 1e-10 * ncell
+
+
+def test_friedel_conjugate_and_translation_invariance():
+    rng = np.random.default_rng(1)
+    pos = rng.random((200, 3)) * 7
+    xm, u, v = [-1.3, -0.9, 0.4], [0.13, 0.02, 0], [0, 0.11, 0.05]
+    num = [21, 17, 1]
+    F = O.strucf_exact(pos, xm, u, v, [0, 0, 0], num)
+    Fm = O.strucf_exact(pos, [-x for x in xm], [-x for x in u], [-x for x in v], [0, 0, 0], num)
+    np.testing.assert_allclose(Fm, np.conj(F), atol=1e-11)
+    Ft = O.strucf_exact(pos + np.array([3.0, -2.0, 5.0]), xm, u, v, [0, 0, 0], num)
+    np.testing.assert_allclose(np.abs(Ft) ** 2, np.abs(F) ** 2, rtol=1e-10, atol=1e-9)
+
+
+def test_discrete_parseval():
+    """Atoms on a 1/M grid, h over one full period of integer points:
+    sum_h |F(h)|^2 = M^3 sum_m c_m^2 (c_m = occupation of grid point m)."""
+    M = 6
+    rng = np.random.default_rng(2)
+    pts = rng.integers(0, M, size=(40, 3))
+    pts = np.vstack([pts, pts[:7]])          # some doubly occupied points
+    # frequencies k/M, k = 0..M-1 per axis, on integer positions: a 3-D DFT of the counts
+    H = O.strucf_exact(pts.astype(float), [0, 0, 0], [1.0 / M, 0, 0], [0, 1.0 / M, 0], [0, 0, 1.0 / M], [M, M, M])
+    _, counts = np.unique(pts, axis=0, return_counts=True)
+    assert abs((np.abs(H) ** 2).sum() - M ** 3 * (counts ** 2).sum()) < 1e-8 * M ** 3 * (counts ** 2).sum()
+
+
+def test_brute_force_rational():
+    """A few atoms with rational coordinates, h.r reduced exactly with Fraction."""
+    pos = [(Fraction(1, 3), Fraction(2, 7), Fraction(5, 11)), (Fraction(7, 5), Fraction(-3, 4), Fraction(9, 13)),
+           (Fraction(11, 2), Fraction(13, 3), Fraction(-17, 6))]
+    xm = (Fraction(-13, 10), Fraction(7, 20), Fraction(1, 4))
+    u, v = (Fraction(3, 20), 0, 0), (0, Fraction(1, 10), Fraction(1, 30))
+    num = [9, 7, 1]
+    F = O.strucf_exact(np.array(pos, float), [float(x) for x in xm], [float(x) for x in u], [float(x) for x in v],
+                       [0, 0, 0], num)
+    for i in range(num[0]):
+        for j in range(num[1]):
+            h = [xm[a] + i * u[a] + j * v[a] for a in range(3)]
+            want = 0
+            for p in pos:
+                ph = sum(h[a] * p[a] for a in range(3))
+                ph -= math.floor(ph)
+                want += cmath.exp(2j * math.pi * float(ph))
+            assert abs(F[i, j, 0] - want) < 1e-12
+
+
+def turbo_expected_single(x, vecs, lmn, num):
+    """DISCUS's integer pipeline for one atom, restated with Python ints:
+    NINT(2^22 (t - INT(t))), iarg sums, idx = floor(iarg/64) mod 2^16."""
+    inc = []
+    for v in vecs:
+        t = sum(a * b for a, b in zip(v, x))
+        fr = t - math.trunc(t)
+        inc.append(int(math.floor(4194304.0 * fr + 0.5)) if fr >= 0 else -int(math.floor(-4194304.0 * fr + 0.5)))
+    out = np.zeros(num, complex)
+    for j in range(num[0]):
+        for i in range(num[1]):
+            for h in range(num[2]):
+                iarg = sum(m * c for m, c in zip(lmn, inc)) + j * inc[0] + i * inc[1] + h * inc[2]
+                idx = (iarg // 64) % 65536
+                out[j, i, h] = cmath.exp(2j * math.pi * idx / 65536)
+    return out
+
+
+def test_turbo_integer_pipeline_kat():
+    """Negative increments, negative lmn and wrap-around of the 2^16 index."""
+    x = (0.3125, -1.71875, 2.4375)
+    vecs = [(0.13, 0.0, 0.0), (0.0, -0.07, 0.02), (0.0, 0.0, 0.0), (0.001, 0, 0), (0, 0.001, 0), (0, 0, 0.001)]
+    lmn = (-40, 17, 0, 3, -5, 0)
+    num = [7, 5, 1]
+    off = np.array(vecs[3:]).T
+    F = O.strucf_turbo(np.array([x]), vecs[0], vecs[1], vecs[2], off, lmn, num)
+    want = turbo_expected_single(x, vecs, lmn, num)
+    np.testing.assert_allclose(F, want, atol=1e-15)
+
+
+def test_turbo_close_to_exact_within_table_bound():
+    """Turbo error per term <= 2 pi (2^-16 + (sum|index| + sum|lmn|) 2^-23)."""
+    c = X.rocksalt(n=3)
+    eck, vi, inc = FR.grid_block((-5, -5, 0), (5, 5, 0), (41, 41, 1))
+    st, lmn, off = O.fourier_lmn(eck, vi, inc)
+    sel = c.iscat == 1
+    Ft = O.strucf_turbo(c.pos[sel], vi[:, 0], vi[:, 1], vi[:, 2], off, lmn, inc)
+    Fe = O.strucf_exact(c.pos[sel], eck[:, 0], vi[:, 0], vi[:, 1], vi[:, 2], inc)
+    N = int(sel.sum())
+    jj = np.arange(inc[0])[:, None, None]
+    ii = np.arange(inc[1])[None, :, None]
+    bound = N * 2 * np.pi * (2.0 ** -16 + (np.abs(lmn).sum() + jj + ii + 1) * 2.0 ** -23)
+    assert np.all(np.abs(Ft - Fe) <= bound)
+    # at the h=0 point (j=i=20) all indices vanish: exact
+    assert abs(Ft[20, 20, 0] - N) < 1e-9
+
+
+def test_fourier_lmn_reconstruction():
+    """eck(:,1) = sum lmn(1:3) vi + sum lmn(4:6) off(:,j) for layers and volumes."""
+    cases = [((-5, -5, 0), (5, 5, 0), (101, 101, 1)), ((-4, -4, -4), (4, 4, 4), (161, 161, 161)),
+             ((-10, -10, 0.5), (10, 10, 0.5), (401, 401, 1)), ((0, 0, 1.5), (3, 2, 1.5), (31, 21, 1)),
+             ((-2, -3, 0.25), (2, 3, 3.75), (41, 61, 15))]
+    for lo, hi, n in cases:
+        eck, vi, inc = FR.grid_block(lo, hi, n)
+        st, lmn, off = O.fourier_lmn(eck, vi, inc)
+        assert st == 0
+        vecs = [vi[:, 0], vi[:, 1], vi[:, 2], off[:, 0], off[:, 1], off[:, 2]]
+        rec = sum(int(m) * v for m, v in zip(lmn, vecs))
+        np.testing.assert_allclose(rec, eck[:, 0], atol=2e-7)
+
+
+def test_fourier_lmn_zero_vector_error():
+    eck, vi, inc = FR.grid_block((0, 0, 0), (0, 0, 0), (11, 11, 1))
+    st, lmn, off = O.fourier_lmn(eck, vi, inc)
+    assert st == -132
+
+
+def test_stltab_cubic_closed_form_and_overflow():
+    a = 4.0
+    rten = X.reciprocal_metric(a, a, a)
+    eck, vi, inc = FR.grid_block((-3, -2, 1), (3, 2, 1), (25, 17, 1))
+    st, istl = O.stltab(eck[:, 0], vi[:, 0], vi[:, 1], vi[:, 2], inc, rten)
+    assert st == 0
+    hh = grid_h(eck[:, 0], vi[:, 0], vi[:, 1], vi[:, 2], inc)
+    s = np.sqrt(hh[0] ** 2 + hh[1] ** 2 + hh[2] ** 2) / (2 * a)
+    want = np.floor(s * 1000 + 0.5).astype(int)
+    assert np.abs(istl - want).max() <= 1 and np.mean(istl == want) > 0.99
+    eck, vi, inc = FR.grid_block((0, 0, 0), (200, 0, 0), (3, 1, 1))
+    st, _ = O.stltab(eck[:, 0], vi[:, 0], vi[:, 1], vi[:, 2], inc, rten)
+    assert st == -3
+
+
+def test_formtab_closed_form():
+    c = X.orthorhombic(ncell=(2, 2, 2))
+    cf = O.formtab(c.scat, c.occ, c.dw, ldbw=True)
+    s = np.arange(10000) * 0.001
+    for t in range(c.nscat):
+        sc = c.scat[t]
+        f = sc[0] + sum(sc[2 * i - 1] * np.exp(-sc[2 * i] * s * s) for i in range(1, 5))
+        want = c.occ[t] * np.exp(-c.dw[t] * s * s) * f
+        np.testing.assert_allclose(cf[:, t].real, want, rtol=1e-13)
+        assert np.all(cf[:, t].imag == 0)
+    # the product-side table builder matches the oracle's
+    np.testing.assert_allclose(FR.formtab(c, ldbw=True), cf, rtol=1e-14)
+
+
+def friedel_rule(num, fnum, arr, conj):
+    """The per-pixel statement of the sequential mirror used by the GPU
+    finalize kernel (dc_lot.hip region_src)."""
+    out = arr.copy()
+    for i in range(num[0]):
+        for j in range(num[1]):
+            for k in range(num[2]):
+                m = (num[0] - 1 - i, num[1] - 1 - j, num[2] - 1 - k)
+                q = (i, j, k)
+                inq = i < fnum[0] and j < fnum[1] and k < fnum[2]
+                inm = m[0] < fnum[0] and m[1] < fnum[1] and m[2] < fnum[2]
+                if m == q:
+                    out[q] = np.conj(arr[q]) if conj else arr[q]
+                elif inm and ((not inq) or m < q):
+                    out[q] = np.conj(arr[m]) if conj else arr[m]
+    return out
+
+
+@pytest.mark.parametrize("num", [(7, 5, 3), (9, 1, 1), (1, 7, 3), (1, 1, 5), (5, 5, 1)])
+def test_apply_friedel_sequential_equals_rule(num):
+    eck, vi, inc = FR.grid_block([-(n - 1) * 0.1 for n in num], [(n - 1) * 0.1 for n in num], num)
+    ax, fnum = O.test_friedel(eck, vi, inc)
+    assert ax in (1, 2, 3)
+    rng = np.random.default_rng(0)
+    a = rng.normal(size=num) + 1j * rng.normal(size=num)
+    d = rng.random(num)
+    acsf, csf, dsi = O.apply_friedel(num, fnum, acsf=a, csf=a, dsi=d)
+    np.testing.assert_array_equal(csf, friedel_rule(num, fnum, a, True))
+    np.testing.assert_array_equal(dsi, friedel_rule(num, fnum, d, False))
+
+
+def test_test_friedel_cases():
+    eck, vi, inc = FR.grid_block((-2, -2, 0), (2, 2, 0), (41, 41, 1))
+    ax, fnum = O.test_friedel(eck, vi, inc)
+    assert ax == 1 and list(fnum) == [21, 41, 1]
+    ax, fnum = O.test_friedel(eck, vi, inc, ano=True)
+    assert ax == 0
+    ax, fnum = O.test_friedel(eck, vi, inc, four_friedel=False)
+    assert ax == 0
+    eck, vi, inc = FR.grid_block((-2, -2, 0.5), (2, 2, 0.5), (41, 41, 1))
+    assert O.test_friedel(eck, vi, inc)[0] == 0
+    eck, vi, inc = FR.grid_block((-2, -2, 0), (2, 2, 0), (40, 41, 1))
+    assert O.test_friedel(eck, vi, inc)[0] == 0
+
+
+def test_four_run_bragg_subtraction_kat():
+    """Perfect crystal, <F> from 100% of the cells, periodic box lots:
+    every lot equals the average structure, dsi == 0 (four_aver + four_run)."""
+    pos, types = X.supercell(X.FCC_SITES, np.array([1, 2, 1, 1], np.int32), (4, 4, 4))
+    c = X.Crystal(pos, types, (4, 4, 4), 4, (4.0, 4.0, 4.0, 90, 90, 90), X.random_form_factors(np.random.default_rng(0), 2),
+                  np.ones(2), np.zeros(2))
+    eck, vi, inc = FR.grid_block((-2, -2, 0.3), (2, 2, 0.3), (17, 17, 1))
+    cf = FR.formtab(c)
+    lbeg = np.array([[1, 1, 1], [3, 2, 4], [4, 4, 4]], np.int32)
+    out = O.four_run(c, eck, vi, inc, cf, ilots=2, ls=(2, 2, 2), lbeg=lbeg, fave=1.0,
+                     sel_cell=np.ones((4, 4, 4), np.uint8))
+    Fmax = np.abs(out["acsf"]).max()
+    assert Fmax > 1
+    assert out["dsi"].max() < 1e-18 * Fmax ** 2 * 1e6
+
+
+def test_four_run_single_lot_is_abs_square():
+    c = X.rocksalt(n=2, seed=4)
+    eck, vi, inc = FR.grid_block((-2, -2, 0), (2, 2, 0), (21, 21, 1))
+    cf = FR.formtab(c)
+    out = O.four_run(c, eck, vi, inc, cf)
+    F = np.zeros(inc, complex)
+    for t in (1, 2):
+        F += O.strucf_exact(c.pos[c.iscat == t], eck[:, 0], vi[:, 0], vi[:, 1], vi[:, 2], inc,
+                            cfact_col=cf[:, t - 1],
+                            istl=O.stltab(eck[:, 0], vi[:, 0], vi[:, 1], vi[:, 2], inc, c.rten())[1])
+    np.testing.assert_allclose(out["dsi"], np.abs(F) ** 2, rtol=1e-12, atol=1e-9)
+    np.testing.assert_allclose(out["csf"], F, rtol=1e-12, atol=1e-9)
+
+
+def test_four_run_lot_variance_matches_two_pass():
+    c = X.fcc_correlated(n=5, seed=2)
+    eck, vi, inc = FR.grid_block((-1, -1, 0.5), (1, 1, 0.5), (11, 11, 1))
+    cf = FR.formtab(c)
+    lbeg = np.array([[1, 1, 1], [2, 4, 5], [5, 5, 5], [3, 1, 2]], np.int32)
+    out = O.four_run(c, eck, vi, inc, cf, ilots=2, ls=(2, 3, 2), lbeg=lbeg, want_var=True)
+    Fs = []
+    for lb in lbeg:
+        o1 = O.four_run(c, eck, vi, inc, cf, ilots=2, ls=(2, 3, 2), lbeg=lb[None, :])
+        Fs.append(o1["csf"])
+    Fs = np.array(Fs)
+    dnorm = 125 / (12 * 4)
+    np.testing.assert_allclose(out["fmean"], Fs.mean(0), rtol=1e-12, atol=1e-10)
+    var = ((np.abs(Fs - Fs.mean(0)) ** 2).sum(0)) * dnorm
+    np.testing.assert_allclose(out["dvar"], var, rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(out["dsi"], (np.abs(Fs) ** 2).sum(0) * dnorm, rtol=1e-12)
+
+
+def test_lot_cell_counts():
+    """Box lots hold ls1*ls2*ls3 cells; ellipsoid lots the cells whose
+    centres satisfy the ellipsoid test (four_getatm :1912-1937)."""
+    c = X.fcc_correlated(n=6, seed=1)
+    eck, vi, inc = FR.grid_block((0.1, 0.1, 0.1), (0.1, 0.1, 0.1), (1, 1, 1))
+    cf = FR.formtab(c)
+    for shape, ls in ((O.LOT_BOX, (3, 2, 4)), (O.LOT_ELI, (5, 4, 5))):
+        out = O.four_run(c, eck, vi, inc, cf, ilots=shape, ls=ls, lbeg=np.array([[2, 3, 4], [6, 6, 6]], np.int32),
+                         four_friedel=False)
+        x0 = np.array(ls) / 2.0
+        cnt = 0
+        for kk in range(ls[2]):
+            for jj in range(ls[1]):
+                for ii in range(ls[0]):
+                    t = sum(((v - x0[d] + 0.5) ** 2) / x0[d] ** 2 for d, v in enumerate((ii, jj, kk)))
+                    cnt += shape == O.LOT_BOX or t <= 1.0
+        # dnorm = Ncells/(ncell*nlots); recover ncell from dsi = dnorm * sum |F_l|^2
+        s = 0.0
+        for lb in ([2, 3, 4], [6, 6, 6]):
+            s += abs(O.four_run(c, eck, vi, inc, cf, ilots=shape, ls=ls, lbeg=np.array([lb], np.int32),
+                                four_friedel=False)["csf"].ravel()[0]) ** 2
+        ncell = 216 * s / (out["dsi"].ravel()[0] * 2)
+        assert abs(ncell - cnt) < 1e-6 * cnt
