@@ -35,6 +35,7 @@ EXPORTS = [
     "dc_create", "dc_destroy", "dc_last_error", "dc_last_error_type", "dc_set_precision",
     "dc_get_precision", "dc_set_gemm_tile", "dc_gemm_num_tiles", "dc_gemm_tile_shape", "dc_set_cancel_flag", "dc_version", "dc_device_info", "dc_strucf",
     "dc_crystal_set", "dc_four_run", "dc_four_run_dev", "dc_last_stats", "dc_synchronize",
+    "dc_model_intensity", "dc_model_vjp", "dc_model_chi2",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -55,7 +56,14 @@ class RunDesc(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("pairs", C.c_double), ("gemm_ms", C.c_double), ("total_ms", C.c_double),
-                ("reduce_ms", C.c_double), ("gemm_launches", C.c_int64), ("slices", C.c_int64)]
+                ("reduce_ms", C.c_double), ("gemm_launches", C.c_int64), ("slices", C.c_int64),
+                ("forward_ms", C.c_double), ("adjoint_ms", C.c_double)]
+
+
+class ModelDesc(C.Structure):
+    _fields_ = [("pos", _dp), ("type", _ip), ("occ", _dp), ("bdw", _dp), ("natoms", C.c_int64),
+                ("nscat", C.c_int), ("ncfact", C.c_int), ("cfact", _dp), ("eck", C.c_double * 12),
+                ("vi", C.c_double * 9), ("inc", C.c_int * 3), ("rten", C.c_double * 9)]
 
 
 _lib = None
@@ -90,6 +98,9 @@ def lib():
                                   C.c_void_p, C.c_void_p]
     L.dc_last_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
     L.dc_synchronize.argtypes = [C.c_void_p]
+    L.dc_model_intensity.argtypes = [C.c_void_p, C.POINTER(ModelDesc), _dp, _dp]
+    L.dc_model_vjp.argtypes = [C.c_void_p, C.POINTER(ModelDesc), _dp, _dp, _dp]
+    L.dc_model_chi2.argtypes = [C.c_void_p, C.POINTER(ModelDesc), _dp, _dp, _dp, _dp, _dp]
     _lib = L
     return L
 
@@ -295,3 +306,60 @@ class Context:
                 return None
             return t if isinstance(t, int) else t.data_ptr()
         return self.check(lib().dc_four_run_dev(self.h, C.byref(desc), p(csf), p(acsf), p(dsi), p(fmean), p(dvar)))
+
+    # -- refinement model ----------------------------------------------------
+    @staticmethod
+    def model_desc(pos, types, occ, bdw, cfact, eck, vi, inc, rten):
+        """dc_model_desc from arrays: pos (N,3) fractional, types (N) 1-based,
+        occ (N), bdw (N) or None, cfact (ncfact, nscat) complex.  Returns the
+        descriptor and the arrays it points to (keep them alive)."""
+        keep = []
+        m = ModelDesc()
+        p = np.ascontiguousarray(np.asarray(pos, np.float64).reshape(-1, 3))
+        t = np.ascontiguousarray(np.asarray(types, np.int32).ravel())
+        o = np.ascontiguousarray(np.asarray(occ, np.float64).ravel())
+        keep += [p, t, o]
+        m.pos, m.type, m.occ = _d(p), _i(t), _d(o)
+        if bdw is not None:
+            b = np.ascontiguousarray(np.asarray(bdw, np.float64).ravel())
+            keep.append(b)
+            m.bdw = _d(b)
+        m.natoms = p.shape[0]
+        cf = np.asfortranarray(np.asarray(cfact, np.complex128))
+        cff = np.ascontiguousarray(cf.ravel(order="F")).view(np.float64)
+        keep.append(cff)
+        m.cfact = _d(cff)
+        m.ncfact, m.nscat = cf.shape[0], cf.shape[1]
+        m.eck[:] = list(_f64(eck, 12))
+        m.vi[:] = list(_f64(vi, 9))
+        m.inc[:] = [int(v) for v in inc]
+        m.rten[:] = list(_f64(rten, 9))
+        return m, keep
+
+    def model_intensity(self, desc):
+        """(I, F) over the grid, Fortran order of inc."""
+        num = tuple(desc.inc)
+        n = int(np.prod(num))
+        icalc = np.zeros(n)
+        fcalc = np.zeros(2 * n)
+        self.check(lib().dc_model_intensity(self.h, C.byref(desc), _d(icalc), _d(fcalc)))
+        return icalc.reshape(num, order="F"), fcalc.view(np.complex128).reshape(num, order="F")
+
+    def model_vjp(self, desc, fcalc, gi):
+        """grad (N, 5): dL/dx, dL/dy, dL/dz, dL/docc, dL/dB for dL/dI = gi."""
+        f = np.ascontiguousarray(np.asarray(fcalc, np.complex128).ravel(order="F")).view(np.float64)
+        g = _f64(gi)
+        grad = np.zeros(5 * desc.natoms)
+        self.check(lib().dc_model_vjp(self.h, C.byref(desc), _d(f), _d(g), _d(grad)))
+        return grad.reshape(-1, 5)
+
+    def model_chi2(self, desc, target, weight=None, want_grad=True):
+        """(chi2, I, grad (N, 5) or None) for chi2 = sum w (I - target)^2."""
+        num = tuple(desc.inc)
+        t = _f64(target)
+        w = None if weight is None else _f64(weight)
+        icalc = np.zeros(t.size)
+        grad = np.zeros(5 * desc.natoms) if want_grad else None
+        chi = C.c_double()
+        self.check(lib().dc_model_chi2(self.h, C.byref(desc), _d(t), _d(w), C.byref(chi), _d(icalc), _d(grad)))
+        return chi.value, icalc.reshape(num, order="F"), None if grad is None else grad.reshape(-1, 5)

@@ -220,6 +220,24 @@ def orthorhombic(ncell=(16, 16, 16), abc=(4.0, 5.0, 6.0), nsite=8, ntypes=4, see
                    scat, occ, dw, name="ortho")
 
 
+def refinement_parameters(cr: Crystal, seed=6, occ_spread=0.1, b_sigma=0.05):
+    """Per-atom occupancies and isotropic B of a crystal (REFINE's parameter
+    vector): the type values of cr.occ / cr.dw with seeded per-atom spread."""
+    rng = np.random.default_rng(seed)
+    t = np.maximum(cr.iscat, 1) - 1
+    occ = cr.occ[t] * rng.uniform(1.0 - occ_spread, 1.0, size=cr.natoms)
+    bdw = np.maximum(cr.dw[t] + rng.normal(0.0, b_sigma, size=cr.natoms), 0.05)
+    return occ, bdw
+
+
+def perturb_parameters(pos, occ, bdw, seed=7, dx=0.002, docc=0.02, db=0.05):
+    """Seeded perturbation (delta r, delta occ, delta B) of a parameter set."""
+    rng = np.random.default_rng(seed)
+    return (pos + rng.normal(0.0, dx, size=pos.shape),
+            occ + rng.normal(0.0, docc, size=occ.shape),
+            np.maximum(bdw + rng.normal(0.0, db, size=bdw.shape), 0.01))
+
+
 def population(nmembers=96, nmin=2000, nmax=60000, seed=5):
     """DIFFEV-like population: fcc/bcc/rocksalt supercells of log-uniform size."""
     rng = np.random.default_rng(seed)

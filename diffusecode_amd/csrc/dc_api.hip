@@ -13,128 +13,20 @@
 #include <vector>
 
 #include "../../include/diffuse_turbo.h"
-#include "dc_internal.h"
+#include "dc_host.h"
 
 using namespace dc;
 
-namespace {
-
-struct DevBuf {
-    void *p = nullptr;
-    size_t n = 0;
-};
+namespace dc {
 
 constexpr int GEMM_KS = 16;
 constexpr int KCHUNK_MIN = 512;
 constexpr int KCHUNK_MAX = 16384;
 constexpr double PARTIAL_BUDGET = 6.0e9;   // bytes of slice planes per batch
 
-}  // namespace
-
-struct dc_ctx {
-    int device = 0;
-    hipStream_t st = nullptr;
-    int prec = DC_PREC_FP32;
-    int tile = -1;                  // forced GEMM tile configuration, -1 = planned
-    std::string err;
-    int err_typ = DC_ER_NONE;
-    const volatile int *cancel = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    dc_stats stats{};
-    // device buffers
-    DevBuf rec, slice_beg, slice_type, lot_slice, partial, cfact, istl, idx, oid, offs, xat, w;
-    DevBuf acsf, dsi, csf, wm, wv, lat, out_tcsf, out[5];
-    // crystal
-    bool have_crystal = false;
-    DevBuf pos, iscat;
-    std::vector<double> h_pos;
-    std::vector<int> h_iscat;
-    int64_t natoms = 0;
-    int icc[3] = {1, 1, 1};
-    int ncatoms = 1, nscat = 0;
-    double dim0[3] = {0, 0, 0}, dim_lo[3] = {0, 0, 0};
-    std::vector<int> type_beg;      // per type offsets into d type list
-    DevBuf type_idx;
-};
+}  // namespace dc
 
 namespace {
-
-int set_err(dc_ctx *c, int code, int typ, const std::string &msg) {
-    if (c) {
-        c->err = msg;
-        c->err_typ = typ;
-    }
-    return code;
-}
-
-int hip_err(dc_ctx *c, hipError_t e, const char *where) {
-    return set_err(c, e == hipErrorOutOfMemory ? DC_ERR_NOMEM : DC_ERR_DEVICE, DC_ER_LIB,
-                   std::string(where) + ": " + hipGetErrorString(e));
-}
-
-#define HIPCHK(call)                                   \
-    do {                                               \
-        hipError_t e_ = (call);                        \
-        if (e_ != hipSuccess) return hip_err(c, e_, #call); \
-    } while (0)
-
-int ensure(dc_ctx *c, DevBuf &b, size_t bytes) {
-    if (bytes == 0) bytes = 16;
-    if (b.n >= bytes) return 0;
-    if (b.p) {
-        hipError_t e = hipStreamSynchronize(c->st);
-        if (e != hipSuccess) return hip_err(c, e, "sync before realloc");
-        (void)hipFree(b.p);
-        b.p = nullptr;
-        b.n = 0;
-    }
-    size_t n = bytes + bytes / 8;
-    hipError_t e = hipMalloc(&b.p, n);
-    if (e != hipSuccess) {
-        b.p = nullptr;
-        return hip_err(c, e, "hipMalloc");
-    }
-    b.n = n;
-    return 0;
-}
-
-template <class T>
-T *ptr(DevBuf &b) {
-    return reinterpret_cast<T *>(b.p);
-}
-
-template <class T>
-int upload(dc_ctx *c, DevBuf &b, const T *src, size_t count) {
-    int st = ensure(c, b, sizeof(T) * count);
-    if (st) return st;
-    if (count) HIPCHK(hipMemcpyAsync(b.p, src, sizeof(T) * count, hipMemcpyHostToDevice, c->st));
-    return 0;
-}
-
-bool cancelled(dc_ctx *c) { return c->cancel && *c->cancel; }
-
-// sin(theta)/lambda bin with the expression of four_stltab
-// (fourier_sup.f90:2663-2674); mirrors istl_point of dc_lot.hip.
-#pragma clang fp contract(off)
-int host_istl(const double *xm, const double *u, const double *v, const double *w, int i, int j, int l,
-              const double *rten) {
-    double h[3], t[3];
-    for (int k = 0; k < 3; ++k) h[k] = xm[k] + u[k] * (double)i + v[k] * (double)j + w[k] * (double)l;
-    for (int r = 0; r < 3; ++r) t[r] = rten[r] * h[0] + rten[r + 3] * h[1] + rten[r + 6] * h[2];
-    double q2 = (h[0] * t[0] + h[1] * t[1] + h[2] * t[2]) / 4.0;
-    return (int)std::llround(std::sqrt(q2) * (1.0 / CFINC));
-}
-#pragma clang fp contract(on)
-
-// s is a norm of h, so its maximum over the parallelepiped grid sits at a corner.
-int max_corner_bin(const Grid &g) {
-    int m = 0;
-    for (int ci = 0; ci < 8; ++ci) {
-        int i = (ci & 1) ? g.n[0] - 1 : 0, j = (ci & 2) ? g.n[1] - 1 : 0, l = (ci & 4) ? g.n[2] - 1 : 0;
-        m = std::max(m, host_istl(g.xm, g.uin, g.vin, g.win, i, j, l, g.rten));
-    }
-    return m;
-}
 
 // four_test_friedel (fourier_sup.f90:3645-3728): reduced sizes; returns the
 // halved axis (1..3) or 0.
@@ -174,6 +66,33 @@ bool zero_increment(const double *vi, const int *inc) {
     return false;
 }
 
+}  // namespace
+
+namespace dc {
+
+// sin(theta)/lambda bin with the expression of four_stltab
+// (fourier_sup.f90:2663-2674); mirrors istl_point of dc_lot.hip.
+#pragma clang fp contract(off)
+int host_istl(const double *xm, const double *u, const double *v, const double *w, int i, int j, int l,
+              const double *rten) {
+    double h[3], t[3];
+    for (int k = 0; k < 3; ++k) h[k] = xm[k] + u[k] * (double)i + v[k] * (double)j + w[k] * (double)l;
+    for (int r = 0; r < 3; ++r) t[r] = rten[r] * h[0] + rten[r + 3] * h[1] + rten[r + 6] * h[2];
+    double q2 = (h[0] * t[0] + h[1] * t[1] + h[2] * t[2]) / 4.0;
+    return (int)std::llround(std::sqrt(q2) * (1.0 / CFINC));
+}
+#pragma clang fp contract(on)
+
+// s is a norm of h, so its maximum over the parallelepiped grid sits at a corner.
+int max_corner_bin(const Grid &g) {
+    int m = 0;
+    for (int ci = 0; ci < 8; ++ci) {
+        int i = (ci & 1) ? g.n[0] - 1 : 0, j = (ci & 2) ? g.n[1] - 1 : 0, l = (ci & 4) ? g.n[2] - 1 : 0;
+        m = std::max(m, host_istl(g.xm, g.uin, g.vin, g.win, i, j, l, g.rten));
+    }
+    return m;
+}
+
 long long padded(long long n, long long t) { return (n + t - 1) / t * t; }
 
 // GEMM plan for the computed region f of grid g.
@@ -182,19 +101,16 @@ long long padded(long long n, long long t) { return (n + t - 1) / t * t; }
 // Fast periods are padded to >= 16 when the side wraps (the two-level operand
 // tables of dc_gemm.hip need at most one wrap per 16-group); padded pixels
 // are computed and never read.  The tile configuration minimises the padded
-// work of whole work-group tiles (ties: larger tiles); forced >= 0 overrides.
-struct Plan {
-    GemmGeom gg{};
-    int mapping = 0;
-    int tile = 0;
-};
+// work of whole work-group tiles (ties: larger tiles); forced >= 0 overrides,
+// only_mapping >= 0 restricts the mapping.
 
-Plan plan_gemm(Grid &g, int forced) {
+Plan plan_gemm(Grid &g, int forced, int only_mapping) {
     Plan best;
     double best_cost = 1e300;
     const int nt = gemm_num_tiles();
     for (int mapping = 0; mapping < 2; ++mapping) {
         if (mapping == 1 && g.f[2] <= 1) continue;
+        if (only_mapping >= 0 && mapping != only_mapping) continue;
         GemmGeom gg{};
         long long ps[3];
         if (mapping == 0) {
@@ -238,13 +154,6 @@ int kchunk_for(const Plan &pl, long long natoms_batch) {
     return (int)padded(k, GEMM_KS);
 }
 
-// Slices over a list of (type, count) segments laid out consecutively.
-struct SliceSet {
-    std::vector<int> beg;        // nslice + 1 atom offsets
-    std::vector<int> type;       // per slice
-    std::vector<int> lot;        // lot boundaries (slice ids), nlots + 1
-    std::vector<int> seg_first;  // first slice of each segment
-};
 
 void add_segments(SliceSet &s, const std::vector<std::pair<int, int>> &segs, int kchunk, int &atom_off) {
     for (auto &sg : segs) {
@@ -259,7 +168,7 @@ void add_segments(SliceSet &s, const std::vector<std::pair<int, int>> &segs, int
     }
 }
 
-int run_gemm(dc_ctx *c, const Plan &pl, int nslice, float2 *partial) {
+int run_gemm(dc_ctx *c, const Plan &pl, int nslice, float2 *partial, const float *sq) {
     const GemmGeom &gg = pl.gg;
     if (nslice <= 0) return 0;
     HIPCHK(hipEventRecord(c->ev[2], c->st));
@@ -269,7 +178,7 @@ int run_gemm(dc_ctx *c, const Plan &pl, int nslice, float2 *partial) {
         if (cancelled(c)) return set_err(c, DC_ERR_CANCEL, DC_ER_COMM, "cancelled");
         int ns = std::min(per, nslice - s0);
         HIPCHK(launch_phase_gemm(c->prec, pl.tile, ptr<AtomRec>(c->rec), ptr<int>(c->slice_beg),
-                                 partial + (long long)s0 * gg.P, gg, s0, ns, c->st));
+                                 partial + (long long)s0 * gg.P, sq, gg, s0, ns, c->st));
         c->stats.gemm_launches++;
     }
     HIPCHK(hipEventRecord(c->ev[3], c->st));
@@ -293,6 +202,10 @@ PrepArgs base_prep(const Grid &g, int mapping) {
     a.mapping = mapping;
     return a;
 }
+
+}  // namespace dc
+
+namespace {
 
 // Compute sum over atom segments (idx list on device or explicit positions)
 // into planes; used by the average structure and the lattice.
@@ -341,11 +254,7 @@ void dc_destroy(dc_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->st);
-    DevBuf *bufs[] = {&c->rec, &c->slice_beg, &c->slice_type, &c->lot_slice, &c->partial, &c->cfact,
-                      &c->istl, &c->idx, &c->oid, &c->offs, &c->xat, &c->w, &c->acsf, &c->dsi, &c->csf,
-                      &c->wm, &c->wv, &c->lat, &c->out_tcsf, &c->pos, &c->iscat, &c->type_idx,
-                      &c->out[0], &c->out[1], &c->out[2], &c->out[3], &c->out[4]};
-    for (DevBuf *b : bufs)
+    for (DevBuf *b : c->buffers())
         if (b->p) (void)hipFree(b->p);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);

@@ -112,6 +112,7 @@ template <class T>
 struct Tables {
     float2 al[2][16 * KS], ah[T::RH * KS];
     float2 bl[2][16 * KS], bh[T::CH * KS];
+    float dmp[KS];                      // B_n log2(e) of the stage's atoms (damped variant)
 };
 
 template <int SCHEME, class T>
@@ -120,6 +121,7 @@ struct Smem {
     Tables<T> t[2];
     int a_k16[T::RH], a_wpos[T::RH];   // wraps before group h, position of the wrap inside it
     int b_k16[T::CH], b_wpos[T::CH];
+    float s_a[T::RT], s_b[T::CT];      // s^2 row / column parts of the tile (damped variant)
 };
 
 // Side geometry of the tile: start index, offset along the fast axis, period.
@@ -149,6 +151,7 @@ __device__ __forceinline__ void build_tables(Tables<T> &tb, const int *__restric
                                              const Side &sa, const Side &sb, int pt) {
     const int a = pt & (KS - 1), r = pt >> 4;
     constexpr int NR = T::NPT / KS;           // helper threads per atom
+    if (r == 0) tb.dmp[a] = live ? ar.pad * 1.4426950408889634f : 0.0f;
     if (r < 8) {
         const bool sideb = r >= 4;
         const int m = r & 3;
@@ -198,9 +201,10 @@ __device__ __forceinline__ void ld4(const float2 *p, float2 (&v)[4]) {
 
 // One 16-row group h of side A (isA) or B: lane (l = row low bits, g = atom
 // quad) forms the 4 elements TH[h] * TL_d[l] and stores them.
-template <int SCHEME, class T, bool isA>
+template <int SCHEME, class T, bool isA, bool DAMP>
 __device__ __forceinline__ void elems_group(Elems<SCHEME, T> &el, const float2 *__restrict__ th, const float2 (&tl0)[4],
-                                            const float2 (&tl1)[4], int h, int wpos, int l, int g) {
+                                            const float2 (&tl1)[4], const f32x4 &dm, const float *__restrict__ s_tab,
+                                            int h, int wpos, int l, int g) {
     float2 hv[4];
     ld4(th + h * KS + 4 * g, hv);
     const bool d = l >= wpos;
@@ -208,6 +212,15 @@ __device__ __forceinline__ void elems_group(Elems<SCHEME, T> &el, const float2 *
 #pragma unroll
     for (int q = 0; q < 4; ++q) v[q] = cmul(hv[q], d ? tl1[q] : tl0[q]);
     const int row = 16 * h + l;
+    if constexpr (DAMP) {
+        // per-atom isotropic displacement factor exp(-B_n s^2) of this row / column
+        const float sv = s_tab[row];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float f = __builtin_amdgcn_exp2f(-dm[q] * sv);
+            v[q] = make_float2(v[q].x * f, v[q].y * f);
+        }
+    }
     if constexpr (SCHEME == SCH_FP32) {
         float *dst = (isA ? el.a : el.b) + row * FST + 8 * g;
         *reinterpret_cast<f32x4 *>(dst) = f32x4{v[0].x, v[0].y, v[1].x, v[1].y};
@@ -234,24 +247,27 @@ __device__ __forceinline__ void elems_group(Elems<SCHEME, T> &el, const float2 *
 
 // Elements of one stage, built by wave pw: its contiguous run of the RH + CH
 // 16-row groups.  The low tables of a side are loaded once.
-template <int SCHEME, class T>
+template <int SCHEME, class T, bool DAMP>
 __device__ __forceinline__ void build_elems(Elems<SCHEME, T> &el, const Tables<T> &tb, const int *__restrict__ a_wpos,
-                                            const int *__restrict__ b_wpos, int pw) {
+                                            const int *__restrict__ b_wpos, const float *__restrict__ s_a,
+                                            const float *__restrict__ s_b, int pw) {
     constexpr int NJOB = T::RH + T::CH;
     const int lane = threadIdx.x & 63, g = lane & 3, l = lane >> 2;
     const int j0 = (pw * NJOB) / T::NW, j1 = ((pw + 1) * NJOB) / T::NW;
     float2 tl0[4], tl1[4];
+    f32x4 dm = f32x4{0.f, 0.f, 0.f, 0.f};
+    if constexpr (DAMP) dm = *reinterpret_cast<const f32x4 *>(&tb.dmp[4 * g]);
     if (j0 < T::RH) {
         ld4(&tb.al[0][l * KS + 4 * g], tl0);
         ld4(&tb.al[1][l * KS + 4 * g], tl1);
         const int je = j1 < T::RH ? j1 : T::RH;
-        for (int j = j0; j < je; ++j) elems_group<SCHEME, T, true>(el, tb.ah, tl0, tl1, j, a_wpos[j], l, g);
+        for (int j = j0; j < je; ++j) elems_group<SCHEME, T, true, DAMP>(el, tb.ah, tl0, tl1, dm, s_a, j, a_wpos[j], l, g);
     }
     if (j1 > T::RH) {
         ld4(&tb.bl[0][l * KS + 4 * g], tl0);
         ld4(&tb.bl[1][l * KS + 4 * g], tl1);
         for (int j = (j0 > T::RH ? j0 : T::RH); j < j1; ++j)
-            elems_group<SCHEME, T, false>(el, tb.bh, tl0, tl1, j - T::RH, b_wpos[j - T::RH], l, g);
+            elems_group<SCHEME, T, false, DAMP>(el, tb.bh, tl0, tl1, dm, s_b, j - T::RH, b_wpos[j - T::RH], l, g);
     }
 }
 
@@ -329,10 +345,10 @@ __device__ __forceinline__ void mfma_stage(const Elems<SCHEME, T> &el, f32x4 (&a
     }
 }
 
-template <int SCHEME, class T>
+template <int SCHEME, class T, bool DAMP>
 __global__ void __launch_bounds__(T::NT)
 phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice_beg, float2 *__restrict__ partial,
-                  GemmGeom g, int slice0) {
+                  const float *__restrict__ sq, GemmGeom g, int slice0) {
     __shared__ Smem<SCHEME, T> sm;
     const int slice = slice0 + blockIdx.z;
     const int kbeg = slice_beg[slice], kend = slice_beg[slice + 1];
@@ -352,6 +368,15 @@ phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice
         const int x = s.a0 + 16 * hh;
         (isa ? sm.a_k16 : sm.b_k16)[hh] = x / s.period;
         (isa ? sm.a_wpos : sm.b_wpos)[hh] = s.period - (x % s.period);   // first wrap inside the group
+    }
+    if constexpr (DAMP) {
+        // sq = [s^2 of the R rows | s^2 of the C columns]; padding reads 0
+        for (int x = threadIdx.x; x < T::RT + T::CT; x += T::NT) {
+            const bool isa = x < T::RT;
+            const int o = isa ? row0 + x : col0 + x - T::RT;
+            const int lim = isa ? g.R : g.C;
+            (isa ? sm.s_a[x] : sm.s_b[x - T::RT]) = o < lim ? sq[(isa ? 0 : g.R) + o] : 0.0f;
+        }
     }
 
     f32x4 accR[T::WR][T::WC], accI[T::WR][T::WC];
@@ -377,14 +402,14 @@ phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice
     __syncthreads();
     build_tables<T>(sm.t[0], sm.a_k16, sm.b_k16, r0, kbeg + at < kend, sa, sb, pt);
     __syncthreads();
-    build_elems<SCHEME, T>(sm.e[0], sm.t[0], sm.a_wpos, sm.b_wpos, wave);
+    build_elems<SCHEME, T, DAMP>(sm.e[0], sm.t[0], sm.a_wpos, sm.b_wpos, sm.s_a, sm.s_b, wave);
     if (nst > 1) build_tables<T>(sm.t[1], sm.a_k16, sm.b_k16, r1, kbeg + KS + at < kend, sa, sb, pt);
     __syncthreads();
     for (int s = 0; s < nst; ++s) {
         const int cur = s & 1, nxt = cur ^ 1;
         const AtomRec rt = rn;
         if (s + 3 < nst) rn = fetch(s + 3);
-        if (s + 1 < nst) build_elems<SCHEME, T>(sm.e[nxt], sm.t[nxt], sm.a_wpos, sm.b_wpos, wave);
+        if (s + 1 < nst) build_elems<SCHEME, T, DAMP>(sm.e[nxt], sm.t[nxt], sm.a_wpos, sm.b_wpos, sm.s_a, sm.s_b, wave);
         if (s + 2 < nst) build_tables<T>(sm.t[cur], sm.a_k16, sm.b_k16, rt, kbeg + (s + 2) * KS + at < kend, sa, sb, pt);
         if (live) mfma_stage<SCHEME, T>(sm.e[cur], accR, accI, wr0, wc0);
         __syncthreads();
@@ -425,36 +450,39 @@ int gemm_num_tiles() { return (int)(sizeof(kTiles) / sizeof(kTiles[0])); }
 TileInfo gemm_tile_info(int t) { return kTiles[t]; }
 
 template <int SCHEME, class T>
-static void launch_one(const AtomRec *rec, const int *slice_beg, float2 *partial, const GemmGeom &g, int slice0,
-                       int nslice, hipStream_t st) {
+static void launch_one(const AtomRec *rec, const int *slice_beg, float2 *partial, const float *sq, const GemmGeom &g,
+                       int slice0, int nslice, hipStream_t st) {
     dim3 grid((g.C + T::CT - 1) / T::CT, (g.R + T::RT - 1) / T::RT, nslice);
-    phase_gemm_kernel<SCHEME, T><<<grid, dim3(T::NT), 0, st>>>(rec, slice_beg, partial, g, slice0);
+    if (sq)
+        phase_gemm_kernel<SCHEME, T, true><<<grid, dim3(T::NT), 0, st>>>(rec, slice_beg, partial, sq, g, slice0);
+    else
+        phase_gemm_kernel<SCHEME, T, false><<<grid, dim3(T::NT), 0, st>>>(rec, slice_beg, partial, sq, g, slice0);
 }
 
 template <int SCHEME>
-static hipError_t launch_scheme(int tile, const AtomRec *rec, const int *slice_beg, float2 *partial,
+static hipError_t launch_scheme(int tile, const AtomRec *rec, const int *slice_beg, float2 *partial, const float *sq,
                                 const GemmGeom &g, int slice0, int nslice, hipStream_t st) {
     switch (tile) {
-    case 0: launch_one<SCHEME, CfgA>(rec, slice_beg, partial, g, slice0, nslice, st); break;
-    case 1: launch_one<SCHEME, CfgB>(rec, slice_beg, partial, g, slice0, nslice, st); break;
-    case 2: launch_one<SCHEME, CfgC>(rec, slice_beg, partial, g, slice0, nslice, st); break;
-    case 3: launch_one<SCHEME, CfgD>(rec, slice_beg, partial, g, slice0, nslice, st); break;
-    case 4: launch_one<SCHEME, CfgE>(rec, slice_beg, partial, g, slice0, nslice, st); break;
-    case 5: launch_one<SCHEME, CfgF>(rec, slice_beg, partial, g, slice0, nslice, st); break;
+    case 0: launch_one<SCHEME, CfgA>(rec, slice_beg, partial, sq, g, slice0, nslice, st); break;
+    case 1: launch_one<SCHEME, CfgB>(rec, slice_beg, partial, sq, g, slice0, nslice, st); break;
+    case 2: launch_one<SCHEME, CfgC>(rec, slice_beg, partial, sq, g, slice0, nslice, st); break;
+    case 3: launch_one<SCHEME, CfgD>(rec, slice_beg, partial, sq, g, slice0, nslice, st); break;
+    case 4: launch_one<SCHEME, CfgE>(rec, slice_beg, partial, sq, g, slice0, nslice, st); break;
+    case 5: launch_one<SCHEME, CfgF>(rec, slice_beg, partial, sq, g, slice0, nslice, st); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
 hipError_t launch_phase_gemm(int scheme, int tile, const AtomRec *rec, const int *slice_beg, float2 *partial,
-                             const GemmGeom &g, int slice0, int nslice, hipStream_t st) {
+                             const float *sq, const GemmGeom &g, int slice0, int nslice, hipStream_t st) {
     // the two-level operand tables need wrap periods of at least 16
     if (g.R0 < 16 && g.R0 != g.R) return hipErrorInvalidValue;
     if (g.C0 < 16 && g.C0 != g.C) return hipErrorInvalidValue;
     switch (scheme) {
-    case SCH_FP32: return launch_scheme<SCH_FP32>(tile, rec, slice_beg, partial, g, slice0, nslice, st);
-    case SCH_BF16X3: return launch_scheme<SCH_BF16X3>(tile, rec, slice_beg, partial, g, slice0, nslice, st);
-    case SCH_FP16X2: return launch_scheme<SCH_FP16X2>(tile, rec, slice_beg, partial, g, slice0, nslice, st);
+    case SCH_FP32: return launch_scheme<SCH_FP32>(tile, rec, slice_beg, partial, sq, g, slice0, nslice, st);
+    case SCH_BF16X3: return launch_scheme<SCH_BF16X3>(tile, rec, slice_beg, partial, sq, g, slice0, nslice, st);
+    case SCH_FP16X2: return launch_scheme<SCH_FP16X2>(tile, rec, slice_beg, partial, sq, g, slice0, nslice, st);
     default: return hipErrorInvalidValue;
     }
 }

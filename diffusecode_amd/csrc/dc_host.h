@@ -1,0 +1,131 @@
+// dc_host.h -- host-side state and helpers shared by the C-ABI translation
+// units (dc_api.hip: the Fourier run, dc_refine.hip: the refinement model).
+#pragma once
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/diffuse_turbo.h"
+#include "dc_internal.h"
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+};
+
+struct dc_ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    int prec = DC_PREC_FP32;
+    int tile = -1;                  // forced GEMM tile configuration, -1 = planned
+    std::string err;
+    int err_typ = DC_ER_NONE;
+    const volatile int *cancel = nullptr;
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    dc_stats stats{};
+    // device buffers of the Fourier run
+    DevBuf rec, slice_beg, slice_type, lot_slice, partial, cfact, istl, idx, oid, offs, xat, w;
+    DevBuf acsf, dsi, csf, wm, wv, lat, out_tcsf, out[5];
+    // device buffers of the refinement model
+    DevBuf m_pos, m_w, m_b, m_sq, m_q, m_part, m_grad, m_tiles, m_gi, m_f, m_tgt, m_qt;
+    // crystal
+    bool have_crystal = false;
+    DevBuf pos, iscat;
+    std::vector<double> h_pos;
+    std::vector<int> h_iscat;
+    int64_t natoms = 0;
+    int icc[3] = {1, 1, 1};
+    int ncatoms = 1, nscat = 0;
+    double dim0[3] = {0, 0, 0}, dim_lo[3] = {0, 0, 0};
+    std::vector<int> type_beg;      // per type offsets into d type list
+    DevBuf type_idx;
+
+    std::vector<DevBuf *> buffers() {
+        return {&rec, &slice_beg, &slice_type, &lot_slice, &partial, &cfact, &istl, &idx, &oid, &offs, &xat, &w,
+                &acsf, &dsi, &csf, &wm, &wv, &lat, &out_tcsf, &out[0], &out[1], &out[2], &out[3], &out[4],
+                &m_pos, &m_w, &m_b, &m_sq, &m_q, &m_part, &m_grad, &m_tiles, &m_gi, &m_f, &m_tgt, &m_qt,
+                &pos, &iscat, &type_idx};
+    }
+};
+
+namespace dc {
+
+inline int set_err(dc_ctx *c, int code, int typ, const std::string &msg) {
+    if (c) {
+        c->err = msg;
+        c->err_typ = typ;
+    }
+    return code;
+}
+
+inline int hip_err(dc_ctx *c, hipError_t e, const char *where) {
+    return set_err(c, e == hipErrorOutOfMemory ? DC_ERR_NOMEM : DC_ERR_DEVICE, DC_ER_LIB,
+                   std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(call)                                        \
+    do {                                                    \
+        hipError_t e_ = (call);                             \
+        if (e_ != hipSuccess) return hip_err(c, e_, #call); \
+    } while (0)
+
+// Grow-only device buffer (reallocation synchronises the context stream).
+inline int ensure(dc_ctx *c, DevBuf &b, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (b.n >= bytes) return 0;
+    if (b.p) {
+        hipError_t e = hipStreamSynchronize(c->st);
+        if (e != hipSuccess) return hip_err(c, e, "sync before realloc");
+        (void)hipFree(b.p);
+        b.p = nullptr;
+        b.n = 0;
+    }
+    size_t n = bytes + bytes / 8;
+    hipError_t e = hipMalloc(&b.p, n);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return hip_err(c, e, "hipMalloc");
+    }
+    b.n = n;
+    return 0;
+}
+
+template <class T>
+T *ptr(DevBuf &b) {
+    return reinterpret_cast<T *>(b.p);
+}
+
+template <class T>
+int upload(dc_ctx *c, DevBuf &b, const T *src, size_t count) {
+    int st = ensure(c, b, sizeof(T) * count);
+    if (st) return st;
+    if (count) HIPCHK(hipMemcpyAsync(b.p, src, sizeof(T) * count, hipMemcpyHostToDevice, c->st));
+    return 0;
+}
+
+inline bool cancelled(dc_ctx *c) { return c->cancel && *c->cancel; }
+
+// GEMM plan of a grid region (dc_api.hip).
+struct Plan {
+    GemmGeom gg{};
+    int mapping = 0;
+    int tile = 0;
+};
+
+// Slices over a list of (type, count) segments laid out consecutively.
+struct SliceSet {
+    std::vector<int> beg;        // nslice + 1 atom offsets
+    std::vector<int> type;       // per slice
+    std::vector<int> lot;        // lot boundaries (slice ids), nlots + 1
+    std::vector<int> seg_first;  // first slice of each segment
+};
+
+long long padded(long long n, long long t);
+int max_corner_bin(const Grid &g);   // largest sin(theta)/lambda bin over the grid corners
+Plan plan_gemm(Grid &g, int forced, int only_mapping = -1);
+int kchunk_for(const Plan &pl, long long natoms_batch);
+void add_segments(SliceSet &s, const std::vector<std::pair<int, int>> &segs, int kchunk, int &atom_off);
+int run_gemm(dc_ctx *c, const Plan &pl, int nslice, float2 *partial, const float *sq = nullptr);
+PrepArgs base_prep(const Grid &g, int mapping);
+
+}  // namespace dc

@@ -15,6 +15,7 @@ struct PrepArgs {
     const int *oid;         // offset id per gathered atom (null: none)
     const double *offs;     // offset table (3 per id)
     const float *w;         // per-atom amplitude (null: 1)
+    const float *bdw;       // per-atom isotropic B in AtomRec::pad (null: 0)
     long long n;
     int mode;               // PREP_CORNER / PREP_LMN
     int mapping;            // 0: rows = abscissa ; 1: rows = (abscissa, ordinate)
@@ -35,6 +36,21 @@ struct Grid {
     long long plane;       // elements of one slice plane (R * C of the GEMM)
     int friedel;
 };
+
+// sin(theta)/lambda bin, four_stltab's expression (fourier_sup.f90:2663-2674,
+// skalpro metric.f90:103) evaluated left to right with no contraction so the
+// bins equal the host restatement bit for bit.
+__device__ inline int istl_point(const Grid &g, int i, int j, int l) {
+#pragma clang fp contract(off)
+    double h[3], t[3];
+    for (int k = 0; k < 3; ++k)
+        h[k] = g.xm[k] + g.uin[k] * (double)i + g.vin[k] * (double)j + g.win[k] * (double)l;
+    for (int r = 0; r < 3; ++r) t[r] = g.rten[r] * h[0] + g.rten[r + 3] * h[1] + g.rten[r + 6] * h[2];
+    double q2 = (h[0] * t[0] + h[1] * t[1] + h[2] * t[2]) / 4.0;
+    double v = __builtin_sqrt(q2) * (1.0 / CFINC);
+    int b = (int)__builtin_round(v);
+    return b < 0 ? 0 : (b > CFPKT ? CFPKT : b);
+}
 
 struct TileInfo {
     int rt, ct, nt;
@@ -64,8 +80,10 @@ struct FinalArgs {
     double sq, dscale, vscale;
 };
 
+// sq (R + C floats: s^2 parts of rows and columns) selects the variant with a
+// per-atom displacement factor exp(-B_n s^2), B_n in AtomRec::pad; null = none.
 hipError_t launch_phase_gemm(int scheme, int tile, const AtomRec *rec, const int *slice_beg, float2 *partial,
-                             const GemmGeom &g, int slice0, int nslice, hipStream_t st);
+                             const float *sq, const GemmGeom &g, int slice0, int nslice, hipStream_t st);
 int gemm_num_tiles();
 TileInfo gemm_tile_info(int t);
 hipError_t launch_prep(const PrepArgs &a, AtomRec *out, hipStream_t st);
@@ -75,5 +93,51 @@ hipError_t launch_aver_combine(const Grid &g, double *acsf, const double *lat, d
 hipError_t launch_finalize(const FinalArgs &a, hipStream_t st);
 hipError_t launch_strucf_out(const ReduceArgs &a, double *tcsf, long long ntot, hipStream_t st);
 hipError_t launch_istl(const Grid &g, int *out, hipStream_t st);
+
+// ---- refinement model (dc_adjoint.hip) --------------------------------------
+// Adjoint of the phase GEMM.  With Q_t(p) = 2 (dL/dI_p) conj(F_p) f_t(p) and
+// E_n(p) = exp(-B_n s_p^2) exp(2 pi i h_p.x_n), every atom n of type t needs
+//   S0 = sum_p Q E,  S_r = sum_p (r - rc) Q E,  S_j, S_m (column indices),
+//   S_s = sum_p s_p^2 Q E;
+// the kernel writes their per-column-tile partial sums
+// [Re S0, Im S0, Im S_r, Im S_j, Im S_m, Re S_s] to part[tile][atom][6].
+struct AdjArgs {
+    const AtomRec *rec;     // type-grouped atoms (mapping 0 records, B_n in pad)
+    const int4 *tiles;      // atom tiles: (first, end, Q plane, 0), each inside one type
+    const float2 *q;        // Q planes: q[t * plane + c + C * r] (column fastest)
+    const float *sq;        // s^2 parts: R rows, then C columns
+    float *part;            // [column tile][natoms][6]
+    int R, C, C0, f1;       // rows, padded columns, column period, valid columns per period
+    long long plane;        // R * C
+    int natoms;
+    float rc, jc, mc;       // centres of the row / column index weights
+};
+struct AdjFinishArgs {
+    const float *part;
+    int ntile_c, natoms;
+    const int *idx;         // sorted atom -> caller index
+    const double *occ;      // sorted occupancies
+    double hc[3], v[9];     // h at the weight centres; grid increments vi(:,1..3)
+    double *grad;           // (5, caller atoms): d/dx, d/dy, d/dz, d/docc, d/dB
+};
+struct QArgs {
+    Grid g;
+    int R, C, C0;
+    long long plane;
+    const double *f;        // F over the grid (complex, abscissa fastest)
+    const double *gi;       // dL/dI per pixel
+    const double2 *cfact;   // (ncfact, nscat)
+    int ncfact;
+    const int *qtype;       // cfact column of each Q plane
+    int nq;
+    float2 *q;
+};
+int adjoint_tile_atoms();
+int adjoint_tile_cols();
+hipError_t launch_adjoint(const AdjArgs &a, int ntiles, hipStream_t st);
+hipError_t launch_adjoint_finish(const AdjFinishArgs &a, hipStream_t st);
+hipError_t launch_model_q(const QArgs &a, hipStream_t st);
+hipError_t launch_chi2_resid(const double *icalc, const double *tgt, const double *wt, long long n, double *gi,
+                             double *blocksum, int nblocks, hipStream_t st);
 
 }  // namespace dc

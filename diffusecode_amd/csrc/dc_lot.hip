@@ -52,24 +52,8 @@ __global__ void prep_records_kernel(PrepArgs a, AtomRec *__restrict__ out) {
         r.ca = inc[2]; r.ca1 = 0;
     }
     r.w = a.w ? a.w[k] : 1.0f;
-    r.pad = 0.0f;
+    r.pad = a.bdw ? a.bdw[k] : 0.0f;
     out[k] = r;
-}
-
-// ---------------------------------------------------------------------------
-// sin(theta)/lambda bin, four_stltab's expression (fourier_sup.f90:2663-2674,
-// skalpro metric.f90:103) evaluated left to right with no contraction so the
-// bins equal the host restatement bit for bit.
-__device__ int istl_point(const Grid &g, int i, int j, int l) {
-#pragma clang fp contract(off)
-    double h[3], t[3];
-    for (int k = 0; k < 3; ++k)
-        h[k] = g.xm[k] + g.uin[k] * (double)i + g.vin[k] * (double)j + g.win[k] * (double)l;
-    for (int r = 0; r < 3; ++r) t[r] = g.rten[r] * h[0] + g.rten[r + 3] * h[1] + g.rten[r + 6] * h[2];
-    double q2 = (h[0] * t[0] + h[1] * t[1] + h[2] * t[2]) / 4.0;
-    double v = __builtin_sqrt(q2) * (1.0 / CFINC);
-    int b = (int)__builtin_round(v);
-    return b < 0 ? 0 : (b > CFPKT ? CFPKT : b);
 }
 
 __device__ __forceinline__ void decode(const Grid &g, long long p, int &i, int &j, int &l) {

@@ -171,9 +171,57 @@ typedef struct dc_stats {
     double reduce_ms;
     int64_t gemm_launches;
     int64_t slices;
+    double forward_ms;          /* refinement model: forward (GEMM + reduction)   */
+    double adjoint_ms;          /* refinement model: reverse pass (Q, GEMM, sums) */
 } dc_stats;
 int dc_last_stats(const dc_ctx *ctx, dc_stats *out);
 int dc_synchronize(dc_ctx *ctx);
+
+/*
+ * Refinement model with per-atom parameters (REFINE's parameter vector,
+ * refine/prog/refine_run.f90:491-600, which today obtains derivatives by
+ * re-running the Fourier calculation for every parameter):
+ *   F(h) = sum_n f_t(h) occ_n exp(-B_n s^2) exp(2 pi i h.x_n),  I = |F|^2,
+ * over the grid h = eck(:,1) + i vi(:,1) + j vi(:,2) + l vi(:,3) (i < inc(1)
+ * fastest, like dsi), s = sin(theta)/lambda from rten, f_t = cfact(istl(h), t)
+ * with four_stltab's bins (fourier_sup.f90:2663-2674).  cfact carries
+ * f + f' + i f'' only: occupancy and B are per atom here.
+ * Per-atom B (and every derivative with respect to B) needs the abscissa
+ * vi(:,1) orthogonal in the metric to vi(:,2) and vi(:,3), so that
+ * s^2 = s^2(i) + s^2(j, l); otherwise DC_ERR_ARG.
+ */
+typedef struct dc_model_desc {
+    const double *pos;          /* (3, natoms) fractional coordinates           */
+    const int *type;            /* (natoms) 1-based scattering type, 0 = vacancy */
+    const double *occ;          /* (natoms)                                      */
+    const double *bdw;          /* (natoms) isotropic B in A^2, NULL = all zero  */
+    int64_t natoms;
+    int nscat;
+    int ncfact;                 /* rows of cfact (CFPKT+1)                       */
+    const double *cfact;        /* (ncfact, nscat) complex                       */
+    double eck[12];
+    double vi[9];
+    int inc[3];
+    double rten[9];
+} dc_model_desc;
+
+/* icalc (real) and fcalc (complex), each inc(1)*inc(2)*inc(3), NULL-able. */
+int dc_model_intensity(dc_ctx *ctx, const dc_model_desc *model, double *icalc, double *fcalc);
+/*
+ * Vector-Jacobian product: given F of the model (fcalc from
+ * dc_model_intensity) and gi = dL/dI per pixel, writes grad(5, natoms):
+ * dL/dx, dL/dy, dL/dz (fractional), dL/docc, dL/dB for every atom
+ * (zero for vacancies).  This is the hand-written adjoint: it regenerates
+ * the phase operands instead of storing them.
+ */
+int dc_model_vjp(dc_ctx *ctx, const dc_model_desc *model, const double *fcalc, const double *gi, double *grad);
+/*
+ * chi^2 = sum_h w(h) (I(h) - target(h))^2 (weight NULL = 1) and its gradient
+ * grad(5, natoms) in one call (forward, residual and adjoint stay on the
+ * device).  icalc and grad are NULL-able.
+ */
+int dc_model_chi2(dc_ctx *ctx, const dc_model_desc *model, const double *target, const double *weight,
+                  double *chi2, double *icalc, double *grad);
 
 #ifdef __cplusplus
 }

@@ -760,3 +760,162 @@ int orc_four_run(const orc_run_t *p, double *csf, double *acsf, double *dsi, dou
 
 /* Size helper for the Python wrapper. */
 long orc_sizeof_run(void) { return (long)sizeof(orc_run_t); }
+
+/* ------------------------------------------------------------------------ */
+/* Refinement model with per-atom occupancy and isotropic B (the product's
+ * dc_model_* entry points; REFINE varies these per atom,
+ * refine/prog/refine_run.f90:491-600):
+ *   F(h) = sum_n f_t(bin h) occ_n exp(-B_n s^2) exp(2 pi i h.x_n),  I = |F|^2
+ * over h = eck(:,1) + i vi(:,1) + j vi(:,2) + l vi(:,3), s^2 = h.rten.h / 4,
+ * bin = four_stltab's nearest bin (orc_istl_point).  pos is (3,N), cfact
+ * (ncfact, nscat) complex, type 1-based with 0 = vacancy. */
+typedef struct {
+    long double h[3];
+    double s2;
+    int bin;
+} orc_pix_t;
+
+static orc_pix_t *model_pixels(const double eck[12], const double vi[9], const int inc[3], const double rten[9],
+                               long *np)
+{
+    long P = (long)inc[0] * inc[1] * inc[2];
+    orc_pix_t *px = (orc_pix_t *)malloc(sizeof(orc_pix_t) * (P > 0 ? P : 1));
+    for (long p = 0; p < P; ++p) {
+        int i = (int)(p % inc[0]), j = (int)((p / inc[0]) % inc[1]), l = (int)(p / ((long)inc[0] * inc[1]));
+        double hd[3];
+        for (int c = 0; c < 3; ++c) {
+            px[p].h[c] = (long double)eck[c] + (long double)i * vi[c] + (long double)j * vi[3 + c] +
+                         (long double)l * vi[6 + c];
+            hd[c] = (double)px[p].h[c];
+        }
+        double q = 0.0;
+        for (int r = 0; r < 3; ++r)
+            for (int k = 0; k < 3; ++k) q += hd[r] * rten[r + 3 * k] * hd[k];
+        px[p].s2 = q / 4.0;
+        px[p].bin = orc_istl_point(eck, vi, vi + 3, vi + 6, i, j, l, rten);
+    }
+    *np = P;
+    return px;
+}
+
+/* one atom's E = exp(-B s^2) exp(2 pi i h.x) at one pixel */
+static void atom_e(const orc_pix_t *p, const double *x, double b, double *er, double *ei)
+{
+    long double ph = p->h[0] * x[0] + p->h[1] * x[1] + p->h[2] * x[2];
+    ph -= floorl(ph);
+    double s, c;
+    sincos(ORC_ZPI * (double)ph, &s, &c);
+    double d = exp(-b * p->s2);
+    *er = c * d;
+    *ei = s * d;
+}
+
+void orc_model_f(const double *pos, const int *type, const double *occ, const double *bdw, long n,
+                 const double *cfact, int ncfact, const double eck[12], const double vi[9], const int inc[3],
+                 const double rten[9], double *F, int nthreads)
+{
+    long P;
+    orc_pix_t *px = model_pixels(eck, vi, inc, rten, &P);
+    set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long p = 0; p < P; ++p) {
+        double fr = 0.0, fi = 0.0;
+        for (long k = 0; k < n; ++k) {
+            if (type[k] <= 0) continue;
+            double er, ei;
+            atom_e(&px[p], pos + 3 * k, bdw ? bdw[k] : 0.0, &er, &ei);
+            const double *u = cfact + 2 * ((long)px[p].bin + (long)ncfact * (type[k] - 1));
+            double ar = occ[k] * (u[0] * er - u[1] * ei), ai = occ[k] * (u[0] * ei + u[1] * er);
+            fr += ar;
+            fi += ai;
+        }
+        F[2 * p] = fr;
+        F[2 * p + 1] = fi;
+    }
+    free(px);
+}
+
+/* Direct FP64 gradient of L with dL/dI = G (per pixel), given F of the model:
+ * grad(5,N) = (dL/dx, dL/dy, dL/dz, dL/docc, dL/dB); vacancies get zeros. */
+void orc_model_grad(const double *pos, const int *type, const double *occ, const double *bdw, long n,
+                    const double *cfact, int ncfact, const double eck[12], const double vi[9], const int inc[3],
+                    const double rten[9], const double *F, const double *G, double *grad, int nthreads)
+{
+    long P;
+    orc_pix_t *px = model_pixels(eck, vi, inc, rten, &P);
+    set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long k = 0; k < n; ++k) {
+        double *g = grad + 5 * k;
+        for (int c = 0; c < 5; ++c) g[c] = 0.0;
+        if (type[k] <= 0) continue;
+        double s0 = 0.0, sh[3] = {0.0, 0.0, 0.0}, ss = 0.0;
+        for (long p = 0; p < P; ++p) {
+            double er, ei;
+            atom_e(&px[p], pos + 3 * k, bdw ? bdw[k] : 0.0, &er, &ei);
+            const double *u = cfact + 2 * ((long)px[p].bin + (long)ncfact * (type[k] - 1));
+            /* Q = 2 G conj(F) u */
+            double cr = 2.0 * G[p] * F[2 * p], ci = -2.0 * G[p] * F[2 * p + 1];
+            double qr = cr * u[0] - ci * u[1], qi = cr * u[1] + ci * u[0];
+            double tr = qr * er - qi * ei, ti = qr * ei + qi * er;
+            s0 += tr;
+            for (int c = 0; c < 3; ++c) sh[c] += (double)px[p].h[c] * ti;
+            ss += px[p].s2 * tr;
+        }
+        for (int c = 0; c < 3; ++c) g[c] = -ORC_ZPI * occ[k] * sh[c];
+        g[3] = s0;
+        g[4] = -occ[k] * ss;
+    }
+    free(px);
+}
+
+/* Central differences of chi^2 = sum w (|F|^2 - T)^2 for nparam single
+ * parameters (atom[q], kind[q] in 0..4 = x, y, z, occ, B), step h, from the
+ * model's exact F0.  Only the perturbed atom's term changes, so each
+ * difference is formed per pixel as w (I+ - I-)(I+ + I- - 2T) with
+ * I+ - I- = 2 Re(conj F0 (d+ - d-)) + |d+|^2 - |d-|^2, d = change of the
+ * atom's term: no cancellation between two large chi^2 sums. */
+void orc_model_fd(const double *pos, const int *type, const double *occ, const double *bdw, long n,
+                  const double *cfact, int ncfact, const double eck[12], const double vi[9], const int inc[3],
+                  const double rten[9], const double *F0, const double *target, const double *weight,
+                  int nparam, const int *atom, const int *kind, double h, double *out, int nthreads)
+{
+    long P;
+    orc_pix_t *px = model_pixels(eck, vi, inc, rten, &P);
+    (void)n;
+    set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int q = 0; q < nparam; ++q) {
+        const long k = atom[q];
+        const int kd = kind[q];
+        if (type[k] <= 0) { out[q] = 0.0; continue; }
+        double x0[3] = {pos[3 * k], pos[3 * k + 1], pos[3 * k + 2]};
+        double o0 = occ[k], b0 = bdw ? bdw[k] : 0.0;
+        double acc = 0.0;
+        for (long p = 0; p < P; ++p) {
+            const double *u = cfact + 2 * ((long)px[p].bin + (long)ncfact * (type[k] - 1));
+            double t[3][2];   /* atom term at theta, theta+h, theta-h */
+            for (int v = 0; v < 3; ++v) {
+                double x[3] = {x0[0], x0[1], x0[2]}, o = o0, b = b0;
+                double dlt = v == 0 ? 0.0 : (v == 1 ? h : -h);
+                if (kd < 3) x[kd] += dlt; else if (kd == 3) o += dlt; else b += dlt;
+                double er, ei;
+                atom_e(&px[p], x, b, &er, &ei);
+                t[v][0] = o * (u[0] * er - u[1] * ei);
+                t[v][1] = o * (u[0] * ei + u[1] * er);
+            }
+            double dpr = t[1][0] - t[0][0], dpi = t[1][1] - t[0][1];
+            double dmr = t[2][0] - t[0][0], dmi = t[2][1] - t[0][1];
+            double fr = F0[2 * p], fi = F0[2 * p + 1];
+            double di = 2.0 * (fr * (dpr - dmr) + fi * (dpi - dmi)) + (dpr * dpr + dpi * dpi) - (dmr * dmr + dmi * dmi);
+            double i0 = fr * fr + fi * fi;
+            double ip = (fr + dpr) * (fr + dpr) + (fi + dpi) * (fi + dpi);
+            double im = (fr + dmr) * (fr + dmr) + (fi + dmi) * (fi + dmi);
+            (void)i0;
+            double w = weight ? weight[p] : 1.0;
+            acc += w * di * (ip + im - 2.0 * target[p]);
+        }
+        out[q] = acc / (2.0 * h);
+    }
+    free(px);
+}

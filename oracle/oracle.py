@@ -269,3 +269,54 @@ def four_run(crystal, eck, vi, inc, cfact, ilots=LOT_OFF, ls=(1, 1, 1), lbeg=Non
         out["fmean"] = fm.view(np.complex128).reshape(num, order="F")
         out["dvar"] = dv.reshape(num, order="F")
     return out
+
+
+# -- refinement model -----------------------------------------------------
+def _model_args(pos, types, occ, bdw, cfact, eck, vi, inc, rten):
+    p = np.ascontiguousarray(np.asarray(pos, np.float64).reshape(-1, 3))
+    t = np.ascontiguousarray(np.asarray(types, np.int32).ravel())
+    o = np.ascontiguousarray(np.asarray(occ, np.float64).ravel())
+    b = None if bdw is None else np.ascontiguousarray(np.asarray(bdw, np.float64).ravel())
+    cf = np.asfortranarray(np.asarray(cfact, np.complex128))
+    cff = np.ascontiguousarray(cf.ravel(order="F")).view(np.float64)
+    keep = (p, t, o, b, cff, _f64(eck, 12), _f64(vi, 9), _i32(inc, 3), _f64(rten, 9))
+    args = [_d(p), _i(t), _d(o), None if b is None else _d(b), C.c_long(p.shape[0]), _d(cff), C.c_int(cf.shape[0]),
+            _d(keep[5]), _d(keep[6]), _i(keep[7]), _d(keep[8])]
+    return args, keep
+
+
+def model_f(pos, types, occ, bdw, cfact, eck, vi, inc, rten, nthreads=None):
+    """Exact FP64 F of the refinement model over the grid (Fortran order)."""
+    args, keep = _model_args(pos, types, occ, bdw, cfact, eck, vi, inc, rten)
+    num = tuple(int(v) for v in inc)
+    out = np.zeros(2 * int(np.prod(num)))
+    lib().orc_model_f(*args, _d(out), C.c_int(nthreads or nthreads_default()))
+    return out.view(np.complex128).reshape(num, order="F")
+
+
+def model_grad(pos, types, occ, bdw, cfact, eck, vi, inc, rten, F, G, nthreads=None):
+    """Direct FP64 gradient (N, 5) of L with dL/dI = G, given the model's F."""
+    args, keep = _model_args(pos, types, occ, bdw, cfact, eck, vi, inc, rten)
+    f = np.ascontiguousarray(np.asarray(F, np.complex128).ravel(order="F")).view(np.float64)
+    g = _f64(G)
+    n = keep[0].shape[0]
+    out = np.zeros(5 * n)
+    lib().orc_model_grad(*args, _d(f), _d(g), _d(out), C.c_int(nthreads or nthreads_default()))
+    return out.reshape(n, 5)
+
+
+def model_fd(pos, types, occ, bdw, cfact, eck, vi, inc, rten, F0, target, weight, atoms, kinds, h=1e-6,
+             nthreads=None):
+    """FP64 central differences of chi^2 = sum w (|F|^2 - T)^2 for the listed
+    (atom, kind) parameters, kind 0..4 = x, y, z, occ, B."""
+    args, keep = _model_args(pos, types, occ, bdw, cfact, eck, vi, inc, rten)
+    f = np.ascontiguousarray(np.asarray(F0, np.complex128).ravel(order="F")).view(np.float64)
+    t = _f64(target)
+    w = None if weight is None else _f64(weight)
+    wp = None if w is None else _d(w)
+    a = _i32(atoms)
+    k = _i32(kinds, a.size)
+    out = np.zeros(a.size)
+    lib().orc_model_fd(*args, _d(f), _d(t), wp, C.c_int(a.size), _i(a), _i(k), C.c_double(h), _d(out),
+                       C.c_int(nthreads or nthreads_default()))
+    return out

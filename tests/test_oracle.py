@@ -324,3 +324,101 @@ def test_lot_cell_counts():
                                 four_friedel=False)["csf"].ravel()[0]) ** 2
         ncell = 216 * s / (out["dsi"].ravel()[0] * 2)
         assert abs(ncell - cnt) < 1e-6 * cnt
+
+
+# -- refinement model ----------------------------------------------------------
+def _ortho_grid(n=(9, 7, 5), lo=(-1.0, -0.8, -0.5), hi=(1.0, 0.8, 0.5)):
+    return FR.grid_block(lo, hi, n)
+
+
+def _small_model(seed=11, natoms=40, nscat=3):
+    rng = np.random.default_rng(seed)
+    pos = rng.random((natoms, 3)) * 3.0
+    types = rng.integers(1, nscat + 1, natoms).astype(np.int32)
+    types[3] = 0                                           # a vacancy
+    occ = rng.uniform(0.5, 1.0, natoms)
+    bdw = rng.uniform(0.2, 1.5, natoms)
+    scat = X.random_form_factors(rng, nscat)
+    from diffusecode_amd.refine import scattering_table
+    cf = scattering_table(scat)
+    return pos, types, occ, bdw, cf
+
+
+def test_model_single_atom_closed_form():
+    """One atom: F = f(bin) occ exp(-B s^2) exp(2 pi i h.x)."""
+    pos, types, occ, bdw, cf = _small_model()
+    rten = X.reciprocal_metric(4.0, 5.0, 6.0)
+    eck, vi, inc = _ortho_grid()
+    x = np.array([[0.31, 0.72, 0.13]])
+    F = O.model_f(x, [2], [0.8], [1.3], cf, eck, vi, inc, rten)
+    hh = grid_h(eck[:, 0], vi[:, 0], vi[:, 1], vi[:, 2], inc)
+    s2 = (hh[0] ** 2 * rten[0, 0] + hh[1] ** 2 * rten[1, 1] + hh[2] ** 2 * rten[2, 2]) / 4.0
+    st, istl = O.stltab(eck[:, 0], vi[:, 0], vi[:, 1], vi[:, 2], inc, rten)
+    ref = cf[istl, 1] * 0.8 * np.exp(-1.3 * s2) * np.exp(2j * np.pi * (hh[0] * x[0, 0] + hh[1] * x[0, 1] + hh[2] * x[0, 2]))
+    np.testing.assert_allclose(F, ref, rtol=1e-12, atol=1e-12)
+
+
+def test_model_equals_per_type_exact_sum():
+    """occ = 1, B = 0: F = sum_t cfact_t(bin) * strucf_exact(atoms of type t)."""
+    pos, types, occ, bdw, cf = _small_model()
+    rten = X.reciprocal_metric(4.0, 5.0, 6.0)
+    eck, vi, inc = _ortho_grid()
+    F = O.model_f(pos, types, np.ones(len(types)), None, cf, eck, vi, inc, rten)
+    st, istl = O.stltab(eck[:, 0], vi[:, 0], vi[:, 1], vi[:, 2], inc, rten)
+    ref = np.zeros(inc, complex)
+    for t in range(1, cf.shape[1] + 1):
+        ref += O.strucf_exact(pos[types == t], eck[:, 0], vi[:, 0], vi[:, 1], vi[:, 2], inc,
+                              cfact_col=cf[:, t - 1], istl=istl)
+    np.testing.assert_allclose(F, ref, rtol=1e-12, atol=1e-10)
+
+
+def test_model_grad_single_atom_closed_form():
+    """L = sum I for one atom: dL/dB = -2 sum s^2 I, dL/docc = 2 sum I / occ, dL/dx = 0."""
+    pos, types, occ, bdw, cf = _small_model()
+    rten = X.reciprocal_metric(4.0, 5.0, 6.0)
+    eck, vi, inc = _ortho_grid()
+    x, o, b = np.array([[0.31, 0.72, 0.13]]), 0.8, 1.3
+    F = O.model_f(x, [1], [o], [b], cf, eck, vi, inc, rten)
+    I = np.abs(F) ** 2
+    hh = grid_h(eck[:, 0], vi[:, 0], vi[:, 1], vi[:, 2], inc)
+    s2 = (hh[0] ** 2 * rten[0, 0] + hh[1] ** 2 * rten[1, 1] + hh[2] ** 2 * rten[2, 2]) / 4.0
+    g = O.model_grad(x, [1], [o], [b], cf, eck, vi, inc, rten, F, np.ones(I.size))
+    assert abs(g[0, 4] - (-2.0 * (s2 * I).sum())) < 1e-10 * abs(g[0, 4])
+    assert abs(g[0, 3] - 2.0 * I.sum() / o) < 1e-10 * abs(g[0, 3])
+    assert np.abs(g[0, :3]).max() < 1e-9 * I.sum()
+
+
+def test_model_fd_closed_form_b():
+    """chi^2 = sum I^2 (target 0) for one atom: d/dB = -4 sum s^2 I^2."""
+    pos, types, occ, bdw, cf = _small_model()
+    rten = X.reciprocal_metric(4.0, 5.0, 6.0)
+    eck, vi, inc = _ortho_grid()
+    x, o, b = np.array([[0.31, 0.72, 0.13]]), 0.8, 1.3
+    F = O.model_f(x, [1], [o], [b], cf, eck, vi, inc, rten)
+    I = np.abs(F) ** 2
+    hh = grid_h(eck[:, 0], vi[:, 0], vi[:, 1], vi[:, 2], inc)
+    s2 = (hh[0] ** 2 * rten[0, 0] + hh[1] ** 2 * rten[1, 1] + hh[2] ** 2 * rten[2, 2]) / 4.0
+    fd = O.model_fd(x, [1], [o], [b], cf, eck, vi, inc, rten, F, np.zeros(I.size), None, [0, 0], [4, 3])
+    assert abs(fd[0] - (-4.0 * (s2 * I * I).sum())) < 1e-7 * abs(fd[0])
+    assert abs(fd[1] - 4.0 * (I * I).sum() / o) < 1e-7 * abs(fd[1])
+
+
+def test_model_grad_matches_central_differences():
+    """Direct FP64 gradient of chi^2 against FP64 central differences, all 5N parameters."""
+    pos, types, occ, bdw, cf = _small_model()
+    rten = X.reciprocal_metric(4.0, 5.0, 6.0)
+    eck, vi, inc = _ortho_grid()
+    rng = np.random.default_rng(3)
+    F = O.model_f(pos, types, occ, bdw, cf, eck, vi, inc, rten)
+    I = (np.abs(F) ** 2).ravel(order="F")
+    T = I * (1.0 + 0.3 * rng.standard_normal(I.size))
+    w = rng.uniform(0.5, 2.0, I.size)
+    g = O.model_grad(pos, types, occ, bdw, cf, eck, vi, inc, rten, F, 2.0 * w * (I - T))
+    n = len(types)
+    atoms = np.repeat(np.arange(n), 5)
+    kinds = np.tile(np.arange(5), n)
+    fd = O.model_fd(pos, types, occ, bdw, cf, eck, vi, inc, rten, F, T, w, atoms, kinds, h=1e-6).reshape(n, 5)
+    for k in range(5):
+        scale = np.abs(fd[:, k]).max()
+        assert np.abs(g[:, k] - fd[:, k]).max() < 1e-6 * scale, k
+    assert np.all(g[3] == 0.0)                             # the vacancy
