@@ -36,6 +36,7 @@ EXPORTS = [
     "dc_get_precision", "dc_set_gemm_tile", "dc_gemm_num_tiles", "dc_gemm_tile_shape", "dc_set_cancel_flag", "dc_version", "dc_device_info", "dc_strucf",
     "dc_crystal_set", "dc_four_run", "dc_four_run_dev", "dc_last_stats", "dc_synchronize",
     "dc_model_intensity", "dc_model_vjp", "dc_model_chi2",
+    "dc_batch_chi2", "dc_set_autotune", "dc_set_graphs", "dc_batch_info",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -64,6 +65,12 @@ class ModelDesc(C.Structure):
     _fields_ = [("pos", _dp), ("type", _ip), ("occ", _dp), ("bdw", _dp), ("natoms", C.c_int64),
                 ("nscat", C.c_int), ("ncfact", C.c_int), ("cfact", _dp), ("eck", C.c_double * 12),
                 ("vi", C.c_double * 9), ("inc", C.c_int * 3), ("rten", C.c_double * 9)]
+
+
+class BatchDesc(C.Structure):
+    _fields_ = [("nmembers", C.c_int), ("atom_beg", C.POINTER(C.c_int64)), ("pos", _dp), ("type", _ip),
+                ("occ", _dp), ("bdw", _dp), ("nscat", C.c_int), ("ncfact", C.c_int), ("cfact", _dp),
+                ("rten", _dp), ("eck", C.c_double * 12), ("vi", C.c_double * 9), ("inc", C.c_int * 3)]
 
 
 _lib = None
@@ -101,6 +108,10 @@ def lib():
     L.dc_model_intensity.argtypes = [C.c_void_p, C.POINTER(ModelDesc), _dp, _dp]
     L.dc_model_vjp.argtypes = [C.c_void_p, C.POINTER(ModelDesc), _dp, _dp, _dp]
     L.dc_model_chi2.argtypes = [C.c_void_p, C.POINTER(ModelDesc), _dp, _dp, _dp, _dp, _dp]
+    L.dc_batch_chi2.argtypes = [C.c_void_p, C.POINTER(BatchDesc), _dp, _dp, _dp, _dp, _dp]
+    L.dc_set_autotune.argtypes = [C.c_void_p, C.c_int]
+    L.dc_set_graphs.argtypes = [C.c_void_p, C.c_int]
+    L.dc_batch_info.argtypes = [C.c_void_p, _ip, _ip, _ip, _dp]
     _lib = L
     return L
 
@@ -363,3 +374,59 @@ class Context:
         chi = C.c_double()
         self.check(lib().dc_model_chi2(self.h, C.byref(desc), _d(t), _d(w), C.byref(chi), _d(icalc), _d(grad)))
         return chi.value, icalc.reshape(num, order="F"), None if grad is None else grad.reshape(-1, 5)
+
+    # -- population scoring ----------------------------------------------------
+    def set_autotune(self, on):
+        self.check(lib().dc_set_autotune(self.h, int(bool(on))))
+
+    def set_graphs(self, on):
+        self.check(lib().dc_set_graphs(self.h, int(bool(on))))
+
+    def batch_info(self):
+        t, n, g, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        self.check(lib().dc_batch_info(self.h, C.byref(t), C.byref(n), C.byref(g), C.byref(ms)))
+        return {"tile": t.value, "nwg": n.value, "graph": bool(g.value), "tune_ms": ms.value}
+
+    @staticmethod
+    def batch_desc(atom_beg, pos, types, occ, bdw, cfact, rten, eck, vi, inc):
+        """dc_batch_desc: atom_beg (M+1), pos (N,3), types/occ/bdw (N) member
+        after member, cfact (ncfact, nscat, M) complex, rten (M, 3, 3)."""
+        keep = []
+        d = BatchDesc()
+        ab = np.ascontiguousarray(np.asarray(atom_beg, np.int64))
+        p = np.ascontiguousarray(np.asarray(pos, np.float64).reshape(-1, 3))
+        t = np.ascontiguousarray(np.asarray(types, np.int32).ravel())
+        o = np.ascontiguousarray(np.asarray(occ, np.float64).ravel())
+        keep += [ab, p, t, o]
+        d.nmembers = ab.size - 1
+        d.atom_beg = ab.ctypes.data_as(C.POINTER(C.c_int64))
+        d.pos, d.type, d.occ = _d(p), _i(t), _d(o)
+        if bdw is not None:
+            b = np.ascontiguousarray(np.asarray(bdw, np.float64).ravel())
+            keep.append(b)
+            d.bdw = _d(b)
+        cf = np.asarray(cfact, np.complex128)
+        cff = np.ascontiguousarray(cf.ravel(order="F")).view(np.float64)
+        rt = np.ascontiguousarray(np.asarray(rten, np.float64).reshape(-1, 3, 3).transpose(0, 2, 1)).ravel()
+        keep += [cff, rt]
+        d.cfact, d.rten = _d(cff), _d(rt)
+        d.ncfact, d.nscat = cf.shape[0], cf.shape[1]
+        d.eck[:] = list(_f64(eck, 12))
+        d.vi[:] = list(_f64(vi, 9))
+        d.inc[:] = [int(v) for v in inc]
+        return d, keep
+
+    def batch_chi2(self, desc, target, weight=None, want_icalc=False, want_grad=True):
+        """(chi2 (M,), I (P, M) or None, grad (N, 5) or None); target (P, M)
+        with P in Fortran order of inc."""
+        M = desc.nmembers
+        P = int(np.prod(list(desc.inc)))
+        t = np.asfortranarray(np.asarray(target, np.float64).reshape(P, M, order="F"))
+        w = None if weight is None else np.asfortranarray(np.asarray(weight, np.float64).reshape(P, M, order="F"))
+        chi = np.zeros(M)
+        ic = np.zeros((P, M), order="F") if want_icalc else None
+        n = int(desc.atom_beg[M])
+        g = np.zeros(5 * n) if want_grad else None
+        self.check(lib().dc_batch_chi2(self.h, C.byref(desc), _d(t), None if w is None else _d(w), _d(chi),
+                                       None if ic is None else _d(ic), None if g is None else _d(g)))
+        return chi, ic, None if g is None else g.reshape(n, 5)

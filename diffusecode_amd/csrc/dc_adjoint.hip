@@ -21,6 +21,8 @@
 // and the column weights (1, j - jc, m - mc, s_c^2).  h_p,a is affine in
 // (r, j, m), so the three row weights and four column weights give every
 // sum above; the weights are centred to keep the FP32 terms small.
+#include <algorithm>
+
 #include "dc_internal.h"
 
 namespace dc {
@@ -52,8 +54,8 @@ __device__ __forceinline__ float2 cis(uint64_t ph) {
 }
 
 // A of stage rows r0..r0+KR-1 (one element per thread) and the Q rows.
-__device__ __forceinline__ void load_stage(AdSmem &sm, int buf, const AdjArgs &a, int r0, int col0, int na,
-                                           long long qbase) {
+__device__ __forceinline__ void load_stage(AdSmem &sm, int buf, const AdjArgs &a, const float *__restrict__ sq,
+                                           int r0, int col0, int na, long long qbase) {
     const int t = threadIdx.x;
     {
         const int at = t & (NA - 1), rr = t / NA;      // NT == NA * KR
@@ -62,7 +64,7 @@ __device__ __forceinline__ void load_stage(AdSmem &sm, int buf, const AdjArgs &a
         if (r < a.R && at < na) {
             const AtomRec &ar = sm.at[at];
             v = cis((uint64_t)r * ar.ra);
-            const float d = __builtin_amdgcn_exp2f(-ar.pad * LOG2E * a.sq[r]);
+            const float d = __builtin_amdgcn_exp2f(-ar.pad * LOG2E * sq[r]);
             v = make_float2(v.x * d, v.y * d);
         }
         sm.a[buf][2 * rr][at] = v.x;
@@ -75,7 +77,7 @@ __device__ __forceinline__ void load_stage(AdSmem &sm, int buf, const AdjArgs &a
         const int r = r0 + rr, col = col0 + cc;
         sm.q[buf][rr][cc] = (r < a.R && col < a.C) ? a.q[qbase + col + (long long)a.C * r] : make_float2(0.f, 0.f);
     }
-    if (t < KR) sm.sr[buf][t] = r0 + t < a.R ? a.sq[r0 + t] : 0.0f;
+    if (t < KR) sm.sr[buf][t] = r0 + t < a.R ? sq[r0 + t] : 0.0f;
 }
 
 __global__ void __launch_bounds__(NT) adjoint_kernel(AdjArgs a) {
@@ -83,6 +85,7 @@ __global__ void __launch_bounds__(NT) adjoint_kernel(AdjArgs a) {
     const int4 tile = a.tiles[blockIdx.y];
     const int atom0 = tile.x, na = tile.y - tile.x;
     const long long qbase = (long long)tile.z * a.plane;
+    const float *sq = a.sq + (long long)tile.w * a.sq_stride;
     const int col0 = blockIdx.x * NC;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
     const int wa0 = (wave % NWA) * 16 * WA, wc0 = (wave / NWA) * 16 * WC;
@@ -106,11 +109,11 @@ __global__ void __launch_bounds__(NT) adjoint_kernel(AdjArgs a) {
             }
 
     const int nst = (a.R + KR - 1) / KR;
-    load_stage(sm, 0, a, 0, col0, na, qbase);
+    load_stage(sm, 0, a, sq, 0, col0, na, qbase);
     __syncthreads();
     for (int s = 0; s < nst; ++s) {
         const int buf = s & 1;
-        if (s + 1 < nst) load_stage(sm, buf ^ 1, a, (s + 1) * KR, col0, na, qbase);
+        if (s + 1 < nst) load_stage(sm, buf ^ 1, a, sq, (s + 1) * KR, col0, na, qbase);
         const bool im = (lg & 1) != 0;
 #pragma unroll
         for (int ks = 0; ks < KR / 2; ++ks) {
@@ -155,7 +158,7 @@ __global__ void __launch_bounds__(NT) adjoint_kernel(AdjArgs a) {
         const int col = col0 + wc0 + 16 * j + li;
         const int jj = col % a.C0, mm = col / a.C0;
         const bool valid = col < a.C && jj < a.f1;
-        const float sc = valid ? a.sq[a.R + col] : 0.0f;
+        const float sc = valid ? sq[a.R + col] : 0.0f;
         const float wj = (float)jj - a.jc, wm = (float)mm - a.mc;
 #pragma unroll
         for (int i = 0; i < WA; ++i)
@@ -211,9 +214,10 @@ __global__ void adjoint_finish_kernel(AdjFinishArgs a) {
         const float *p = a.part + ((long long)t * a.natoms + n) * 6;
         for (int k = 0; k < 6; ++k) s[k] += (double)p[k];
     }
-    const double o = a.occ[n];
+    const long long ci = a.idx[n];
+    const double o = a.occ[ci];
     const double twopi = 6.283185307179586476925;
-    double *g = a.grad + 5 * (long long)a.idx[n];
+    double *g = a.grad + 5 * ci;
     for (int c = 0; c < 3; ++c)
         g[c] = -twopi * o * (a.hc[c] * s[1] + a.v[c] * s[2] + a.v[3 + c] * s[3] + a.v[6 + c] * s[4]);
     g[3] = s[0];
@@ -258,7 +262,75 @@ __global__ void chi2_resid_kernel(const double *icalc, const double *tgt, const 
     if (threadIdx.x == 0) blocksum[blockIdx.x] = red[0];
 }
 
+// One member per blockIdx.y; grid-stride over the member's pixels.
+__global__ void batch_reduce_kernel(BatchReduceArgs a) {
+    __shared__ double red[256];
+    const int m = blockIdx.y;
+    Grid g = a.g;
+    for (int k = 0; k < 9; ++k) g.rten[k] = a.rten[9 * m + k];
+    const long long P = g.P;
+    const int s0 = a.mslice[m], s1 = a.mslice[m + 1];
+    const double2 *cf = a.cfact + (long long)m * a.nscat * a.ncfact;
+    double acc = 0.0;
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(p % g.f[0]);
+        const long long q = p / g.f[0];
+        const int j = (int)(q % g.f[1]), l = (int)(q / g.f[1]);
+        int bin = istl_point(g, i, j, l);
+        bin = bin >= a.ncfact ? a.ncfact - 1 : bin;
+        const long long po = i * g.ps[0] + j * g.ps[1] + l * g.ps[2];
+        double fr = 0.0, fi = 0.0;
+        for (int s = s0; s < s1; ++s) {
+            const float2 v = a.partial[(long long)s * g.plane + po];
+            const double2 u = cf[(long long)a.slice_type[s] * a.ncfact + bin];
+            fr += u.x * (double)v.x - u.y * (double)v.y;
+            fi += u.x * (double)v.y + u.y * (double)v.x;
+        }
+        const long long mp = (long long)m * P + p;
+        const double I = fr * fr + fi * fi;
+        if (a.icalc) a.icalc[mp] = I;
+        const double w = a.weight ? a.weight[mp] : 1.0;
+        const double d = I - a.target[mp];
+        acc += w * d * d;
+        if (a.q) {
+            const double g2 = 4.0 * w * d;                       // 2 G, G = 2 w (I - T)
+            const double rr = g2 * fr, ri = -g2 * fi;
+            const long long o = (long long)j + (long long)a.C0 * l + (long long)a.C * i;
+            for (int t = 0; t < a.nscat; ++t) {
+                const double2 u = cf[(long long)t * a.ncfact + bin];
+                a.q[((long long)m * a.nscat + t) * a.plane + o] =
+                    make_float2((float)(rr * u.x - ri * u.y), (float)(rr * u.y + ri * u.x));
+            }
+        }
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.chi_part[(long long)m * gridDim.x + blockIdx.x] = red[0];
+}
+
+__global__ void fill_zero_kernel(double *p, long long n) {
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x)
+        p[k] = 0.0;
+}
+
 }  // namespace
+
+hipError_t launch_batch_reduce(const BatchReduceArgs &a, int nblocks, hipStream_t st) {
+    if (a.nmembers <= 0) return hipSuccess;
+    batch_reduce_kernel<<<dim3(nblocks, a.nmembers), 256, 0, st>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_zero(double *p, long long n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const long long nb = std::min<long long>((n + 255) / 256, 4096);
+    fill_zero_kernel<<<(unsigned)nb, 256, 0, st>>>(p, n);
+    return hipGetLastError();
+}
 
 int adjoint_tile_atoms() { return NA; }
 int adjoint_tile_cols() { return NC; }

@@ -254,6 +254,7 @@ void dc_destroy(dc_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->st);
+    batch_release(c);
     for (DevBuf *b : c->buffers())
         if (b->p) (void)hipFree(b->p);
     for (auto &e : c->ev)

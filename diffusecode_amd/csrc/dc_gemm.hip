@@ -23,8 +23,8 @@
 // (RT + CT) x 16 complex products instead of (RT + CT) x 16 sincos.
 //
 // The complex product is embedded as a real GEMM with the atom dimension
-// interleaved (re, im):  Cr = [Ar -Ai].[Br Bi],  Ci = [Ai Ar].[Br Bi]; the
-// consumer forms both A variants from one (re, im) load by sign/half swaps.
+// interleaved (re, im):  Cr = [Ar -Ai].[Br Bi],  Ci = [Ai Ar].[Br Bi]; each
+// wave forms both A variants from one (re, im) load by sign/half swaps.
 //
 // Schemes (template SCHEME):
 //   SCH_FP32   v_mfma_f32_16x16x4_f32, FP32 operands: the plain drop-in.
@@ -34,9 +34,9 @@
 //              products ah*b + al*b.
 //
 // Pipeline: LDS holds two element buffers and two table buffers.  In
-// iteration s the producer waves build the elements of stage s+1 and the
-// tables of stage s+2 while the consumer waves run the MFMAs of stage s; one
-// barrier per stage.
+// iteration s every wave builds its share of the elements of stage s+1 and
+// of the tables of stage s+2, then runs its MFMAs of stage s; one barrier per
+// stage.  The atom records of stage s+3 are loaded during stage s.
 //
 // Output: partial[slice * P + r + R * c] (float2), one plane per slice
 // (a slice = a contiguous atom range of one type of one lot).
@@ -89,7 +89,7 @@ struct TileCfg {
 
 // Packed planes hold one u32 (re | im << 16) per (row, atom): 16 words per
 // row, the four 16-byte atom quads of a row XOR-swizzled by bits 2..3 of the
-// row so that 16-byte accesses of the producers (4 atom quads x 4 rows) and
+// row so that 16-byte accesses of the element builders (4 atom quads x 4 rows) and
 // of the MFMA operand reads (16 rows, one quad) fall on distinct banks.
 __device__ __forceinline__ int pk_off(int row, int quad) { return row * 16 + 4 * (quad ^ ((row >> 2) & 3)); }
 // FP32 planes: (row, k') with k' = 2*atom + (0 re, 1 im), 36 floats per row.
@@ -348,88 +348,96 @@ __device__ __forceinline__ void mfma_stage(const Elems<SCHEME, T> &el, f32x4 (&a
 template <int SCHEME, class T, bool DAMP>
 __global__ void __launch_bounds__(T::NT)
 phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice_beg, float2 *__restrict__ partial,
-                  const float *__restrict__ sq, GemmGeom g, int slice0) {
+                  const float *__restrict__ sq, const int *__restrict__ slice_sq, GemmGeom g, int slice0,
+                  int nitems) {
     __shared__ Smem<SCHEME, T> sm;
-    const int slice = slice0 + blockIdx.z;
-    const int kbeg = slice_beg[slice], kend = slice_beg[slice + 1];
-    const int row0 = blockIdx.y * T::RT, col0 = blockIdx.x * T::CT;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wr0 = (wave % T::NWR) * 16 * T::WR, wc0 = (wave / T::NWR) * 16 * T::WC;
-    const bool live = g.R - row0 - wr0 > 0 && g.C - col0 - wc0 > 0;   // this wave's block touches the grid
-    const int pt = threadIdx.x;
+    const int nct = (g.C + T::CT - 1) / T::CT, nrt = (g.R + T::RT - 1) / T::RT;
+    // work items (column tile fastest, then row tile, then slice); a launch
+    // with fewer work-groups than items walks them with a grid stride
+    for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
+        const int slice = slice0 + item / (nct * nrt);
+        const int kbeg = slice_beg[slice], kend = slice_beg[slice + 1];
+        const int row0 = ((item / nct) % nrt) * T::RT, col0 = (item % nct) * T::CT;
+        const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        const int wr0 = (wave % T::NWR) * 16 * T::WR, wc0 = (wave / T::NWR) * 16 * T::WC;
+        const bool live = g.R - row0 - wr0 > 0 && g.C - col0 - wc0 > 0;   // this wave's block touches the grid
+        const int pt = threadIdx.x;
 
-    Side sa{row0, row0 % g.R0, row0 / g.R0, g.R0};
-    Side sb{col0, col0 % g.C0, col0 / g.C0, g.C0};
-    // wrap bookkeeping of the 16-groups of the tile (periods >= 16 when a side wraps)
-    for (int h = threadIdx.x; h < T::RH + T::CH; h += T::NT) {
-        const bool isa = h < T::RH;
-        const Side &s = isa ? sa : sb;
-        const int hh = isa ? h : h - T::RH;
-        const int x = s.a0 + 16 * hh;
-        (isa ? sm.a_k16 : sm.b_k16)[hh] = x / s.period;
-        (isa ? sm.a_wpos : sm.b_wpos)[hh] = s.period - (x % s.period);   // first wrap inside the group
-    }
-    if constexpr (DAMP) {
-        // sq = [s^2 of the R rows | s^2 of the C columns]; padding reads 0
-        for (int x = threadIdx.x; x < T::RT + T::CT; x += T::NT) {
-            const bool isa = x < T::RT;
-            const int o = isa ? row0 + x : col0 + x - T::RT;
-            const int lim = isa ? g.R : g.C;
-            (isa ? sm.s_a[x] : sm.s_b[x - T::RT]) = o < lim ? sq[(isa ? 0 : g.R) + o] : 0.0f;
+        Side sa{row0, row0 % g.R0, row0 / g.R0, g.R0};
+        Side sb{col0, col0 % g.C0, col0 / g.C0, g.C0};
+        // wrap bookkeeping of the 16-groups of the tile (periods >= 16 when a side wraps)
+        for (int h = threadIdx.x; h < T::RH + T::CH; h += T::NT) {
+            const bool isa = h < T::RH;
+            const Side &s = isa ? sa : sb;
+            const int hh = isa ? h : h - T::RH;
+            const int x = s.a0 + 16 * hh;
+            (isa ? sm.a_k16 : sm.b_k16)[hh] = x / s.period;
+            (isa ? sm.a_wpos : sm.b_wpos)[hh] = s.period - (x % s.period);   // first wrap inside the group
         }
-    }
-
-    f32x4 accR[T::WR][T::WC], accI[T::WR][T::WC];
-#pragma unroll
-    for (int i = 0; i < T::WR; ++i)
-#pragma unroll
-        for (int j = 0; j < T::WC; ++j) {
-            accR[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-            accI[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-        }
-
-    const int nst = (kend - kbeg + KS - 1) / KS;
-    // records of this thread's atom for the stages whose tables come next,
-    // loaded one stage ahead so the global-memory latency is hidden
-    const int at = pt & (KS - 1);
-    auto fetch = [&](int st) {
-        const int k = kbeg + st * KS + at;
-        return rec[k < kend ? k : kend - 1];
-    };
-    const AtomRec r0 = fetch(0);
-    const AtomRec r1 = fetch(nst > 1 ? 1 : 0);
-    AtomRec rn = fetch(nst > 2 ? 2 : 0);
-    __syncthreads();
-    build_tables<T>(sm.t[0], sm.a_k16, sm.b_k16, r0, kbeg + at < kend, sa, sb, pt);
-    __syncthreads();
-    build_elems<SCHEME, T, DAMP>(sm.e[0], sm.t[0], sm.a_wpos, sm.b_wpos, sm.s_a, sm.s_b, wave);
-    if (nst > 1) build_tables<T>(sm.t[1], sm.a_k16, sm.b_k16, r1, kbeg + KS + at < kend, sa, sb, pt);
-    __syncthreads();
-    for (int s = 0; s < nst; ++s) {
-        const int cur = s & 1, nxt = cur ^ 1;
-        const AtomRec rt = rn;
-        if (s + 3 < nst) rn = fetch(s + 3);
-        if (s + 1 < nst) build_elems<SCHEME, T, DAMP>(sm.e[nxt], sm.t[nxt], sm.a_wpos, sm.b_wpos, sm.s_a, sm.s_b, wave);
-        if (s + 2 < nst) build_tables<T>(sm.t[cur], sm.a_k16, sm.b_k16, rt, kbeg + (s + 2) * KS + at < kend, sa, sb, pt);
-        if (live) mfma_stage<SCHEME, T>(sm.e[cur], accR, accI, wr0, wc0);
-        __syncthreads();
-    }
-
-    // epilogue: lane holds rows 4*lg + v (v = 0..3) of column li of each 16x16 tile
-    const int lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
-    float2 *out = partial + (long long)(slice - slice0) * g.P;
-#pragma unroll
-    for (int i = 0; i < T::WR; ++i)
-#pragma unroll
-        for (int j = 0; j < T::WC; ++j) {
-            const int c = col0 + wc0 + 16 * j + li;
-            if (c >= g.C) continue;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int r = row0 + wr0 + 16 * i + 4 * lg + v;
-                if (r < g.R) out[r + (long long)g.R * c] = make_float2(accR[i][j][v], accI[i][j][v]);
+        if constexpr (DAMP) {
+            // sq = [s^2 of the R rows | s^2 of the C columns] of the slice's
+            // metric (slice_sq: offset per slice); padding reads 0
+            const float *sqs = sq + (slice_sq ? slice_sq[slice] : 0);
+            for (int x = threadIdx.x; x < T::RT + T::CT; x += T::NT) {
+                const bool isa = x < T::RT;
+                const int o = isa ? row0 + x : col0 + x - T::RT;
+                const int lim = isa ? g.R : g.C;
+                (isa ? sm.s_a[x] : sm.s_b[x - T::RT]) = o < lim ? sqs[(isa ? 0 : g.R) + o] : 0.0f;
             }
         }
+
+        f32x4 accR[T::WR][T::WC], accI[T::WR][T::WC];
+#pragma unroll
+        for (int i = 0; i < T::WR; ++i)
+#pragma unroll
+            for (int j = 0; j < T::WC; ++j) {
+                accR[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+                accI[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+            }
+
+        const int nst = (kend - kbeg + KS - 1) / KS;
+        // records of this thread's atom for the stages whose tables come next,
+        // loaded one stage ahead so the global-memory latency is hidden
+        const int at = pt & (KS - 1);
+        auto fetch = [&](int st) {
+            const int k = kbeg + st * KS + at;
+            return rec[k < kend ? k : kend - 1];
+        };
+        const AtomRec r0 = fetch(0);
+        const AtomRec r1 = fetch(nst > 1 ? 1 : 0);
+        AtomRec rn = fetch(nst > 2 ? 2 : 0);
+        __syncthreads();
+        build_tables<T>(sm.t[0], sm.a_k16, sm.b_k16, r0, kbeg + at < kend, sa, sb, pt);
+        __syncthreads();
+        build_elems<SCHEME, T, DAMP>(sm.e[0], sm.t[0], sm.a_wpos, sm.b_wpos, sm.s_a, sm.s_b, wave);
+        if (nst > 1) build_tables<T>(sm.t[1], sm.a_k16, sm.b_k16, r1, kbeg + KS + at < kend, sa, sb, pt);
+        __syncthreads();
+        for (int s = 0; s < nst; ++s) {
+            const int cur = s & 1, nxt = cur ^ 1;
+            const AtomRec rt = rn;
+            if (s + 3 < nst) rn = fetch(s + 3);
+            if (s + 1 < nst) build_elems<SCHEME, T, DAMP>(sm.e[nxt], sm.t[nxt], sm.a_wpos, sm.b_wpos, sm.s_a, sm.s_b, wave);
+            if (s + 2 < nst) build_tables<T>(sm.t[cur], sm.a_k16, sm.b_k16, rt, kbeg + (s + 2) * KS + at < kend, sa, sb, pt);
+            if (live) mfma_stage<SCHEME, T>(sm.e[cur], accR, accI, wr0, wc0);
+            __syncthreads();
+        }
+
+        // epilogue: lane holds rows 4*lg + v (v = 0..3) of column li of each 16x16 tile
+        const int lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
+        float2 *out = partial + (long long)(slice - slice0) * g.P;
+#pragma unroll
+        for (int i = 0; i < T::WR; ++i)
+#pragma unroll
+            for (int j = 0; j < T::WC; ++j) {
+                const int c = col0 + wc0 + 16 * j + li;
+                if (c >= g.C) continue;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = row0 + wr0 + 16 * i + 4 * lg + v;
+                    if (r < g.R) out[r + (long long)g.R * c] = make_float2(accR[i][j][v], accI[i][j][v]);
+                }
+            }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -450,39 +458,44 @@ int gemm_num_tiles() { return (int)(sizeof(kTiles) / sizeof(kTiles[0])); }
 TileInfo gemm_tile_info(int t) { return kTiles[t]; }
 
 template <int SCHEME, class T>
-static void launch_one(const AtomRec *rec, const int *slice_beg, float2 *partial, const float *sq, const GemmGeom &g,
-                       int slice0, int nslice, hipStream_t st) {
-    dim3 grid((g.C + T::CT - 1) / T::CT, (g.R + T::RT - 1) / T::RT, nslice);
+static void launch_one(const AtomRec *rec, const int *slice_beg, float2 *partial, const float *sq, const int *slice_sq,
+                       const GemmGeom &g, int slice0, int nslice, int nwg, hipStream_t st) {
+    const long long items = (long long)((g.C + T::CT - 1) / T::CT) * ((g.R + T::RT - 1) / T::RT) * nslice;
+    const int grid = (int)(nwg > 0 && nwg < items ? nwg : items);
     if (sq)
-        phase_gemm_kernel<SCHEME, T, true><<<grid, dim3(T::NT), 0, st>>>(rec, slice_beg, partial, sq, g, slice0);
+        phase_gemm_kernel<SCHEME, T, true><<<grid, dim3(T::NT), 0, st>>>(rec, slice_beg, partial, sq, slice_sq, g, slice0, (int)items);
     else
-        phase_gemm_kernel<SCHEME, T, false><<<grid, dim3(T::NT), 0, st>>>(rec, slice_beg, partial, sq, g, slice0);
+        phase_gemm_kernel<SCHEME, T, false><<<grid, dim3(T::NT), 0, st>>>(rec, slice_beg, partial, sq, slice_sq, g, slice0, (int)items);
 }
 
 template <int SCHEME>
 static hipError_t launch_scheme(int tile, const AtomRec *rec, const int *slice_beg, float2 *partial, const float *sq,
-                                const GemmGeom &g, int slice0, int nslice, hipStream_t st) {
+                                const int *slice_sq, const GemmGeom &g, int slice0, int nslice, int nwg,
+                                hipStream_t st) {
     switch (tile) {
-    case 0: launch_one<SCHEME, CfgA>(rec, slice_beg, partial, sq, g, slice0, nslice, st); break;
-    case 1: launch_one<SCHEME, CfgB>(rec, slice_beg, partial, sq, g, slice0, nslice, st); break;
-    case 2: launch_one<SCHEME, CfgC>(rec, slice_beg, partial, sq, g, slice0, nslice, st); break;
-    case 3: launch_one<SCHEME, CfgD>(rec, slice_beg, partial, sq, g, slice0, nslice, st); break;
-    case 4: launch_one<SCHEME, CfgE>(rec, slice_beg, partial, sq, g, slice0, nslice, st); break;
-    case 5: launch_one<SCHEME, CfgF>(rec, slice_beg, partial, sq, g, slice0, nslice, st); break;
+    case 0: launch_one<SCHEME, CfgA>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st); break;
+    case 1: launch_one<SCHEME, CfgB>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st); break;
+    case 2: launch_one<SCHEME, CfgC>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st); break;
+    case 3: launch_one<SCHEME, CfgD>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st); break;
+    case 4: launch_one<SCHEME, CfgE>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st); break;
+    case 5: launch_one<SCHEME, CfgF>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
 hipError_t launch_phase_gemm(int scheme, int tile, const AtomRec *rec, const int *slice_beg, float2 *partial,
-                             const float *sq, const GemmGeom &g, int slice0, int nslice, hipStream_t st) {
+                             const float *sq, const GemmGeom &g, int slice0, int nslice, hipStream_t st, int nwg,
+                             const int *slice_sq) {
     // the two-level operand tables need wrap periods of at least 16
     if (g.R0 < 16 && g.R0 != g.R) return hipErrorInvalidValue;
     if (g.C0 < 16 && g.C0 != g.C) return hipErrorInvalidValue;
+    if (nslice <= 0) return hipSuccess;
+    if ((long long)((g.C + 15) / 16) * ((g.R + 15) / 16) * nslice > INT32_MAX) return hipErrorInvalidValue;
     switch (scheme) {
-    case SCH_FP32: return launch_scheme<SCH_FP32>(tile, rec, slice_beg, partial, sq, g, slice0, nslice, st);
-    case SCH_BF16X3: return launch_scheme<SCH_BF16X3>(tile, rec, slice_beg, partial, sq, g, slice0, nslice, st);
-    case SCH_FP16X2: return launch_scheme<SCH_FP16X2>(tile, rec, slice_beg, partial, sq, g, slice0, nslice, st);
+    case SCH_FP32: return launch_scheme<SCH_FP32>(tile, rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st);
+    case SCH_BF16X3: return launch_scheme<SCH_BF16X3>(tile, rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st);
+    case SCH_FP16X2: return launch_scheme<SCH_FP16X2>(tile, rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -8,6 +8,8 @@
 #include "../../include/diffuse_turbo.h"
 #include "dc_internal.h"
 
+struct BatchState;   // dc_batch.hip
+
 struct DevBuf {
     void *p = nullptr;
     size_t n = 0;
@@ -18,6 +20,9 @@ struct dc_ctx {
     hipStream_t st = nullptr;
     int prec = DC_PREC_FP32;
     int tile = -1;                  // forced GEMM tile configuration, -1 = planned
+    int autotune = 1;               // batch: time tile shapes on the first call of a shape
+    int graphs = 1;                 // batch: capture the population step as a hipGraph
+    BatchState *batch = nullptr;    // cached plan / buffers / graph of the last batch shape
     std::string err;
     int err_typ = DC_ER_NONE;
     const volatile int *cancel = nullptr;
@@ -126,6 +131,7 @@ Plan plan_gemm(Grid &g, int forced, int only_mapping = -1);
 int kchunk_for(const Plan &pl, long long natoms_batch);
 void add_segments(SliceSet &s, const std::vector<std::pair<int, int>> &segs, int kchunk, int &atom_off);
 int run_gemm(dc_ctx *c, const Plan &pl, int nslice, float2 *partial, const float *sq = nullptr);
+void batch_release(dc_ctx *c);   // frees the cached batch plan, buffers and graph
 PrepArgs base_prep(const Grid &g, int mapping);
 
 }  // namespace dc

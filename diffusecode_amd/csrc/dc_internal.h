@@ -16,6 +16,8 @@ struct PrepArgs {
     const double *offs;     // offset table (3 per id)
     const float *w;         // per-atom amplitude (null: 1)
     const float *bdw;       // per-atom isotropic B in AtomRec::pad (null: 0)
+    const double *occ_src;  // caller-order occupancy read through idx (overrides w)
+    const double *bdw_src;  // caller-order B read through idx (overrides bdw)
     long long n;
     int mode;               // PREP_CORNER / PREP_LMN
     int mapping;            // 0: rows = abscissa ; 1: rows = (abscissa, ordinate)
@@ -82,8 +84,12 @@ struct FinalArgs {
 
 // sq (R + C floats: s^2 parts of rows and columns) selects the variant with a
 // per-atom displacement factor exp(-B_n s^2), B_n in AtomRec::pad; null = none.
+// nwg > 0 launches that many persistent work-groups that stride over the
+// (slice, tile) work items; 0 = one work-group per item.  slice_sq (absolute
+// slice ids) offsets sq per slice when slices use different metrics.
 hipError_t launch_phase_gemm(int scheme, int tile, const AtomRec *rec, const int *slice_beg, float2 *partial,
-                             const float *sq, const GemmGeom &g, int slice0, int nslice, hipStream_t st);
+                             const float *sq, const GemmGeom &g, int slice0, int nslice, hipStream_t st,
+                             int nwg = 0, const int *slice_sq = nullptr);
 int gemm_num_tiles();
 TileInfo gemm_tile_info(int t);
 hipError_t launch_prep(const PrepArgs &a, AtomRec *out, hipStream_t st);
@@ -103,9 +109,10 @@ hipError_t launch_istl(const Grid &g, int *out, hipStream_t st);
 // [Re S0, Im S0, Im S_r, Im S_j, Im S_m, Re S_s] to part[tile][atom][6].
 struct AdjArgs {
     const AtomRec *rec;     // type-grouped atoms (mapping 0 records, B_n in pad)
-    const int4 *tiles;      // atom tiles: (first, end, Q plane, 0), each inside one type
+    const int4 *tiles;      // atom tiles: (first, end, Q plane, member), each inside one type
     const float2 *q;        // Q planes: q[t * plane + c + C * r] (column fastest)
-    const float *sq;        // s^2 parts: R rows, then C columns
+    const float *sq;        // s^2 parts: R rows, then C columns (per member: + tile.w * sq_stride)
+    long long sq_stride;
     float *part;            // [column tile][natoms][6]
     int R, C, C0, f1;       // rows, padded columns, column period, valid columns per period
     long long plane;        // R * C
@@ -116,7 +123,7 @@ struct AdjFinishArgs {
     const float *part;
     int ntile_c, natoms;
     const int *idx;         // sorted atom -> caller index
-    const double *occ;      // sorted occupancies
+    const double *occ;      // caller-order occupancies
     double hc[3], v[9];     // h at the weight centres; grid increments vi(:,1..3)
     double *grad;           // (5, caller atoms): d/dx, d/dy, d/dz, d/docc, d/dB
 };
@@ -132,6 +139,28 @@ struct QArgs {
     int nq;
     float2 *q;
 };
+// Batched scoring of a population of models sharing one grid (member m owns
+// GEMM slices [mslice[m], mslice[m+1]) and Q planes m*nscat .. m*nscat+nscat-1).
+// Per pixel: F_m (FP64) = sum_s cfact_m(type_s, bin_m) partial_s, I = |F|^2,
+// G = 2 w (I - T), chi^2 block sums, Q planes 2 G conj(F) cfact_m(t, bin_m).
+struct BatchReduceArgs {
+    Grid g;                 // shared grid (rten per member below), ps/plane of the GEMM
+    int R, C, C0;
+    long long plane;        // R * C (Q plane)
+    const float2 *partial;
+    const int *mslice;      // nmembers + 1
+    const int *slice_type;  // 0-based type per slice
+    const double2 *cfact;   // (ncfact, nscat, nmembers)
+    int ncfact, nscat, nmembers;
+    const double *rten;     // (9, nmembers)
+    const double *target;   // (P, nmembers)
+    const double *weight;   // (P, nmembers) or null
+    double *icalc;          // (P, nmembers) or null
+    double *chi_part;       // (nblocks, nmembers)
+    float2 *q;              // (plane, nscat, nmembers) or null
+};
+hipError_t launch_batch_reduce(const BatchReduceArgs &a, int nblocks, hipStream_t st);
+hipError_t launch_fill_zero(double *p, long long n, hipStream_t st);
 int adjoint_tile_atoms();
 int adjoint_tile_cols();
 hipError_t launch_adjoint(const AdjArgs &a, int ntiles, hipStream_t st);

@@ -51,8 +51,9 @@ __global__ void prep_records_kernel(PrepArgs a, AtomRec *__restrict__ out) {
         r.ra = inc[0]; r.ra1 = inc[1];
         r.ca = inc[2]; r.ca1 = 0;
     }
-    r.w = a.w ? a.w[k] : 1.0f;
-    r.pad = a.bdw ? a.bdw[k] : 0.0f;
+    const long long src = a.idx ? (long long)a.idx[k] : k;
+    r.w = a.occ_src ? (float)a.occ_src[src] : (a.w ? a.w[k] : 1.0f);
+    r.pad = a.bdw_src ? (float)a.bdw_src[src] : (a.bdw ? a.bdw[k] : 0.0f);
     out[k] = r;
 }
 

@@ -203,13 +203,11 @@ int model_adjoint(dc_ctx *c, const dc_model_desc *m, ModelPlan &mp) {
     }
     const int ntiles = (int)tiles.size() / 4;
     const int ntc = (gg.C + adjoint_tile_cols() - 1) / adjoint_tile_cols();
-    std::vector<double> occ(std::max(mp.nsorted, 1), 0.0);
-    for (int k = 0; k < mp.nsorted; ++k) occ[k] = m->occ[mp.order[k]];
 
     HIPCHK(hipEventRecord(c->ev[4], c->st));
     if ((st = upload(c, c->m_qt, qtype.data(), qtype.size()))) return st;
     if (ntiles > 0 && (st = upload(c, c->m_tiles, tiles.data(), tiles.size()))) return st;
-    if ((st = upload(c, c->m_tgt, occ.data(), occ.size()))) return st;
+    if ((st = upload(c, c->m_tgt, m->occ, (size_t)m->natoms))) return st;
     if ((st = ensure(c, c->m_q, sizeof(float2) * gg.P * std::max(nq, 1)))) return st;
     if ((st = ensure(c, c->m_part, sizeof(float) * 6 * (size_t)ntc * std::max(mp.nsorted, 1)))) return st;
     if ((st = ensure(c, c->m_grad, sizeof(double) * 5 * std::max<int64_t>(m->natoms, 1)))) return st;

@@ -223,6 +223,50 @@ int dc_model_vjp(dc_ctx *ctx, const dc_model_desc *model, const double *fcalc, c
 int dc_model_chi2(dc_ctx *ctx, const dc_model_desc *model, const double *target, const double *weight,
                   double *chi2, double *icalc, double *grad);
 
+/*
+ * Population scoring (DIFFEV, diffev/prog/diffev_kdo.f90:676 running one
+ * macro per member over lib_f90/with_mpi.f90:281-609): nmembers refinement
+ * models sharing one grid (eck/vi/inc in each member's reciprocal units),
+ * each with its own atoms, form factors and metric, scored by
+ *   chi2[m] = sum_h w_m(h) (I_m(h) - target_m(h))^2
+ * in one pass: one persistent phase-GEMM launch over all members' atoms, one
+ * fused reduction, and (grad != NULL) one adjoint launch.  The first call of
+ * a batch shape (member sizes, types, grid, precision, outputs requested)
+ * plans, autotunes the GEMM tile and grid size, and captures the step as a
+ * hipGraph; later calls of the same shape (new positions, occupancies, B,
+ * form factors, metrics, targets) upload and replay it.
+ * Arrays: pos (3, total), type/occ/bdw (total) member after member
+ * (atom_beg[m] .. atom_beg[m+1]-1); cfact (ncfact, nscat, nmembers);
+ * rten (9, nmembers); target/weight/icalc (P, nmembers) with P =
+ * inc(1)*inc(2)*inc(3) abscissa fastest; chi2 (nmembers); grad (5, total)
+ * as in dc_model_vjp.  Per-atom B and gradients need the abscissa
+ * metric-orthogonal to the other axes in every member's metric.
+ */
+typedef struct dc_batch_desc {
+    int nmembers;
+    const int64_t *atom_beg;    /* nmembers + 1 offsets, atom_beg[0] = 0 */
+    const double *pos;
+    const int *type;            /* 1-based within the member's nscat types, 0 = vacancy */
+    const double *occ;
+    const double *bdw;          /* NULL = all zero */
+    int nscat;
+    int ncfact;
+    const double *cfact;
+    const double *rten;
+    double eck[12];
+    double vi[9];
+    int inc[3];
+} dc_batch_desc;
+
+int dc_batch_chi2(dc_ctx *ctx, const dc_batch_desc *batch, const double *target, const double *weight,
+                  double *chi2, double *icalc, double *grad);
+/* Autotuner and hipGraph capture of batch calls (both on by default). */
+int dc_set_autotune(dc_ctx *ctx, int on);
+int dc_set_graphs(dc_ctx *ctx, int on);
+/* Choice of the last batch shape: tile configuration, persistent work-groups
+ * (0 = one per work item), whether a graph replays it, autotuned GEMM ms. */
+int dc_batch_info(const dc_ctx *ctx, int *tile, int *nwg, int *graph, double *tune_ms);
+
 #ifdef __cplusplus
 }
 #endif
