@@ -1,24 +1,37 @@
 """Benchmark of the MI355X structure-factor path.
 
-Workload (the precision configuration): fcc a = 4.0 A, 40x40x40 cells
-(256,000 atoms, 3 types) with Warren-Cowley-like short-range order from
-10^5 seeded swap-MC steps; 3D block h,k,l in [-4, 4], 161^3 points;
+Headline workload (the precision configuration): fcc a = 4.0 A, 40x40x40
+cells (256,000 atoms, 3 types) with Warren-Cowley-like short-range order
+from 10^5 seeded swap-MC steps; 3D block h,k,l in [-4, 4], 161^3 points;
 X-ray 4-Gaussian form factors.  One step = one full Fourier run
 (dc_four_run_dev: gather + exact phase reduction, phase GEMM, FP64
-reduction with form factors, Friedel mirror) with the crystal resident in HBM
-and outputs left in HBM.
+reduction with form factors, Friedel mirror) with the crystal resident in
+HBM and outputs left in HBM.
 
 Metric: atom.hkl pairs/s = sum_types N_type x 161^3 / wall time of a step,
 for the split-BF16x3 MFMA path; the plain FP32 MFMA path runs the same
 workload for the ratio, and a bounded sample is checked against the FP64
 oracle (max|dI|/max(I)).
 
+The same JSON line carries the other workloads (each timed with the
+library's HIP events on its stream):
+  lots     fcc 100^3 cells (4M atoms, 10% substitution, correlated
+           displacements), 64 periodic box lots of 8^3 cells, 8 hk-layers of
+           401^2, FP64 <F> and cross-lot variance
+  adjoint  orthorhombic 4/5/6 A, 32,768 atoms, 4 types, 65^3 block:
+           chi^2 forward and adjoint (dL/d(x,y,z,occ,B) of every atom)
+  batch    96 trial structures of log-uniform size in [2k, 60k] atoms, each
+           against its own 256^2 layer: one population step (persistent
+           GEMM + fused reduction + adjoint) replayed as a hipGraph
+
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--prec bf16x3]
+  python bench.py --calibrate-copy 1073741824   (plain copy kernel of known
+           size, for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)
 """
 import argparse
 import json
 import os
-import platform
+import re
 import subprocess
 import sys
 import time
@@ -31,6 +44,8 @@ sys.path.insert(0, ROOT)
 METRIC = "atom·hkl pairs/s, split-BF16 MFMA vs plain FP32 HIP, at ≤1e-5 rel err vs FP64"
 PEAK = {"bf16x3": 2500.0, "fp16x2": 2500.0, "fp32": 157.3}   # TFLOP/s dense
 KSPLIT = {"bf16x3": 3, "fp16x2": 2, "fp32": 1}
+HBM_PEAK = 8000.0   # GB/s
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "phase_gemm_fcc40_161cube_traffic.md")
 
 
 def log(*a):
@@ -50,6 +65,114 @@ def cpu_info():
     return info
 
 
+def traffic_from_profile(prec):
+    """HBM bytes per launch of the phase GEMM from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (already scaled by the copy-kernel
+    calibration there); None when absent."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    pat = re.compile(r"^\|\s*" + re.escape(prec) + r"\s*\|.*\|\s*([0-9.eE+]+)\s*\|\s*$")
+    for line in open(TRAFFIC_FILE):
+        m = pat.match(line.strip())
+        if m:
+            return float(m.group(1))
+    return None
+
+
+def calibrate_copy(nbytes):
+    import torch
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    log(f"copied {nbytes} bytes x 3")
+
+
+def bench_lots(_lib, X, FR, torch, prec):
+    c = X.fcc_correlated(n=100, a=4.0, subst=0.10, sigma=0.08, seed=3)
+    eck, vi, inc = FR.grid_block((-10.0, -10.0, 0.0), (10.0, 10.0, 3.5), (401, 401, 8))
+    ls = (8, 8, 8)
+    lbeg = X.lot_origins(c.icc, 64, seed=11, lperiod=True, ls=ls)
+    ntot = int(np.prod(inc))
+    dev = torch.device("cuda:0")
+    bufs = {k: torch.empty(n * ntot, dtype=torch.float64, device=dev)
+            for k, n in (("csf", 2), ("dsi", 1), ("fmean", 2), ("dvar", 1))}
+    ctx = _lib.Context(device=0, precision=prec)
+    ctx.crystal_set(c)
+    desc, keep = ctx.run_desc(eck, vi, inc, c.rten(), FR.formtab(c), ilots=2, ls=ls, lbeg=lbeg, lperiod=True,
+                              four_friedel=True)
+    stats = []
+    for it in range(4):
+        ctx.four_run_dev(desc, csf=bufs["csf"], dsi=bufs["dsi"], fmean=bufs["fmean"], dvar=bufs["dvar"])
+        torch.cuda.synchronize()
+        if it:
+            stats.append(ctx.stats())
+    ctx.close()
+    ms = float(np.median([s["total_ms"] for s in stats]))
+    gemm = float(np.median([s["gemm_ms"] for s in stats]))
+    red = float(np.median([s["reduce_ms"] for s in stats]))
+    pairs = stats[0]["pairs"]
+    # lot reduction (all lots of a batch in one pass): per pixel it reads the
+    # FP32 slice planes (8 B per slice) and reads + writes the FP64 accumulators
+    # dsi (8 B), Welford mean (16 B) and M2 (8 B), and writes csf (16 B): 80 B
+    nsl = stats[0]["slices"]
+    red_bytes = 8.0 * ntot * nsl + 80.0 * ntot
+    return {"workload": "fcc 100^3 cells (4M atoms, 2 types, 10% subst., corr. displ. 0.08 A), 64 periodic box "
+                        "lots of 8^3 cells, 8 hk-layers of 401^2 (h,k in [-10,10], l = 0..3.5)",
+            "precision": prec, "pairs": pairs, "ms": ms, "value": pairs / (ms * 1e-3), "gemm_ms": gemm,
+            "reduce_ms": red, "reduce_hbm_GBps": red_bytes / (red * 1e-3) / 1e9 if red > 0 else None,
+            "reduce_hbm_frac": red_bytes / (red * 1e-3) / 1e9 / HBM_PEAK if red > 0 else None}
+
+
+def bench_adjoint(_lib, X, FR):
+    from diffusecode_amd.refine import scattering_table
+    cr = X.orthorhombic()
+    occ, bdw = X.refinement_parameters(cr)
+    cf = scattering_table(cr.scat)
+    eck, vi, inc = FR.grid_block((-2.0, -2.0, -2.0), (2.0, 2.0, 2.0), (65, 65, 65))
+    out = {"workload": "orthorhombic 4/5/6 A, 16^3 cells x 8 sites (32,768 atoms, 4 types), 65^3 block, "
+                       "chi^2 gradient wrt 3N positions, N occupancies, N B"}
+    ctx = _lib.Context(device=0)
+    for prec in ("fp32", "bf16x3"):
+        ctx.set_precision(prec)
+        d, keep = ctx.model_desc(cr.pos, cr.iscat, occ, bdw, cf, eck, vi, inc, cr.rten())
+        icalc, _ = ctx.model_intensity(d)
+        target = icalc.ravel(order="F") * 1.01
+        rows = []
+        for it in range(4):
+            ctx.model_chi2(d, target)
+            if it:
+                rows.append(ctx.stats())
+        fwd = float(np.median([s["forward_ms"] for s in rows]))
+        adj = float(np.median([s["adjoint_ms"] for s in rows]))
+        out[prec] = {"forward_ms": fwd, "adjoint_ms": adj, "adjoint_over_forward": adj / fwd,
+                     "pairs_per_forward": rows[0]["pairs"]}
+    ctx.close()
+    return out
+
+
+def bench_batch(_lib, X, FR, prec):
+    from diffusecode_amd.population import Population
+    crs = X.population()
+    eck, vi, inc = FR.grid_block((-4.0, -4.0, 0.0), (4.0, 4.0, 0.0), (256, 256, 1))
+    pop = Population.from_crystals(crs, eck, vi, inc, precision=prec)
+    T = np.ones((pop.npixels, len(crs)), order="F")
+    out = {"workload": f"{len(crs)} members (fcc/bcc/rocksalt, log-uniform 2k-60k atoms, {pop.natoms} atoms), "
+                       "own 256^2 hk0 layer each", "precision": prec}
+    for want_grad in (False, True):
+        rows = []
+        for it in range(4):
+            pop.score(T, want_grad=want_grad)
+            if it:
+                rows.append(pop.ctx.stats())
+        ms = float(np.median([s["forward_ms"] for s in rows]))
+        key = "forward_adjoint" if want_grad else "forward"
+        out[key] = {"step_ms": ms, "pairs_per_s": rows[0]["pairs"] / (ms * 1e-3), "info": pop.ctx.batch_info()}
+    pop.ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -59,12 +182,17 @@ def main():
     ap.add_argument("--fp32-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
-    ap.add_argument("--cpu-layers", type=int, default=6)
+    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--cpu-layers", type=int, default=40)
+    ap.add_argument("--calibrate-copy", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU sharding of this path is described in DESIGN.md, not measured.", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
 
     import torch
     from diffusecode_amd import _lib, crystal as X, fourier as FR
@@ -75,7 +203,8 @@ def main():
     cf = FR.formtab(c)
     log(f"crystal {c.natoms} atoms types {c.type_counts().tolist()} built in {time.time() - t0:.1f}s")
     ntot = int(np.prod(inc))
-    pairs = int(np.count_nonzero(c.iscat > 0)) * ntot
+    natoms = int(np.count_nonzero(c.iscat > 0))
+    pairs = natoms * ntot
 
     dev = torch.device("cuda:0")
     d_dsi = torch.empty(ntot, dtype=torch.float64, device=dev)
@@ -89,7 +218,7 @@ def main():
         for _ in range(warmup):
             ctx.four_run_dev(desc, csf=d_csf, dsi=d_dsi)
         torch.cuda.synchronize()
-        gemm_ms, launches, gflops_pairs = 0.0, 0, 0.0
+        gemm_ms, launches = 0.0, 0
         t = time.perf_counter()
         for _ in range(steps):
             ctx.four_run_dev(desc, csf=d_csf, dsi=d_dsi)
@@ -114,11 +243,15 @@ def main():
     # dominant kernel: the phase GEMM; algorithmic MFMA flops = 8 x k_split per
     # computed atom.hkl pair (the GEMM computes the Friedel-reduced region)
     fnum0 = (inc[0] + 1) // 2
-    pairs_computed_per_launch = int(np.count_nonzero(c.iscat > 0)) * fnum0 * inc[1] * inc[2] / launches_per_step
+    pairs_computed_per_launch = natoms * fnum0 * inc[1] * inc[2] / launches_per_step
     achieved = 8.0 * KSPLIT[args.prec] * pairs_computed_per_launch / (gemm_launch_ms * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK[args.prec], "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK[args.prec], 4), "traffic": None,
-                "kernel": f"phase_gemm_kernel<{args.prec}>", "avg_launch_ms": round(gemm_launch_ms, 3)}
+                "frac": round(achieved / PEAK[args.prec], 4), "traffic": traffic_from_profile(args.prec),
+                "kernel": f"phase_gemm_kernel<{args.prec}>", "avg_launch_ms": round(gemm_launch_ms, 3),
+                "flops_per_launch": 8.0 * KSPLIT[args.prec] * pairs_computed_per_launch,
+                "fp32_kernel": {"avg_launch_ms": round(g32, 3),
+                                "achieved": round(8.0 * pairs_computed_per_launch / (g32 * 1e-3) / 1e12, 2),
+                                "peak": PEAK["fp32"]}}
 
     # precision check on a bounded sample against the FP64 oracle: two l-layers
     # (l = 0 and l = 1) subsampled every 4th point (h, k step 0.2)
@@ -133,6 +266,16 @@ def main():
             errs.append((np.abs(got - ref).max(), ref.max()))
         max_err = float(max(e for e, _ in errs) / max(m for _, m in errs))
         log(f"sampled max|dI|/max(I) vs FP64 oracle: {max_err:.3e}")
+    ctx.close()
+
+    extras = {}
+    if not args.no_extras:
+        extras["lots"] = bench_lots(_lib, X, FR, torch, args.prec)
+        log(f"lots: {extras['lots']['ms']:.2f} ms, {extras['lots']['value'] / 1e12:.2f} Tpairs/s")
+        extras["adjoint"] = bench_adjoint(_lib, X, FR)
+        log(f"adjoint: {json.dumps(extras['adjoint'])}")
+        extras["batch"] = bench_batch(_lib, X, FR, args.prec)
+        log(f"batch: {json.dumps(extras['batch'])}")
 
     cpu = None
     if not args.no_cpu:
@@ -142,17 +285,17 @@ def main():
         st, lmn, off = O.fourier_lmn(eck, vi, inc)
         num_s = [inc[0], inc[1], nl]
         fnum_s = [(inc[0] + 1) // 2, inc[1], nl]
-        ist = np.zeros(int(np.prod(num_s)), np.int32)
         t = time.perf_counter()
         for t1 in range(1, c.nscat + 1):
             sel = c.iscat == t1
             O.strucf_turbo(c.pos[sel], vi[:, 0], vi[:, 1], vi[:, 2], off, lmn, num_s, fnum=fnum_s,
                            nthreads=nthr)
         dt = time.perf_counter() - t
-        cpairs = int(np.count_nonzero(c.iscat > 0)) * inc[0] * inc[1] * nl
+        cpairs = natoms * inc[0] * inc[1] * nl
         cpu = {"value": cpairs / dt, "unit": "atom·hkl pairs/s", "cores": nthr, "kind": "port",
-               "sample": f"DISCUS turbo restatement (oracle/dc_oracle.c, OpenMP, Friedel-halved like DISCUS) "
-                         f"on {nl} of 161 l-layers, all 256k atoms: {cpairs:.3e} pairs in {dt:.1f}s",
+               "sample": f"DISCUS turbo restatement (oracle/dc_oracle.c, OpenMP over abscissa rows, "
+                         f"Friedel-halved like DISCUS) on {nl} of 161 l-layers of the same workload, all "
+                         f"{natoms} atoms: {cpairs:.3e} pairs in {dt:.1f}s",
                "host": cpu_info()}
         log(f"cpu turbo baseline: {cpu['value'] / 1e9:.2f} Gpairs/s on {nthr} threads")
 
@@ -167,8 +310,8 @@ def main():
         "max_rel_err_sampled": max_err,
         "roofline": roofline, "cpu_baseline": cpu,
     }
+    line.update(extras)
     print(json.dumps(line), flush=True)
-    ctx.close()
 
 
 if __name__ == "__main__":
