@@ -102,9 +102,10 @@ long long padded(long long n, long long t) { return (n + t - 1) / t * t; }
 // tables of dc_gemm.hip need at most one wrap per 16-group); padded pixels
 // are computed and never read.  The tile configuration minimises the padded
 // work of whole work-group tiles (ties: larger tiles); forced >= 0 overrides,
-// only_mapping >= 0 restricts the mapping.
+// only_mapping >= 0 restricts the mapping, and the split-only tiles are
+// skipped for the FP32 scheme.
 
-Plan plan_gemm(Grid &g, int forced, int only_mapping) {
+Plan plan_gemm(Grid &g, int forced, int only_mapping, int prec) {
     Plan best;
     double best_cost = 1e300;
     const int nt = gemm_num_tiles();
@@ -130,6 +131,7 @@ Plan plan_gemm(Grid &g, int forced, int only_mapping) {
         for (int t = 0; t < nt; ++t) {
             if (forced >= 0 && t != forced) continue;
             TileInfo ti = gemm_tile_info(t);
+            if (forced < 0 && ti.split_only && prec == DC_PREC_FP32) continue;
             double work = (double)padded(gg.R, ti.rt) * (double)padded(gg.C, ti.ct);
             double cost = work * (1.0 + 0.5 * (16.0 / ti.rt + 16.0 / ti.ct));
             if (cost < best_cost) {
@@ -367,7 +369,7 @@ int dc_strucf(dc_ctx *c, const double *xat, int64_t nxat, int64_t ld_xat, const 
         if (max_corner_bin(g) >= ncfact) return set_err(c, DC_ERR_ISTL, DC_ER_FOUR, "sin(theta)/lambda beyond table");
     }
 
-    const Plan pl = plan_gemm(g, c->tile);
+    const Plan pl = plan_gemm(g, c->tile, -1, c->prec);
     const GemmGeom &gg = pl.gg;
     const int mapping = pl.mapping;
 
@@ -622,7 +624,7 @@ int dc_four_run_dev(dc_ctx *c, const dc_run_desc *r, double *d_csf, double *d_ac
     if (max_corner_bin(g) > std::min(CFPKT, r->ncfact - 1))
         return set_err(c, DC_ERR_ISTL, DC_ER_FOUR, "sin(theta)/lambda beyond the form factor table");
 
-    const Plan pl = plan_gemm(g, c->tile);
+    const Plan pl = plan_gemm(g, c->tile, -1, c->prec);
     int st = 0;
     if ((st = upload(c, c->cfact, r->cfact, 2 * (size_t)r->ncfact * c->nscat))) return st;
 
@@ -720,7 +722,7 @@ int dc_four_run_dev(dc_ctx *c, const dc_run_desc *r, double *d_csf, double *d_ac
         for (int k = 0; k < 3; ++k) gl_full.f[k] = g.n[k];
         gl_full.P = ntot;
         gl_full.friedel = 0;
-        const Plan pll = plan_gemm(gl_full, c->tile);
+        const Plan pll = plan_gemm(gl_full, c->tile, -1, c->prec);
         GatherList gll;
         gll.segs.push_back({0, (int)(lp.size() / 3)});
         if ((st = gemm_segments(c, gl_full, pll, gll, lp.data(), ss, nslice, false))) return st;

@@ -303,8 +303,10 @@ __device__ __forceinline__ void mfma_stage(const Elems<SCHEME, T> &el, f32x4 (&a
         }
     } else {
         // operands of the whole stage in registers, then each split term sweeps
-        // all WR x WC tiles so consecutive MFMAs never share an accumulator
-        u32x4 arh[T::WR], arl[T::WR], aih[T::WR], ail[T::WR], bh[T::WC], bl[T::WC];
+        // all WR x WC tiles so consecutive MFMAs never share an accumulator;
+        // wide column blocks (WC > 4) load B per 16-column tile instead
+        constexpr bool JOUTER = T::WC > 4;
+        u32x4 arh[T::WR], arl[T::WR], aih[T::WR], ail[T::WR], bh[JOUTER ? 1 : T::WC], bl[JOUTER ? 1 : T::WC];
 #pragma unroll
         for (int i = 0; i < T::WR; ++i) {
             const int o = pk_off(wr0 + 16 * i + li, lg);
@@ -315,17 +317,43 @@ __device__ __forceinline__ void mfma_stage(const Elems<SCHEME, T> &el, f32x4 (&a
             arl[i] = l ^ 0x80000000u;
             ail[i] = (l >> 16) | (l << 16);
         }
+        constexpr int NTERM = SCHEME == SCH_BF16X3 ? 3 : 2;
+        if constexpr (JOUTER) {
 #pragma unroll
-        for (int j = 0; j < T::WC; ++j) {
+            for (int j = 0; j < T::WC; ++j) {
+                const int o = pk_off(wc0 + 16 * j + li, lg);
+                const u32x4 yh = *reinterpret_cast<const u32x4 *>(&el.b_hi[o]);
+                u32x4 yl = yh;
+                if constexpr (SCHEME == SCH_BF16X3) yl = *reinterpret_cast<const u32x4 *>(&el.b_lo[o]);
+#pragma unroll
+                for (int t = 0; t < NTERM; ++t)
+#pragma unroll
+                    for (int i = 0; i < T::WR; ++i) {
+                        const bool alo = (t == NTERM - 1);
+                        const u32x4 xr = alo ? arl[i] : arh[i];
+                        const u32x4 xi = alo ? ail[i] : aih[i];
+                        const u32x4 y = (SCHEME == SCH_BF16X3 && t == 1) ? yl : yh;
+                        if constexpr (SCHEME == SCH_BF16X3) {
+                            accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, xr), __builtin_bit_cast(bf16x8, y), accR[i][j], 0, 0, 0);
+                            accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, xi), __builtin_bit_cast(bf16x8, y), accI[i][j], 0, 0, 0);
+                        } else {
+                            accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, xr), __builtin_bit_cast(f16x8, y), accR[i][j], 0, 0, 0);
+                            accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, xi), __builtin_bit_cast(f16x8, y), accI[i][j], 0, 0, 0);
+                        }
+                    }
+            }
+            return;
+        }
+#pragma unroll
+        for (int j = 0; j < (JOUTER ? 0 : T::WC); ++j) {
             const int o = pk_off(wc0 + 16 * j + li, lg);
             bh[j] = *reinterpret_cast<const u32x4 *>(&el.b_hi[o]);
             if constexpr (SCHEME == SCH_BF16X3) bl[j] = *reinterpret_cast<const u32x4 *>(&el.b_lo[o]);
         }
-        constexpr int NTERM = SCHEME == SCH_BF16X3 ? 3 : 2;
 #pragma unroll
         for (int t = 0; t < NTERM; ++t) {
 #pragma unroll
-            for (int j = 0; j < T::WC; ++j)
+            for (int j = 0; j < (JOUTER ? 0 : T::WC); ++j)
 #pragma unroll
                 for (int i = 0; i < T::WR; ++i) {
                     // terms: (a_hi, b_hi), (a_hi, b_lo) [BF16X3 only], (a_lo, b_hi)
@@ -448,10 +476,13 @@ using CfgC = TileCfg<2, 4, 2, 4>;   //  64 x 256
 using CfgD = TileCfg<4, 4, 2, 2>;   // 128 x 128, 4 waves
 using CfgE = TileCfg<4, 2, 2, 4>;   // 128 x 128, 8 waves
 using CfgF = TileCfg<1, 4, 1, 4>;   //  16 x 256 (lines / few rows)
+using CfgG = TileCfg<2, 11, 8, 1>;  // 256 x 176 (flattened rows x one short axis, e.g. 161)
+using CfgH = TileCfg<1, 11, 8, 1>;  // 128 x 176
 
 static const TileInfo kTiles[] = {
-    {CfgA::RT, CfgA::CT, CfgA::NT}, {CfgB::RT, CfgB::CT, CfgB::NT}, {CfgC::RT, CfgC::CT, CfgC::NT},
-    {CfgD::RT, CfgD::CT, CfgD::NT}, {CfgE::RT, CfgE::CT, CfgE::NT}, {CfgF::RT, CfgF::CT, CfgF::NT},
+    {CfgA::RT, CfgA::CT, CfgA::NT, 0}, {CfgB::RT, CfgB::CT, CfgB::NT, 0}, {CfgC::RT, CfgC::CT, CfgC::NT, 0},
+    {CfgD::RT, CfgD::CT, CfgD::NT, 0}, {CfgE::RT, CfgE::CT, CfgE::NT, 0}, {CfgF::RT, CfgF::CT, CfgF::NT, 0},
+    {CfgG::RT, CfgG::CT, CfgG::NT, 1}, {CfgH::RT, CfgH::CT, CfgH::NT, 1},
 };
 
 int gemm_num_tiles() { return (int)(sizeof(kTiles) / sizeof(kTiles[0])); }
@@ -479,6 +510,8 @@ static hipError_t launch_scheme(int tile, const AtomRec *rec, const int *slice_b
     case 3: launch_one<SCHEME, CfgD>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st); break;
     case 4: launch_one<SCHEME, CfgE>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st); break;
     case 5: launch_one<SCHEME, CfgF>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st); break;
+    case 6: launch_one<SCHEME, CfgG>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st); break;
+    case 7: launch_one<SCHEME, CfgH>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -56,6 +56,7 @@ __device__ inline int istl_point(const Grid &g, int i, int j, int l) {
 
 struct TileInfo {
     int rt, ct, nt;
+    int split_only;         // register-heavy tile used by the split schemes only (FP32 spills)
 };
 
 struct ReduceArgs {

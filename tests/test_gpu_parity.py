@@ -75,6 +75,40 @@ def test_strucf_layer_call(ctx, prec, shape):
     assert err <= bar
 
 
+def _ntiles():
+    from diffusecode_amd import _lib
+    return len(_lib.Context.gemm_tiles())
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("shape", [(41, 23, 19), (19, 37, 1), (17, 9, 5)])
+@pytest.mark.parametrize("tile", range(8))
+def test_every_tile_configuration(ctx, prec, shape, tile):
+    """Each work-group tile shape (forced), with wrapping rows or columns
+    (flattened axes), on a rocksalt crystal with vacancies and displacements
+    over grids through integer hkl, against the exact FP64 sum."""
+    if tile >= _ntiles():
+        pytest.skip("tile not built")
+    print(f"case tile={tile} prec={prec} shape={shape}", flush=True)
+    c = X.rocksalt(n=4, seed=5)
+    pos = c.pos[c.iscat > 0]
+    step = 0.1
+    lo = [-(n // 2) * step for n in shape]
+    eck, vi, num = FR.grid_block(lo, [l + (n - 1) * step for l, n in zip(lo, shape)], shape)
+    st, lmn, off = O.fourier_lmn(eck, vi, num)
+    ctx.set_precision(prec)
+    ctx.set_gemm_tile(tile)
+    try:
+        F = ctx.strucf(pos, vi[:, 0], vi[:, 1], vi[:, 2], off, lmn, num)
+    finally:
+        ctx.set_gemm_tile(-1)
+        ctx.set_precision("fp32")
+    Fref = O.strucf_exact(pos, lmn_corner(lmn, off, vi), vi[:, 0], vi[:, 1], vi[:, 2], num)
+    err = rel_intensity_err(F, Fref)
+    print(f"  err = {err:.3e}", flush=True)
+    assert err <= BAR[prec]
+
+
 def test_fp16x2_bragg_dominated(ctx):
     """FP16x2 on a Bragg-dominated crystal: its per-term error (2^-12 on the
     unsplit operand) partly averages out; measured ~2e-5 of max(I) here, so
