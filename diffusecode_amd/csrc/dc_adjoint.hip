@@ -21,6 +21,13 @@
 // and the column weights (1, j - jc, m - mc, s_c^2).  h_p,a is affine in
 // (r, j, m), so the three row weights and four column weights give every
 // sum above; the weights are centred to keep the FP32 terms small.
+//
+// Split variant (BF16x3, v_mfma_f32_16x16x32_bf16): the weighted A elements
+// are split into BF16 hi + lo in FP32 and stored in LDS in the two forms of
+// the real embedding; Q comes pre-split (q_split_kernel) in a layout whose
+// 16-byte words are exactly one lane's MFMA operand, loaded from L2 into
+// registers one stage ahead.  Products hi.hi + hi.lo + lo.hi, FP32
+// accumulation, the same epilogue.
 #include <algorithm>
 
 #include "dc_internal.h"
@@ -28,6 +35,10 @@
 namespace dc {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
 namespace {
 
@@ -35,6 +46,7 @@ constexpr int WA = 2, WC = 2, NWA = 2, NWC = 4;   // 16x16 tiles per wave, waves
 constexpr int NA = 16 * WA * NWA;                  // atoms per work-group
 constexpr int NC = 16 * WC * NWC;                  // grid columns per work-group
 constexpr int NT = 64 * NWA * NWC;
+static_assert(NT == NA * 8, "one A element per thread and stage row (FP32) / 2 rows per thread (split)");
 constexpr int KR = 8;                              // grid rows per LDS stage
 constexpr int KP = 2 * KR;                         // real K per stage (re, im interleaved)
 constexpr float LOG2E = 1.4426950408889634f;
@@ -78,6 +90,74 @@ __device__ __forceinline__ void load_stage(AdSmem &sm, int buf, const AdjArgs &a
         sm.q[buf][rr][cc] = (r < a.R && col < a.C) ? a.q[qbase + col + (long long)a.C * r] : make_float2(0.f, 0.f);
     }
     if (t < KR) sm.sr[buf][t] = r0 + t < a.R ? sq[r0 + t] : 0.0f;
+}
+
+// Epilogue shared by the FP32 and split kernels: lane holds atoms 4 lg + v of
+// column li of every 16x16 tile; contract with the generated column factor
+// and weights, reduce over columns in a fixed order, write the partials.
+__device__ __forceinline__ void adjoint_epilogue(const AdjArgs &a, const float *__restrict__ sq,
+                                                 const AtomRec *at, float (&red)[NWC][NA][6],
+                                                 f32x4 (&accR)[3][WA][WC], f32x4 (&accI)[3][WA][WC], int col0,
+                                                 int atom0, int na) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
+    const int wa0 = (wave % NWA) * 16 * WA, wc0 = (wave / NWA) * 16 * WC;
+    float acc[WA][4][6];
+#pragma unroll
+    for (int i = 0; i < WA; ++i)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) acc[i][v][k] = 0.0f;
+#pragma unroll
+    for (int j = 0; j < WC; ++j) {
+        const int col = col0 + wc0 + 16 * j + li;
+        const int jj = col % a.C0, mm = col / a.C0;
+        const bool valid = col < a.C && jj < a.f1;
+        const float sc = valid ? sq[a.R + col] : 0.0f;
+        const float wj = (float)jj - a.jc, wm = (float)mm - a.mc;
+#pragma unroll
+        for (int i = 0; i < WA; ++i)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const AtomRec &ar = at[wa0 + 16 * i + 4 * lg + v];
+                float2 b = cis(ar.p0 + (uint64_t)jj * ar.ca + (uint64_t)mm * ar.ca1);
+                const float d = valid ? __builtin_amdgcn_exp2f(-ar.pad * LOG2E * sc) : 0.0f;
+                b = make_float2(b.x * d, b.y * d);
+                const float z0r = accR[0][i][j][v], z0i = accI[0][i][j][v];
+                const float tr = z0r * b.x - z0i * b.y;
+                const float ti = z0r * b.y + z0i * b.x;
+                const float u1 = accR[1][i][j][v] * b.y + accI[1][i][j][v] * b.x;
+                const float u2 = accR[2][i][j][v] * b.x - accI[2][i][j][v] * b.y;
+                acc[i][v][0] += tr;
+                acc[i][v][1] += ti;
+                acc[i][v][2] += u1;
+                acc[i][v][3] += wj * ti;
+                acc[i][v][4] += wm * ti;
+                acc[i][v][5] += u2 + sc * tr;
+            }
+    }
+    // sum over the 16 columns of a lane group, then over the waves of a row of tiles
+#pragma unroll
+    for (int i = 0; i < WA; ++i)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                float x = acc[i][v][k];
+                x += __shfl_xor(x, 1);
+                x += __shfl_xor(x, 2);
+                x += __shfl_xor(x, 4);
+                x += __shfl_xor(x, 8);
+                if (li == 0) red[wave / NWA][wa0 + 16 * i + 4 * lg + v][k] = x;
+            }
+    __syncthreads();
+    float *out = a.part + ((long long)blockIdx.x * a.natoms + atom0) * 6;
+    for (int x = threadIdx.x; x < na * 6; x += NT) {
+        float v = 0.0f;
+#pragma unroll
+        for (int w = 0; w < NWC; ++w) v += (&red[w][0][0])[x];
+        out[x] = v;
+    }
 }
 
 __global__ void __launch_bounds__(NT) adjoint_kernel(AdjArgs a) {
@@ -145,63 +225,184 @@ __global__ void __launch_bounds__(NT) adjoint_kernel(AdjArgs a) {
         __syncthreads();
     }
 
-    // epilogue: lane holds atoms 4 lg + v of column li of every 16x16 tile
-    float acc[WA][4][6];
+    adjoint_epilogue(a, sq, sm.at, sm.red, accR, accI, col0, atom0, na);
+}
+
+// ---------------------------------------------------------------------------
+// Split BF16x3 adjoint.
+constexpr int SKR = 16;   // grid rows per stage: one 16x16x32 k-step (K' = 32)
+
+__device__ __forceinline__ uint32_t bpack(float x, float y) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){x, y}, bf16x2));
+}
+// (x, y) = hi + lo, both packed BF16 pairs (x in the low half)
+__device__ __forceinline__ void bsplit(float x, float y, uint32_t &hi, uint32_t &lo) {
+    hi = bpack(x, y);
+    lo = bpack(x - __builtin_bit_cast(float, hi << 16), y - __builtin_bit_cast(float, hi & 0xFFFF0000u));
+}
+__device__ __forceinline__ uint32_t rot16(uint32_t w) { return (w >> 16) | (w << 16); }
+
+struct AdSmemS {
+    u32x4 a[2][2][3][2][4][NA];   // [buffer][form (re,-im) | (im,re)][weight][hi | lo][row group][atom]
+    AtomRec at[NA];
+    float red[NWC][NA][6];
+};
+
+// A of stage rows r0..r0+15: thread (atom, row group g, half hf) makes rows
+// 4g + 2hf + {0, 1}, all three weights, both forms, hi and lo.
+__device__ __forceinline__ void gen_stage_split(AdSmemS &sm, int buf, const AdjArgs &a, const float *__restrict__ sq,
+                                                int r0, int na) {
+    const int t = threadIdx.x;
+    const int at = t & (NA - 1), g = (t >> 6) & 3, hf = t >> 8;
+    uint32_t w[2][3][2][2];
 #pragma unroll
-    for (int i = 0; i < WA; ++i)
+    for (int e = 0; e < 2; ++e) {
+        const int r = r0 + 4 * g + 2 * hf + e;
+        float2 v = make_float2(0.f, 0.f);
+        float s2 = 0.0f;
+        if (r < a.R && at < na) {
+            const AtomRec &ar = sm.at[at];
+            s2 = sq[r];
+            v = cis((uint64_t)r * ar.ra);
+            const float d = __builtin_amdgcn_exp2f(-ar.pad * LOG2E * s2);
+            v = make_float2(v.x * d, v.y * d);
+        }
+        const float wt[3] = {1.0f, (float)r - a.rc, s2};
 #pragma unroll
-        for (int v = 0; v < 4; ++v)
+        for (int k = 0; k < 3; ++k) {
+            uint32_t hi, lo;
+            bsplit(v.x * wt[k], v.y * wt[k], hi, lo);
+            w[0][k][0][e] = hi ^ 0x80000000u;
+            w[1][k][0][e] = rot16(hi);
+            w[0][k][1][e] = lo ^ 0x80000000u;
+            w[1][k][1][e] = rot16(lo);
+        }
+    }
 #pragma unroll
-            for (int k = 0; k < 6; ++k) acc[i][v][k] = 0.0f;
+    for (int f = 0; f < 2; ++f)
 #pragma unroll
-    for (int j = 0; j < WC; ++j) {
-        const int col = col0 + wc0 + 16 * j + li;
-        const int jj = col % a.C0, mm = col / a.C0;
-        const bool valid = col < a.C && jj < a.f1;
-        const float sc = valid ? sq[a.R + col] : 0.0f;
-        const float wj = (float)jj - a.jc, wm = (float)mm - a.mc;
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                uint32_t *dst = reinterpret_cast<uint32_t *>(&sm.a[buf][f][k][p][g][at]) + 2 * hf;
+                *reinterpret_cast<uint2 *>(dst) = make_uint2(w[f][k][p][0], w[f][k][p][1]);
+            }
+}
+
+__global__ void __launch_bounds__(NT) adjoint_split_kernel(AdjArgs a) {
+    __shared__ AdSmemS sm;
+    const int4 tile = a.tiles[blockIdx.y];
+    const int atom0 = tile.x, na = tile.y - tile.x;
+    const float *sq = a.sq + (long long)tile.w * a.sq_stride;
+    const uint32_t *qh = a.qs + (long long)tile.z * 2 * a.Rg * a.C * 4;
+    const uint32_t *ql = qh + (long long)a.Rg * a.C * 4;
+    const int col0 = blockIdx.x * NC;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
+    const int wa0 = (wave % NWA) * 16 * WA, wc0 = (wave / NWA) * 16 * WC;
+
+    for (int x = threadIdx.x; x < NA; x += NT) {
+        AtomRec r{};
+        if (x < na) r = a.rec[atom0 + x];
+        sm.at[x] = r;
+    }
+    __syncthreads();
+
+    f32x4 accR[3][WA][WC], accI[3][WA][WC];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
 #pragma unroll
         for (int i = 0; i < WA; ++i)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const AtomRec &ar = sm.at[wa0 + 16 * i + 4 * lg + v];
-                float2 b = cis(ar.p0 + (uint64_t)jj * ar.ca + (uint64_t)mm * ar.ca1);
-                const float d = valid ? __builtin_amdgcn_exp2f(-ar.pad * LOG2E * sc) : 0.0f;
-                b = make_float2(b.x * d, b.y * d);
-                const float z0r = accR[0][i][j][v], z0i = accI[0][i][j][v];
-                const float tr = z0r * b.x - z0i * b.y;
-                const float ti = z0r * b.y + z0i * b.x;
-                const float u1 = accR[1][i][j][v] * b.y + accI[1][i][j][v] * b.x;
-                const float u2 = accR[2][i][j][v] * b.x - accI[2][i][j][v] * b.y;
-                acc[i][v][0] += tr;
-                acc[i][v][1] += ti;
-                acc[i][v][2] += u1;
-                acc[i][v][3] += wj * ti;
-                acc[i][v][4] += wm * ti;
-                acc[i][v][5] += u2 + sc * tr;
+            for (int j = 0; j < WC; ++j) {
+                accR[k][i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+                accI[k][i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
             }
-    }
-    // sum over the 16 columns of a lane group, then over the waves of a row of tiles
+
+    // B operand words of one stage: lane (column li, row group lg) of each 16-column tile
+    auto load_b = [&](int st, u32x4 (&bh)[WC], u32x4 (&bl)[WC]) {
 #pragma unroll
-    for (int i = 0; i < WA; ++i)
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-#pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                float x = acc[i][v][k];
-                x += __shfl_xor(x, 1);
-                x += __shfl_xor(x, 2);
-                x += __shfl_xor(x, 4);
-                x += __shfl_xor(x, 8);
-                if (li == 0) sm.red[wave / NWA][wa0 + 16 * i + 4 * lg + v][k] = x;
+        for (int j = 0; j < WC; ++j) {
+            const int col = col0 + wc0 + 16 * j + li;
+            const long long o = ((long long)(st * 4 + lg) * a.C + col) * 4;
+            if (col < a.C) {
+                bh[j] = *reinterpret_cast<const u32x4 *>(qh + o);
+                bl[j] = *reinterpret_cast<const u32x4 *>(ql + o);
+            } else {
+                bh[j] = u32x4{0u, 0u, 0u, 0u};
+                bl[j] = u32x4{0u, 0u, 0u, 0u};
             }
+        }
+    };
+
+    const int nst = (a.R + SKR - 1) / SKR;
+    u32x4 bh[WC], bl[WC], nbh[WC], nbl[WC];
+    load_b(0, bh, bl);
+    gen_stage_split(sm, 0, a, sq, 0, na);
     __syncthreads();
-    float *out = a.part + ((long long)blockIdx.x * a.natoms + atom0) * 6;
-    for (int x = threadIdx.x; x < na * 6; x += NT) {
-        float v = 0.0f;
+    for (int s = 0; s < nst; ++s) {
+        const int buf = s & 1;
+        if (s + 1 < nst) {
+            load_b(s + 1, nbh, nbl);
+            gen_stage_split(sm, buf ^ 1, a, sq, (s + 1) * SKR, na);
+        }
 #pragma unroll
-        for (int w = 0; w < NWC; ++w) v += (&sm.red[w][0][0])[x];
-        out[x] = v;
+        for (int i = 0; i < WA; ++i) {
+            u32x4 A[2][3][2];
+#pragma unroll
+            for (int f = 0; f < 2; ++f)
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+#pragma unroll
+                    for (int p = 0; p < 2; ++p) A[f][k][p] = sm.a[buf][f][k][p][lg][wa0 + 16 * i + li];
+#pragma unroll
+            for (int j = 0; j < WC; ++j)
+#pragma unroll
+                for (int t = 0; t < 3; ++t)          // hi.hi, hi.lo, lo.hi
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const int p = t == 2 ? 1 : 0;
+                        const u32x4 y = t == 1 ? bl[j] : bh[j];
+                        accR[k][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            __builtin_bit_cast(bf16x8, A[0][k][p]), __builtin_bit_cast(bf16x8, y), accR[k][i][j], 0, 0, 0);
+                        accI[k][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            __builtin_bit_cast(bf16x8, A[1][k][p]), __builtin_bit_cast(bf16x8, y), accI[k][i][j], 0, 0, 0);
+                    }
+        }
+        if (s + 1 < nst) {
+#pragma unroll
+            for (int j = 0; j < WC; ++j) {
+                bh[j] = nbh[j];
+                bl[j] = nbl[j];
+            }
+        }
+        __syncthreads();
+    }
+    adjoint_epilogue(a, sq, sm.at, sm.red, accR, accI, col0, atom0, na);
+}
+
+// FP32 complex Q planes -> split words: for plane t, part (hi, lo), row
+// group g (rows 4g..4g+3) and column c one 16-byte word of 4 packed pairs.
+__global__ void q_split_kernel(const float2 *__restrict__ q, int nq, int R, int C, long long plane, int Rg,
+                               uint32_t *__restrict__ qs) {
+    const long long n = (long long)nq * Rg * C;
+    for (long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (long long)gridDim.x * blockDim.x) {
+        const int c = (int)(x % C);
+        const long long y = x / C;
+        const int g = (int)(y % Rg);
+        const long long t = y / Rg;
+        u32x4 hi, lo;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int r = 4 * g + e;
+            const float2 v = r < R ? q[t * plane + c + (long long)C * r] : make_float2(0.f, 0.f);
+            uint32_t h, l;
+            bsplit(v.x, v.y, h, l);
+            hi[e] = h;
+            lo[e] = l;
+        }
+        uint32_t *base = qs + t * 2 * Rg * (long long)C * 4;
+        *reinterpret_cast<u32x4 *>(base + ((long long)g * C + c) * 4) = hi;
+        *reinterpret_cast<u32x4 *>(base + ((long long)(Rg + g) * C + c) * 4) = lo;
     }
 }
 
@@ -335,10 +536,24 @@ hipError_t launch_fill_zero(double *p, long long n, hipStream_t st) {
 int adjoint_tile_atoms() { return NA; }
 int adjoint_tile_cols() { return NC; }
 
-hipError_t launch_adjoint(const AdjArgs &a, int ntiles, hipStream_t st) {
+hipError_t launch_adjoint(const AdjArgs &a, int ntiles, hipStream_t st, bool split) {
     if (ntiles <= 0 || a.R <= 0 || a.C <= 0) return hipSuccess;
     dim3 grid((a.C + NC - 1) / NC, ntiles);
-    adjoint_kernel<<<grid, dim3(NT), 0, st>>>(a);
+    if (split) {
+        if (!a.qs || a.Rg * 4 < a.R || a.Rg % 4) return hipErrorInvalidValue;
+        adjoint_split_kernel<<<grid, dim3(NT), 0, st>>>(a);
+    } else {
+        adjoint_kernel<<<grid, dim3(NT), 0, st>>>(a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_q_split(const float2 *q, int nq, int R, int C, long long plane, int Rg, uint32_t *qs,
+                          hipStream_t st) {
+    const long long n = (long long)nq * Rg * C;
+    if (n <= 0) return hipSuccess;
+    const long long nb = std::min<long long>((n + 255) / 256, 8192);
+    q_split_kernel<<<(unsigned)nb, 256, 0, st>>>(q, nq, R, C, plane, Rg, qs);
     return hipGetLastError();
 }
 

@@ -35,7 +35,7 @@ struct BatchState {
     int nsorted = 0, nslice = 0, ntiles = 0, ntc = 0, nblk = 0;
     int64_t natoms = 0;
     DevBuf pos, occ, bdw, idx, rec, sbeg, stype, ssq, msl, partial, cfact, rten, sq, target, weight, icalc, chi,
-        q, atiles, part, grad;
+        q, qs, atiles, part, grad;
     bool primed = false;                 // records built and autotuned once
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -43,7 +43,7 @@ struct BatchState {
 
     std::vector<DevBuf *> buffers() {
         return {&pos, &occ, &bdw, &idx, &rec, &sbeg, &stype, &ssq, &msl, &partial, &cfact, &rten, &sq, &target,
-                &weight, &icalc, &chi, &q, &atiles, &part, &grad};
+                &weight, &icalc, &chi, &q, &qs, &atiles, &part, &grad};
     }
 };
 
@@ -232,6 +232,10 @@ int build_state(dc_ctx *c, const dc_batch_desc *d, BatchState &b) {
         // padded columns of the Q planes are never written: zero them once
         HIPCHK(hipMemsetAsync(b.q.p, 0, sizeof(float2) * gg.P * d->nscat * M, c->st));
         if ((st = ensure(c, b.part, sizeof(float) * 6 * (size_t)b.ntc * std::max(b.nsorted, 1)))) return st;
+        if (c->prec != DC_PREC_FP32) {
+            const size_t rg = (size_t)((gg.R + 15) / 16) * 4;
+            if ((st = ensure(c, b.qs, sizeof(uint32_t) * 8 * rg * gg.C * d->nscat * M))) return st;
+        }
         if ((st = ensure(c, b.grad, sizeof(double) * 5 * T))) return st;
     }
     return 0;
@@ -291,7 +295,13 @@ int enqueue(dc_ctx *c, BatchState &b, int tile, int nwg, bool prep, bool rest) {
         aa.rc = 0.5f * (float)(b.g.f[0] - 1);
         aa.jc = 0.5f * (float)(b.g.f[1] - 1);
         aa.mc = 0.5f * (float)(b.g.f[2] - 1);
-        HIPCHK(launch_adjoint(aa, b.ntiles, c->st));
+        const bool split = b.prec != DC_PREC_FP32;
+        if (split) {
+            aa.Rg = (gg.R + 15) / 16 * 4;
+            aa.qs = ptr<uint32_t>(b.qs);
+            HIPCHK(launch_q_split(ptr<float2>(b.q), b.nmembers * b.nscat, gg.R, gg.C, gg.P, aa.Rg, ptr<uint32_t>(b.qs), c->st));
+        }
+        HIPCHK(launch_adjoint(aa, b.ntiles, c->st, split));
         AdjFinishArgs fa{};
         fa.part = ptr<float>(b.part);
         fa.ntile_c = b.ntc;

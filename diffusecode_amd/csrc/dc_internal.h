@@ -115,6 +115,8 @@ struct AdjArgs {
     const float *sq;        // s^2 parts: R rows, then C columns (per member: + tile.w * sq_stride)
     long long sq_stride;
     float *part;            // [column tile][natoms][6]
+    const uint32_t *qs;     // split scheme: Q as BF16 hi/lo words, per plane [hi|lo][row group][C][4 rows]
+    int Rg;                 // row groups of 4 rows in qs (R padded to a multiple of 16)
     int R, C, C0, f1;       // rows, padded columns, column period, valid columns per period
     long long plane;        // R * C
     int natoms;
@@ -164,7 +166,11 @@ hipError_t launch_batch_reduce(const BatchReduceArgs &a, int nblocks, hipStream_
 hipError_t launch_fill_zero(double *p, long long n, hipStream_t st);
 int adjoint_tile_atoms();
 int adjoint_tile_cols();
-hipError_t launch_adjoint(const AdjArgs &a, int ntiles, hipStream_t st);
+// split = true: BF16x3 MFMA on qs (made by launch_q_split), else FP32 on q.
+hipError_t launch_adjoint(const AdjArgs &a, int ntiles, hipStream_t st, bool split = false);
+// FP32 complex Q planes (c + C r) -> BF16 hi/lo words for the split adjoint.
+hipError_t launch_q_split(const float2 *q, int nq, int R, int C, long long plane, int Rg, uint32_t *qs,
+                          hipStream_t st);
 hipError_t launch_adjoint_finish(const AdjFinishArgs &a, hipStream_t st);
 hipError_t launch_model_q(const QArgs &a, hipStream_t st);
 hipError_t launch_chi2_resid(const double *icalc, const double *tgt, const double *wt, long long n, double *gi,

@@ -244,7 +244,14 @@ int model_adjoint(dc_ctx *c, const dc_model_desc *m, ModelPlan &mp) {
         aa.rc = 0.5f * (float)(mp.g.f[0] - 1);
         aa.jc = 0.5f * (float)(mp.g.f[1] - 1);
         aa.mc = 0.5f * (float)(mp.g.f[2] - 1);
-        HIPCHK(launch_adjoint(aa, ntiles, c->st));
+        const bool split = c->prec != DC_PREC_FP32;
+        if (split) {
+            aa.Rg = (int)((gg.R + 15) / 16) * 4;
+            if ((st = ensure(c, c->m_qs, sizeof(uint32_t) * 8 * (size_t)aa.Rg * gg.C * nq))) return st;
+            aa.qs = ptr<uint32_t>(c->m_qs);
+            HIPCHK(launch_q_split(ptr<float2>(c->m_q), nq, gg.R, gg.C, gg.P, aa.Rg, ptr<uint32_t>(c->m_qs), c->st));
+        }
+        HIPCHK(launch_adjoint(aa, ntiles, c->st, split));
 
         AdjFinishArgs fa{};
         fa.part = ptr<float>(c->m_part);

@@ -4,8 +4,9 @@ hand-written adjoint against the oracle's direct FP64 gradient, and chi^2
 gradients against FP64 central differences (h = 1e-6) of the oracle chi^2.
 
 Bars: forward max|dI|/max(I) <= 1e-6 (FP32) and <= 1e-5 (BF16x3); gradients
-max|dg| <= 2e-5 * max|g| per parameter kind (x, y, z, occ, B) -- the FP32
-adjoint accumulates sum_p over up to ~1e4 pixels per partial sum."""
+max|dg| <= 2e-5 * max|g| per parameter kind (x, y, z, occ, B) with the FP32
+adjoint and <= 1e-4 with the split BF16x3 adjoint (products carry ~2^-16
+relative error before the FP32 accumulation over the grid)."""
 import numpy as np
 import pytest
 
@@ -18,6 +19,7 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 GRAD_BAR = 2e-5
+GRAD_BAR_PREC = {"fp32": 2e-5, "bf16x3": 1e-4}
 
 
 def problem(ncell=(4, 4, 3), n=(33, 25, 17), lo=(-2.0, -1.5, -1.0), hi=(2.0, 1.5, 1.0), seed=4):
@@ -67,22 +69,28 @@ def grad_err(g, ref):
     return [np.abs(g[:, k] - ref[:, k]).max() / np.abs(ref[:, k]).max() for k in range(5)]
 
 
-@pytest.mark.parametrize("n", [(33, 25, 17), (33, 33, 1), (17, 9, 5)])
-def test_model_vjp_matches_fp64_gradient(ctx, n):
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("n", [(33, 25, 17), (33, 33, 1), (17, 9, 5), (40, 7, 3)])
+def test_model_vjp_matches_fp64_gradient(ctx, n, prec):
     cr, occ, bdw, cf, eck, vi, inc = problem(n=n)
     rng = np.random.default_rng(9)
     d, keep = ctx.model_desc(cr.pos, cr.iscat, occ, bdw, cf, eck, vi, inc, cr.rten())
-    icalc, fcalc = ctx.model_intensity(d)
-    gi = rng.standard_normal(icalc.size)
-    g = ctx.model_vjp(d, fcalc, gi)
+    ctx.set_precision(prec)
+    try:
+        icalc, fcalc = ctx.model_intensity(d)
+        gi = rng.standard_normal(icalc.size)
+        g = ctx.model_vjp(d, fcalc, gi)
+    finally:
+        ctx.set_precision("fp32")
     F = O.model_f(cr.pos, cr.iscat, occ, bdw, cf, eck, vi, inc, cr.rten())
     ref = O.model_grad(cr.pos, cr.iscat, occ, bdw, cf, eck, vi, inc, cr.rten(), F, gi)
     e = grad_err(g, ref)
-    print("  vjp", n, " ".join(f"{x:.2e}" for x in e))
-    assert max(e) <= GRAD_BAR
+    print("  vjp", prec, n, " ".join(f"{x:.2e}" for x in e))
+    assert max(e) <= GRAD_BAR_PREC[prec]
 
 
-def test_model_chi2_gradient_vs_central_differences(ctx):
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_model_chi2_gradient_vs_central_differences(ctx, prec):
     """200 random parameters: GPU chi^2 gradient against FP64 central differences (h = 1e-6)."""
     cr, occ, bdw, cf, eck, vi, inc = problem()
     rten = cr.rten()
@@ -93,11 +101,15 @@ def test_model_chi2_gradient_vs_central_differences(ctx):
     target = It * (1.0 + 0.01 * rng.standard_normal(It.size))
     wt = 1.0 / np.maximum(target, 1e-3 * target.max())
     d, keep = ctx.model_desc(cr.pos, cr.iscat, occ, bdw, cf, eck, vi, inc, rten)
-    chi, icalc, g = ctx.model_chi2(d, target, wt)
+    ctx.set_precision(prec)
+    try:
+        chi, icalc, g = ctx.model_chi2(d, target, wt)
+    finally:
+        ctx.set_precision("fp32")
     F0 = O.model_f(cr.pos, cr.iscat, occ, bdw, cf, eck, vi, inc, rten)
     I0 = (np.abs(F0) ** 2).ravel(order="F")
     chi_ref = float((wt * (I0 - target) ** 2).sum())
-    assert abs(chi - chi_ref) <= 1e-5 * chi_ref
+    assert abs(chi - chi_ref) <= (1e-5 if prec == "fp32" else 1e-4) * chi_ref
     n = len(cr.iscat)
     atoms = rng.integers(0, n, 200)
     kinds = rng.integers(0, 5, 200)
@@ -111,8 +123,8 @@ def test_model_chi2_gradient_vs_central_differences(ctx):
         scale = np.abs(ref[:, k]).max()
         e = np.abs(g[atoms[sel], k] - fd[sel]).max() / scale
         worst = max(worst, e)
-        print(f"  kind {k}: max|g - fd|/max|g| = {e:.2e} over {sel.sum()} parameters")
-    assert worst <= GRAD_BAR
+        print(f"  {prec} kind {k}: max|g - fd|/max|g| = {e:.2e} over {sel.sum()} parameters")
+    assert worst <= GRAD_BAR_PREC[prec]
 
 
 def test_model_vacancies_and_type_order(ctx):
