@@ -42,14 +42,32 @@ typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
 namespace {
 
-constexpr int WA = 2, WC = 2, NWA = 2, NWC = 4;   // 16x16 tiles per wave, waves per work-group
+constexpr int WA = 2, WC = 2, NWA = 1, NWC = 8;   // 16x16 tiles per wave, waves per work-group
 constexpr int NA = 16 * WA * NWA;                  // atoms per work-group
 constexpr int NC = 16 * WC * NWC;                  // grid columns per work-group
 constexpr int NT = 64 * NWA * NWC;
-static_assert(NT == NA * 8, "one A element per thread and stage row (FP32) / 2 rows per thread (split)");
-constexpr int KR = 8;                              // grid rows per LDS stage
+constexpr int KR = 16;                             // grid rows per LDS stage
+static_assert(NT == NA * KR, "one A element per thread and stage");
+static_assert(KR == 16, "row tables hold 16 rows per stage");
+constexpr int MAXST = 64;                          // stages with tabulated row factors (R <= 1024)
 constexpr int KP = 2 * KR;                         // real K per stage (re, im interleaved)
 constexpr float LOG2E = 1.4426950408889634f;
+
+// Column factors of the epilogue, exact two-level: column col0 + 16h + l of
+// atom n has phase TH[n][h] + l ca + delta w (delta = 1 past the wrap of the
+// flattened columns, at l >= wpos[h]; w = ca1 - C0 ca).
+struct ColTables {
+    float2 th[NA][NC / 16];
+    float2 tl[NA][2][16];
+    int wpos[NC / 16];
+};
+
+// Row factors, exact two-level: row 16 s + rr of atom n has phase
+// 16 s ra + rr ra, so A = RH[s][n] * RL[n][rr] (grids of up to MAXST stages).
+struct RowTables {
+    float2 rh[MAXST][NA];
+    float2 rl[NA][16];
+};
 
 struct AdSmem {
     float a[2][KP][NA];       // A(k', atom), k' = 2 rr + (0 re, 1 im)
@@ -57,6 +75,8 @@ struct AdSmem {
     float sr[2][KR];          // s^2 row parts of the stage rows
     AtomRec at[NA];
     float red[NWC][NA][6];    // per column-wave partial sums (fixed-order reduction)
+    ColTables ct;
+    RowTables rt;
 };
 
 __device__ __forceinline__ float2 cis(uint64_t ph) {
@@ -65,20 +85,40 @@ __device__ __forceinline__ float2 cis(uint64_t ph) {
     return make_float2(c, s);
 }
 
+__device__ __forceinline__ void build_row_tables(RowTables &rt, const AtomRec *at, int nst) {
+    for (int x = threadIdx.x; x < NA * (16 + nst); x += NT) {
+        const int n = x % NA, e = x / NA;
+        if (e < 16)
+            rt.rl[n][e] = cis((uint64_t)e * at[n].ra);
+        else
+            rt.rh[e - 16][n] = cis((uint64_t)(16 * (e - 16)) * at[n].ra);
+    }
+}
+
+// A element of atom n at row r = 16 s + rr (r < R): row phase and exp(-B s_r^2)
+__device__ __forceinline__ float2 row_element(const RowTables &rt, const AtomRec &ar, int n, int s, int rr, int r,
+                                              bool tab, float s2) {
+    float2 v;
+    if (tab) {
+        const float2 h = rt.rh[s][n], l = rt.rl[n][rr];
+        v = make_float2(h.x * l.x - h.y * l.y, h.x * l.y + h.y * l.x);
+    } else {
+        v = cis((uint64_t)r * ar.ra);
+    }
+    const float d = __builtin_amdgcn_exp2f(-ar.pad * LOG2E * s2);
+    return make_float2(v.x * d, v.y * d);
+}
+
 // A of stage rows r0..r0+KR-1 (one element per thread) and the Q rows.
 __device__ __forceinline__ void load_stage(AdSmem &sm, int buf, const AdjArgs &a, const float *__restrict__ sq,
-                                           int r0, int col0, int na, long long qbase) {
+                                           int st, int col0, int na, long long qbase, bool tab) {
     const int t = threadIdx.x;
+    const int r0 = st * KR;
     {
         const int at = t & (NA - 1), rr = t / NA;      // NT == NA * KR
         const int r = r0 + rr;
         float2 v = make_float2(0.f, 0.f);
-        if (r < a.R && at < na) {
-            const AtomRec &ar = sm.at[at];
-            v = cis((uint64_t)r * ar.ra);
-            const float d = __builtin_amdgcn_exp2f(-ar.pad * LOG2E * sq[r]);
-            v = make_float2(v.x * d, v.y * d);
-        }
+        if (r < a.R && at < na) v = row_element(sm.rt, sm.at[at], at, st, rr, r, tab, sq[r]);
         sm.a[buf][2 * rr][at] = v.x;
         sm.a[buf][2 * rr + 1][at] = v.y;
     }
@@ -96,11 +136,27 @@ __device__ __forceinline__ void load_stage(AdSmem &sm, int buf, const AdjArgs &a
 // column li of every 16x16 tile; contract with the generated column factor
 // and weights, reduce over columns in a fixed order, write the partials.
 __device__ __forceinline__ void adjoint_epilogue(const AdjArgs &a, const float *__restrict__ sq,
-                                                 const AtomRec *at, float (&red)[NWC][NA][6],
+                                                 const AtomRec *at, float (&red)[NWC][NA][6], ColTables &ct,
                                                  f32x4 (&accR)[3][WA][WC], f32x4 (&accI)[3][WA][WC], int col0,
                                                  int atom0, int na) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
     const int wa0 = (wave % NWA) * 16 * WA, wc0 = (wave / NWA) * 16 * WC;
+    // two-level column tables of the work-group's atoms (caller synchronised the LDS)
+    constexpr int NH = NC / 16, NE = NH + 32;
+    for (int x = threadIdx.x; x < NA * NE; x += NT) {
+        const int n = x / NE, e = x % NE;
+        const AtomRec &ar = at[n];
+        if (e < NH) {
+            const int c0 = col0 + 16 * e;
+            ct.th[n][e] = cis(ar.p0 + (uint64_t)(c0 % a.C0) * ar.ca + (uint64_t)(c0 / a.C0) * ar.ca1);
+            if (n == 0) ct.wpos[e] = a.C0 - c0 % a.C0;
+        } else {
+            const int d = (e - NH) >> 4, l = (e - NH) & 15;
+            const uint64_t w = ar.ca1 - (uint64_t)a.C0 * ar.ca;
+            ct.tl[n][d][l] = cis((uint64_t)l * ar.ca + (d ? w : 0ull));
+        }
+    }
+    __syncthreads();
     float acc[WA][4][6];
 #pragma unroll
     for (int i = 0; i < WA; ++i)
@@ -115,13 +171,16 @@ __device__ __forceinline__ void adjoint_epilogue(const AdjArgs &a, const float *
         const bool valid = col < a.C && jj < a.f1;
         const float sc = valid ? sq[a.R + col] : 0.0f;
         const float wj = (float)jj - a.jc, wm = (float)mm - a.mc;
+        const int h = wc0 / 16 + j;
+        const int dl = li >= ct.wpos[h] ? 1 : 0;
 #pragma unroll
         for (int i = 0; i < WA; ++i)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const AtomRec &ar = at[wa0 + 16 * i + 4 * lg + v];
-                float2 b = cis(ar.p0 + (uint64_t)jj * ar.ca + (uint64_t)mm * ar.ca1);
-                const float d = valid ? __builtin_amdgcn_exp2f(-ar.pad * LOG2E * sc) : 0.0f;
+                const int n = wa0 + 16 * i + 4 * lg + v;
+                const float2 th = ct.th[n][h], tl = ct.tl[n][dl][li];
+                float2 b = make_float2(th.x * tl.x - th.y * tl.y, th.x * tl.y + th.y * tl.x);
+                const float d = valid ? __builtin_amdgcn_exp2f(-at[n].pad * LOG2E * sc) : 0.0f;
                 b = make_float2(b.x * d, b.y * d);
                 const float z0r = accR[0][i][j][v], z0i = accI[0][i][j][v];
                 const float tr = z0r * b.x - z0i * b.y;
@@ -189,11 +248,14 @@ __global__ void __launch_bounds__(NT) adjoint_kernel(AdjArgs a) {
             }
 
     const int nst = (a.R + KR - 1) / KR;
-    load_stage(sm, 0, a, sq, 0, col0, na, qbase);
+    const bool tab = nst <= MAXST;
+    if (tab) build_row_tables(sm.rt, sm.at, nst);
+    __syncthreads();
+    load_stage(sm, 0, a, sq, 0, col0, na, qbase, tab);
     __syncthreads();
     for (int s = 0; s < nst; ++s) {
         const int buf = s & 1;
-        if (s + 1 < nst) load_stage(sm, buf ^ 1, a, sq, (s + 1) * KR, col0, na, qbase);
+        if (s + 1 < nst) load_stage(sm, buf ^ 1, a, sq, s + 1, col0, na, qbase, tab);
         const bool im = (lg & 1) != 0;
 #pragma unroll
         for (int ks = 0; ks < KR / 2; ++ks) {
@@ -225,7 +287,7 @@ __global__ void __launch_bounds__(NT) adjoint_kernel(AdjArgs a) {
         __syncthreads();
     }
 
-    adjoint_epilogue(a, sq, sm.at, sm.red, accR, accI, col0, atom0, na);
+    adjoint_epilogue(a, sq, sm.at, sm.red, sm.ct, accR, accI, col0, atom0, na);
 }
 
 // ---------------------------------------------------------------------------
@@ -246,47 +308,33 @@ struct AdSmemS {
     u32x4 a[2][2][3][2][4][NA];   // [buffer][form (re,-im) | (im,re)][weight][hi | lo][row group][atom]
     AtomRec at[NA];
     float red[NWC][NA][6];
+    ColTables ct;
+    RowTables rt;
 };
 
-// A of stage rows r0..r0+15: thread (atom, row group g, half hf) makes rows
-// 4g + 2hf + {0, 1}, all three weights, both forms, hi and lo.
+// A of stage st (rows 16 st .. 16 st + 15): thread (atom, row group g, e)
+// makes row 4g + e, all three weights, both forms, hi and lo.
 __device__ __forceinline__ void gen_stage_split(AdSmemS &sm, int buf, const AdjArgs &a, const float *__restrict__ sq,
-                                                int r0, int na) {
+                                                int st, int na, bool tab) {
     const int t = threadIdx.x;
-    const int at = t & (NA - 1), g = (t >> 6) & 3, hf = t >> 8;
-    uint32_t w[2][3][2][2];
-#pragma unroll
-    for (int e = 0; e < 2; ++e) {
-        const int r = r0 + 4 * g + 2 * hf + e;
-        float2 v = make_float2(0.f, 0.f);
-        float s2 = 0.0f;
-        if (r < a.R && at < na) {
-            const AtomRec &ar = sm.at[at];
-            s2 = sq[r];
-            v = cis((uint64_t)r * ar.ra);
-            const float d = __builtin_amdgcn_exp2f(-ar.pad * LOG2E * s2);
-            v = make_float2(v.x * d, v.y * d);
-        }
-        const float wt[3] = {1.0f, (float)r - a.rc, s2};
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            uint32_t hi, lo;
-            bsplit(v.x * wt[k], v.y * wt[k], hi, lo);
-            w[0][k][0][e] = hi ^ 0x80000000u;
-            w[1][k][0][e] = rot16(hi);
-            w[0][k][1][e] = lo ^ 0x80000000u;
-            w[1][k][1][e] = rot16(lo);
-        }
+    const int at = t & (NA - 1), g = (t / NA) & 3, e = t / (4 * NA);
+    const int rr = 4 * g + e, r = st * SKR + rr;
+    float2 v = make_float2(0.f, 0.f);
+    float s2 = 0.0f;
+    if (r < a.R && at < na) {
+        s2 = sq[r];
+        v = row_element(sm.rt, sm.at[at], at, st, rr, r, tab, s2);
     }
+    const float wt[3] = {1.0f, (float)r - a.rc, s2};
 #pragma unroll
-    for (int f = 0; f < 2; ++f)
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-#pragma unroll
-            for (int p = 0; p < 2; ++p) {
-                uint32_t *dst = reinterpret_cast<uint32_t *>(&sm.a[buf][f][k][p][g][at]) + 2 * hf;
-                *reinterpret_cast<uint2 *>(dst) = make_uint2(w[f][k][p][0], w[f][k][p][1]);
-            }
+    for (int k = 0; k < 3; ++k) {
+        uint32_t hi, lo;
+        bsplit(v.x * wt[k], v.y * wt[k], hi, lo);
+        reinterpret_cast<uint32_t *>(&sm.a[buf][0][k][0][g][at])[e] = hi ^ 0x80000000u;
+        reinterpret_cast<uint32_t *>(&sm.a[buf][1][k][0][g][at])[e] = rot16(hi);
+        reinterpret_cast<uint32_t *>(&sm.a[buf][0][k][1][g][at])[e] = lo ^ 0x80000000u;
+        reinterpret_cast<uint32_t *>(&sm.a[buf][1][k][1][g][at])[e] = rot16(lo);
+    }
 }
 
 __global__ void __launch_bounds__(NT) adjoint_split_kernel(AdjArgs a) {
@@ -335,15 +383,18 @@ __global__ void __launch_bounds__(NT) adjoint_split_kernel(AdjArgs a) {
     };
 
     const int nst = (a.R + SKR - 1) / SKR;
+    const bool tab = nst <= MAXST;
+    if (tab) build_row_tables(sm.rt, sm.at, nst);
+    __syncthreads();
     u32x4 bh[WC], bl[WC], nbh[WC], nbl[WC];
     load_b(0, bh, bl);
-    gen_stage_split(sm, 0, a, sq, 0, na);
+    gen_stage_split(sm, 0, a, sq, 0, na, tab);
     __syncthreads();
     for (int s = 0; s < nst; ++s) {
         const int buf = s & 1;
         if (s + 1 < nst) {
             load_b(s + 1, nbh, nbl);
-            gen_stage_split(sm, buf ^ 1, a, sq, (s + 1) * SKR, na);
+            gen_stage_split(sm, buf ^ 1, a, sq, s + 1, na, tab);
         }
 #pragma unroll
         for (int i = 0; i < WA; ++i) {
@@ -377,7 +428,7 @@ __global__ void __launch_bounds__(NT) adjoint_split_kernel(AdjArgs a) {
         }
         __syncthreads();
     }
-    adjoint_epilogue(a, sq, sm.at, sm.red, accR, accI, col0, atom0, na);
+    adjoint_epilogue(a, sq, sm.at, sm.red, sm.ct, accR, accI, col0, atom0, na);
 }
 
 // FP32 complex Q planes -> split words: for plane t, part (hi, lo), row
