@@ -35,7 +35,7 @@ EXPORTS = [
     "dc_create", "dc_destroy", "dc_last_error", "dc_last_error_type", "dc_set_precision",
     "dc_get_precision", "dc_set_gemm_tile", "dc_gemm_num_tiles", "dc_gemm_tile_shape", "dc_set_cancel_flag", "dc_version", "dc_device_info", "dc_strucf",
     "dc_crystal_set", "dc_four_run", "dc_four_run_dev", "dc_last_stats", "dc_synchronize",
-    "dc_model_intensity", "dc_model_vjp", "dc_model_chi2",
+    "dc_model_intensity", "dc_model_vjp", "dc_model_chi2", "dc_model_jvp",
     "dc_batch_chi2", "dc_set_autotune", "dc_set_graphs", "dc_batch_info",
 ]
 
@@ -108,6 +108,7 @@ def lib():
     L.dc_model_intensity.argtypes = [C.c_void_p, C.POINTER(ModelDesc), _dp, _dp]
     L.dc_model_vjp.argtypes = [C.c_void_p, C.POINTER(ModelDesc), _dp, _dp, _dp]
     L.dc_model_chi2.argtypes = [C.c_void_p, C.POINTER(ModelDesc), _dp, _dp, _dp, _dp, _dp]
+    L.dc_model_jvp.argtypes = [C.c_void_p, C.POINTER(ModelDesc), _dp, C.c_int, _dp, _dp, _dp]
     L.dc_batch_chi2.argtypes = [C.c_void_p, C.POINTER(BatchDesc), _dp, _dp, _dp, _dp, _dp]
     L.dc_set_autotune.argtypes = [C.c_void_p, C.c_int]
     L.dc_set_graphs.argtypes = [C.c_void_p, C.c_int]
@@ -363,6 +364,20 @@ class Context:
         grad = np.zeros(5 * desc.natoms)
         self.check(lib().dc_model_vjp(self.h, C.byref(desc), _d(f), _d(g), _d(grad)))
         return grad.reshape(-1, 5)
+
+    def model_jvp(self, desc, fcalc, dparam, want_df=False):
+        """dI (P, ndir) [and dF] along directions dparam (ndir, N, 5) (grad layout)."""
+        dp = np.ascontiguousarray(np.asarray(dparam, np.float64).reshape(-1, desc.natoms, 5))
+        nd = dp.shape[0]
+        f = np.ascontiguousarray(np.asarray(fcalc, np.complex128).ravel(order="F")).view(np.float64)
+        P = int(np.prod(list(desc.inc)))
+        di = np.zeros(P * nd)
+        df = np.zeros(2 * P * nd) if want_df else None
+        self.check(lib().dc_model_jvp(self.h, C.byref(desc), _d(f), nd, _d(dp), _d(di), _d(df)))
+        di = di.reshape((P, nd), order="F")
+        if want_df:
+            return di, df.view(np.complex128).reshape((P, nd), order="F")
+        return di
 
     def model_chi2(self, desc, target, weight=None, want_grad=True):
         """(chi2, I, grad (N, 5) or None) for chi2 = sum w (I - target)^2."""

@@ -564,6 +564,48 @@ __global__ void batch_reduce_kernel(BatchReduceArgs a) {
     if (threadIdx.x == 0) a.chi_part[(long long)m * gridDim.x + blockIdx.x] = red[0];
 }
 
+__global__ void jvp_reduce_kernel(JvpArgs a) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.g.P) return;
+    const Grid &g = a.g;
+    const int i = (int)(p % g.f[0]);
+    const long long q = p / g.f[0];
+    const int j = (int)(q % g.f[1]), l = (int)(q / g.f[1]);
+    int bin = istl_point(g, i, j, l);
+    bin = bin >= a.ncfact ? a.ncfact - 1 : bin;
+    double h[3];
+    for (int k = 0; k < 3; ++k) h[k] = g.xm[k] + g.uin[k] * i + g.vin[k] * j + g.win[k] * l;
+    double s2 = 0.0;
+    for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < 3; ++k) s2 += h[r] * g.rten[r + 3 * k] * h[k];
+    s2 *= 0.25;
+    const long long po = i * g.ps[0] + j * g.ps[1] + l * g.ps[2];
+    const double fr = a.f[2 * p], fi = a.f[2 * p + 1];
+    const double twopi = 6.283185307179586476925;
+    for (int d = 0; d < a.ndir; ++d) {
+        double gr[5], gi[5];
+        for (int c = 0; c < 5; ++c) {
+            gr[c] = 0.0;
+            gi[c] = 0.0;
+            for (int sl = a.dslice[5 * d + c]; sl < a.dslice[5 * d + c + 1]; ++sl) {
+                const float2 v = a.partial[(long long)sl * g.plane + po];
+                const double2 u = a.cfact[(long long)a.slice_type[sl] * a.ncfact + bin];
+                gr[c] += u.x * (double)v.x - u.y * (double)v.y;
+                gi[c] += u.x * (double)v.y + u.y * (double)v.x;
+            }
+        }
+        const double hr = h[0] * gr[1] + h[1] * gr[2] + h[2] * gr[3];
+        const double hi = h[0] * gi[1] + h[1] * gi[2] + h[2] * gi[3];
+        const double dr = gr[0] - twopi * hi - s2 * gr[4];
+        const double di = gi[0] + twopi * hr - s2 * gi[4];
+        a.di[(long long)d * g.P + p] = 2.0 * (fr * dr + fi * di);
+        if (a.df) {
+            a.df[2 * ((long long)d * g.P + p)] = dr;
+            a.df[2 * ((long long)d * g.P + p) + 1] = di;
+        }
+    }
+}
+
 __global__ void fill_zero_kernel(double *p, long long n) {
     for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x)
         p[k] = 0.0;
@@ -574,6 +616,12 @@ __global__ void fill_zero_kernel(double *p, long long n) {
 hipError_t launch_batch_reduce(const BatchReduceArgs &a, int nblocks, hipStream_t st) {
     if (a.nmembers <= 0) return hipSuccess;
     batch_reduce_kernel<<<dim3(nblocks, a.nmembers), 256, 0, st>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_jvp_reduce(const JvpArgs &a, hipStream_t st) {
+    if (a.g.P <= 0 || a.ndir <= 0) return hipSuccess;
+    jvp_reduce_kernel<<<(unsigned)((a.g.P + 255) / 256), 256, 0, st>>>(a);
     return hipGetLastError();
 }
 

@@ -164,6 +164,23 @@ struct BatchReduceArgs {
 };
 hipError_t launch_batch_reduce(const BatchReduceArgs &a, int nblocks, hipStream_t st);
 hipError_t launch_fill_zero(double *p, long long n, hipStream_t st);
+// Forward-mode derivative (JVP) of I along ndir parameter directions: slice
+// group 5k + q of the GEMM holds direction k weighted by q = (docc,
+// occ dx, occ dy, occ dz, occ dB); per pixel
+//   dF_k = sum_t f_t(bin) [G0 + 2 pi i h.(G1, G2, G3) - s^2 G4],  dI_k = 2 Re(conj(F) dF_k).
+struct JvpArgs {
+    Grid g;                 // grid with the GEMM plan's ps / plane
+    const float2 *partial;
+    const int *dslice;      // 5 ndir + 1 slice boundaries (relative to partial)
+    const int *slice_type;  // per slice (relative)
+    const double2 *cfact;
+    int ncfact;
+    const double *f;        // F over the grid (complex)
+    int ndir;
+    double *di;             // (P, ndir)
+    double *df;             // (P, ndir) complex or null
+};
+hipError_t launch_jvp_reduce(const JvpArgs &a, hipStream_t st);
 int adjoint_tile_atoms();
 int adjoint_tile_cols();
 // split = true: BF16x3 MFMA on qs (made by launch_q_split), else FP32 on q.

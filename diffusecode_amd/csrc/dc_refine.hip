@@ -17,6 +17,7 @@ using namespace dc;
 namespace {
 
 constexpr int CHI2_BLOCKS = 1024;
+constexpr double JVP_PARTIAL_BUDGET = 4.0e9;   // bytes of slice planes per batch of directions
 
 struct ModelPlan {
     Grid g{};
@@ -322,6 +323,98 @@ int dc_model_vjp(dc_ctx *c, const dc_model_desc *m, const double *fcalc, const d
     if ((st = model_adjoint(c, m, mp))) return st;
     HIPCHK(hipMemcpyAsync(grad, c->m_grad.p, sizeof(double) * 5 * m->natoms, hipMemcpyDeviceToHost, c->st));
     return model_end(c);
+}
+
+int dc_model_jvp(dc_ctx *c, const dc_model_desc *m, const double *fcalc, int ndir, const double *dparam,
+                 double *di, double *df) {
+    if (!c) return DC_ERR_ARG;
+    if (!fcalc || !dparam || !di || ndir < 0) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "dc_model_jvp: null pointer or bad ndir");
+    ModelPlan mp;
+    int st;
+    if ((st = model_begin(c, m, mp))) return st;
+    const GemmGeom &gg = mp.pl.gg;
+    const long long P = mp.g.P;
+    const int ns = mp.nsorted;
+    if ((st = upload(c, c->m_f, fcalc, 2 * (size_t)P))) return st;
+    if ((st = upload(c, c->cfact, m->cfact, 2 * (size_t)m->ncfact * m->nscat))) return st;
+    if (m->bdw && (st = upload(c, c->m_bd, m->bdw, (size_t)m->natoms))) return st;
+    if ((st = ensure(c, c->m_di, sizeof(double) * (size_t)P * std::max(ndir, 1) * (df ? 3 : 1)))) return st;
+    // directions per batch so that the slice planes fit the budget
+    const int kchunk = kchunk_for(mp.pl, (long long)ns * 5);
+    long long slices_per_dir = 0;
+    for (auto &sg : mp.segs) slices_per_dir += 5 * ((sg.second + kchunk - 1) / kchunk);
+    const double dir_bytes = (double)std::max<long long>(slices_per_dir, 1) * gg.P * sizeof(float2);
+    const int per = std::max(1, std::min(ndir, (int)(JVP_PARTIAL_BUDGET / dir_bytes)));
+    HIPCHK(hipEventRecord(c->ev[4], c->st));
+    for (int d0 = 0; d0 < ndir; d0 += per) {
+        if (cancelled(c)) return set_err(c, DC_ERR_CANCEL, DC_ER_COMM, "cancelled");
+        const int nd = std::min(per, ndir - d0);
+        // weights per (direction, component) in sorted order, records, slices
+        std::vector<float> w((size_t)nd * 5 * ns);
+        std::vector<int> idx((size_t)nd * 5 * ns);
+        for (int k = 0; k < nd; ++k)
+            for (int q = 0; q < 5; ++q)
+                for (int t = 0; t < ns; ++t) {
+                    const int n = mp.order[t];
+                    const double *dp = dparam + 5 * ((size_t)(d0 + k) * m->natoms + n);
+                    const double o = m->occ[n];
+                    const double v = q == 0 ? dp[3] : (q < 4 ? o * dp[q - 1] : o * dp[4]);
+                    w[((size_t)k * 5 + q) * ns + t] = (float)v;
+                    idx[((size_t)k * 5 + q) * ns + t] = n;
+                }
+        SliceSet ss;
+        std::vector<int> dsl;
+        int off = 0;
+        for (int g5 = 0; g5 < 5 * nd; ++g5) {
+            dsl.push_back((int)ss.type.size());
+            add_segments(ss, mp.segs, kchunk, off);
+        }
+        dsl.push_back((int)ss.type.size());
+        if (ss.beg.empty()) ss.beg.push_back(0);
+        const int nslice = (int)ss.type.size();
+        if ((st = upload(c, c->m_jw, w.data(), w.size()))) return st;
+        if ((st = upload(c, c->m_jidx, idx.data(), idx.size()))) return st;
+        if ((st = upload(c, c->m_dsl, dsl.data(), dsl.size()))) return st;
+        if ((st = upload(c, c->slice_beg, ss.beg.data(), ss.beg.size()))) return st;
+        if (nslice > 0 && (st = upload(c, c->slice_type, ss.type.data(), ss.type.size()))) return st;
+        if ((st = ensure(c, c->rec, sizeof(AtomRec) * std::max<size_t>(w.size(), 1)))) return st;
+        if ((st = ensure(c, c->partial, sizeof(float2) * gg.P * std::max(nslice, 1)))) return st;
+        PrepArgs pa = base_prep(mp.g, 0);
+        pa.src = ptr<double>(c->m_pos);
+        pa.src_ld = 0;
+        pa.idx = ptr<int>(c->m_jidx);
+        pa.w = ptr<float>(c->m_jw);
+        pa.bdw_src = m->bdw ? ptr<double>(c->m_bd) : nullptr;
+        pa.n = (long long)w.size();
+        if (pa.n > 0) HIPCHK(launch_prep(pa, ptr<AtomRec>(c->rec), c->st));
+        if ((st = run_gemm(c, mp.pl, nslice, ptr<float2>(c->partial), mp.any_b ? ptr<float>(c->m_sq) : nullptr))) return st;
+        JvpArgs ja{};
+        ja.g = mp.g;
+        ja.partial = ptr<float2>(c->partial);
+        ja.dslice = ptr<int>(c->m_dsl);
+        ja.slice_type = ptr<int>(c->slice_type);
+        ja.cfact = ptr<double2>(c->cfact);
+        ja.ncfact = m->ncfact;
+        ja.f = ptr<double>(c->m_f);
+        ja.ndir = nd;
+        ja.di = ptr<double>(c->m_di) + (size_t)d0 * P;
+        ja.df = df ? ptr<double>(c->m_di) + (size_t)ndir * P + 2 * (size_t)d0 * P : nullptr;
+        HIPCHK(launch_jvp_reduce(ja, c->st));
+        c->stats.pairs += (double)w.size() * (double)P;
+        c->stats.slices += nslice;
+    }
+    HIPCHK(hipEventRecord(c->ev[5], c->st));
+    if (ndir > 0) {
+        HIPCHK(hipMemcpyAsync(di, c->m_di.p, sizeof(double) * (size_t)P * ndir, hipMemcpyDeviceToHost, c->st));
+        if (df)
+            HIPCHK(hipMemcpyAsync(df, ptr<double>(c->m_di) + (size_t)ndir * P, sizeof(double) * 2 * (size_t)P * ndir,
+                                  hipMemcpyDeviceToHost, c->st));
+    }
+    if ((st = model_end(c))) return st;
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[4], c->ev[5]));
+    c->stats.forward_ms = ms;
+    return DC_OK;
 }
 
 int dc_model_chi2(dc_ctx *c, const dc_model_desc *m, const double *target, const double *weight, double *chi2,

@@ -61,6 +61,12 @@ class RefineModel:
         d, keep = self.desc(pos, occ, bdw)
         return self.ctx.model_vjp(d, fcalc, gi)
 
+    def jvp_np(self, pos, occ, bdw, fcalc, directions):
+        """dI along directions (ndir, N, 5): the per-pixel derivative columns
+        REFINE's least-squares loop needs (one per refined parameter)."""
+        d, keep = self.desc(pos, occ, bdw)
+        return self.ctx.model_jvp(d, fcalc, directions)
+
     def chi2_np(self, pos, occ, bdw, target, weight=None, want_grad=True):
         d, keep = self.desc(pos, occ, bdw)
         return self.ctx.model_chi2(d, target, weight, want_grad)

@@ -216,6 +216,17 @@ int dc_model_intensity(dc_ctx *ctx, const dc_model_desc *model, double *icalc, d
  */
 int dc_model_vjp(dc_ctx *ctx, const dc_model_desc *model, const double *fcalc, const double *gi, double *grad);
 /*
+ * Jacobian-vector products (forward mode): for ndir directions
+ * dparam(5, natoms, ndir) in the layout of grad, writes
+ *   di(P, ndir) = dI/dt along each direction, and dF (complex, NULL-able),
+ * given F of the model (fcalc from dc_model_intensity).  These are the
+ * per-pixel derivative columns REFINE's Levenberg-Marquardt loop builds by
+ * finite differences (refine/prog/refine_run.f90:491-600, gl_is_der hook
+ * :550-552): one direction per refined parameter.
+ */
+int dc_model_jvp(dc_ctx *ctx, const dc_model_desc *model, const double *fcalc, int ndir, const double *dparam,
+                 double *di, double *df);
+/*
  * chi^2 = sum_h w(h) (I(h) - target(h))^2 (weight NULL = 1) and its gradient
  * grad(5, natoms) in one call (forward, residual and adjoint stay on the
  * device).  icalc and grad are NULL-able.

@@ -181,3 +181,36 @@ def test_autograd_functions(ctx):
     assert abs(chi.item() - chi2.item()) <= 1e-9 * chi2.item()
     for a, t in zip(g1, (pos, o, b)):
         assert (a - t.grad).abs().max().item() <= 1e-6 * t.grad.abs().max().item()
+
+
+@pytest.mark.parametrize("prec,bar", [("fp32", 1e-5), ("bf16x3", 1e-4)])
+def test_model_jvp_vs_central_differences_and_vjp(ctx, prec, bar):
+    """Jacobian-vector products along random directions: against FP64 central
+    differences of the oracle intensity (h = 1e-6, per pixel, relative to the
+    largest |dI|), and dual to the vjp: <G, J v> = <J^T G, v>."""
+    cr, occ, bdw, cf, eck, vi, inc = problem(ncell=(3, 3, 2), n=(25, 17, 9))
+    rten = cr.rten()
+    rng = np.random.default_rng(21)
+    n = len(cr.iscat)
+    dirs = rng.standard_normal((3, n, 5))
+    dirs[:, :, :3] *= 0.01
+    d, keep = ctx.model_desc(cr.pos, cr.iscat, occ, bdw, cf, eck, vi, inc, rten)
+    ctx.set_precision(prec)
+    try:
+        icalc, fcalc = ctx.model_intensity(d)
+        di = ctx.model_jvp(d, fcalc, dirs)
+        gi = rng.standard_normal(icalc.size)
+        g = ctx.model_vjp(d, fcalc, gi)
+    finally:
+        ctx.set_precision("fp32")
+    h = 1e-6
+    for k in range(dirs.shape[0]):
+        v = dirs[k]
+        Fp = O.model_f(cr.pos + h * v[:, :3], cr.iscat, occ + h * v[:, 3], bdw + h * v[:, 4], cf, eck, vi, inc, rten)
+        Fm = O.model_f(cr.pos - h * v[:, :3], cr.iscat, occ - h * v[:, 3], bdw - h * v[:, 4], cf, eck, vi, inc, rten)
+        fd = ((np.abs(Fp) ** 2 - np.abs(Fm) ** 2) / (2 * h)).ravel(order="F")
+        e = np.abs(di[:, k] - fd).max() / np.abs(fd).max()
+        dual = abs(gi @ di[:, k] - (g * v).sum()) / (np.abs(gi) @ np.abs(di[:, k]))
+        print(f"  {prec} direction {k}: max|dI - fd|/max|fd| = {e:.2e}, duality {dual:.2e}")
+        assert e <= bar
+        assert dual <= bar
