@@ -425,15 +425,19 @@ phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice
 
         const int nst = (kend - kbeg + KS - 1) / KS;
         // records of this thread's atom for the stages whose tables come next,
-        // loaded one stage ahead so the global-memory latency is hidden
+        // loaded ahead of use so the global-memory latency is hidden
         const int at = pt & (KS - 1);
         auto fetch = [&](int st) {
             const int k = kbeg + st * KS + at;
             return rec[k < kend ? k : kend - 1];
         };
+        // wide column blocks are register-bound: they load each record one
+        // stage ahead only, the others three stages ahead
+        constexpr bool DEEP = T::WC <= 4;
         const AtomRec r0 = fetch(0);
         const AtomRec r1 = fetch(nst > 1 ? 1 : 0);
-        AtomRec rn = fetch(nst > 2 ? 2 : 0);
+        AtomRec rn = r1;
+        if constexpr (DEEP) rn = fetch(nst > 2 ? 2 : 0);
         __syncthreads();
         build_tables<T>(sm.t[0], sm.a_k16, sm.b_k16, r0, kbeg + at < kend, sa, sb, pt);
         __syncthreads();
@@ -442,8 +446,13 @@ phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice
         __syncthreads();
         for (int s = 0; s < nst; ++s) {
             const int cur = s & 1, nxt = cur ^ 1;
-            const AtomRec rt = rn;
-            if (s + 3 < nst) rn = fetch(s + 3);
+            AtomRec rt;
+            if constexpr (DEEP) {
+                rt = rn;
+                if (s + 3 < nst) rn = fetch(s + 3);
+            } else {
+                rt = fetch(s + 2 < nst ? s + 2 : 0);
+            }
             if (s + 1 < nst) build_elems<SCHEME, T, DAMP>(sm.e[nxt], sm.t[nxt], sm.a_wpos, sm.b_wpos, sm.s_a, sm.s_b, wave);
             if (s + 2 < nst) build_tables<T>(sm.t[cur], sm.a_k16, sm.b_k16, rt, kbeg + (s + 2) * KS + at < kend, sa, sb, pt);
             if (live) mfma_stage<SCHEME, T>(sm.e[cur], accR, accI, wr0, wc0);
