@@ -188,7 +188,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU sharding of this path is described in DESIGN.md, not measured.", file=sys.stderr)
+              "this path shards over hkl slabs, one slab per GPU, which is not measured here.", file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)

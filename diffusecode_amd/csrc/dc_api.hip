@@ -22,6 +22,13 @@ namespace dc {
 constexpr int GEMM_KS = 16;
 constexpr int KCHUNK_MIN = 512;
 constexpr int KCHUNK_MAX = 16384;
+// FP32 scheme: atoms per slice.  The FP32 kernel adds groups of 64 atoms
+// into a second set of FP32 sums; with 1,024-atom slices summed in FP64 the
+// outer sum has 16 terms, keeping max|dI|/max(I) near 1e-7 on coherent sums
+// (Bragg peaks of displaced crystals), where one FP32 accumulator over
+// thousands of atoms reaches 1e-6.  The split schemes round once per 16 atoms
+// and keep long slices.
+constexpr int KCHUNK_FP32 = 1024;
 constexpr double PARTIAL_BUDGET = 6.0e9;   // bytes of slice planes per batch
 
 }  // namespace dc
@@ -147,7 +154,8 @@ Plan plan_gemm(Grid &g, int forced, int only_mapping, int prec) {
     return best;
 }
 
-int kchunk_for(const Plan &pl, long long natoms_batch) {
+int kchunk_for(const Plan &pl, long long natoms_batch, int prec) {
+    if (prec == DC_PREC_FP32) return KCHUNK_FP32;
     TileInfo ti = gemm_tile_info(pl.tile);
     long long wgs = ((pl.gg.R + ti.rt - 1) / ti.rt) * ((pl.gg.C + ti.ct - 1) / ti.ct);
     long long want = std::max<long long>(1, (1024 + wgs - 1) / wgs);
@@ -391,7 +399,7 @@ int dc_strucf(dc_ctx *c, const double *xat, int64_t nxat, int64_t ld_xat, const 
 
     SliceSet ss;
     int off = 0;
-    add_segments(ss, {{0, (int)nxat}}, kchunk_for(pl, nxat), off);
+    add_segments(ss, {{0, (int)nxat}}, kchunk_for(pl, nxat, c->prec), off);
     if (ss.beg.empty()) ss.beg.push_back(0);
     int nslice = (int)ss.type.size();
     ss.lot = {0, nslice};
@@ -561,7 +569,7 @@ int gemm_segments(dc_ctx *c, const Grid &g, const Plan &pl, GatherList &gl,
     HIPCHK(launch_prep(pa, ptr<AtomRec>(c->rec), c->st));
     int off = 0;
     ss = SliceSet();
-    add_segments(ss, gl.segs, kchunk_for(pl, n), off);
+    add_segments(ss, gl.segs, kchunk_for(pl, n, c->prec), off);
     if (ss.beg.empty()) ss.beg.push_back(0);
     nslice = (int)ss.type.size();
     if ((st = upload(c, c->slice_beg, ss.beg.data(), ss.beg.size()))) return st;

@@ -185,7 +185,7 @@ int build_state(dc_ctx *c, const dc_batch_desc *d, BatchState &b) {
         }
     }
     b.nsorted = (int)b.order.size();
-    const int kchunk = kchunk_for(b.pl, b.nsorted);
+    const int kchunk = kchunk_for(b.pl, b.nsorted, b.prec);
     int off = 0;
     const int na = adjoint_tile_atoms();
     size_t sg = 0;

@@ -415,12 +415,27 @@ phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice
         }
 
         f32x4 accR[T::WR][T::WC], accI[T::WR][T::WC];
+        // FP32 scheme: two-level accumulation.  The 16x16x4 MFMA rounds its
+        // FP32 accumulator once per 2 atoms; on coherent sums (Bragg peaks of
+        // displaced crystals) that error grows with the number of atoms, so
+        // the MFMAs accumulate 64 atoms (4 stages) at a time and the groups
+        // are added into a second set of FP32 sums.
+        constexpr bool TWO = SCHEME == SCH_FP32;
+        constexpr int SR = TWO ? T::WR : 1, SC = TWO ? T::WC : 1;
+        f32x4 sumR[SR][SC], sumI[SR][SC];
 #pragma unroll
         for (int i = 0; i < T::WR; ++i)
 #pragma unroll
             for (int j = 0; j < T::WC; ++j) {
                 accR[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
                 accI[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+            }
+#pragma unroll
+        for (int i = 0; i < SR; ++i)
+#pragma unroll
+            for (int j = 0; j < SC; ++j) {
+                sumR[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+                sumI[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
             }
 
         const int nst = (kend - kbeg + KS - 1) / KS;
@@ -456,7 +471,29 @@ phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice
             if (s + 1 < nst) build_elems<SCHEME, T, DAMP>(sm.e[nxt], sm.t[nxt], sm.a_wpos, sm.b_wpos, sm.s_a, sm.s_b, wave);
             if (s + 2 < nst) build_tables<T>(sm.t[cur], sm.a_k16, sm.b_k16, rt, kbeg + (s + 2) * KS + at < kend, sa, sb, pt);
             if (live) mfma_stage<SCHEME, T>(sm.e[cur], accR, accI, wr0, wc0);
+            if constexpr (TWO) {
+                if ((s & 3) == 3 || s + 1 == nst) {
+#pragma unroll
+                    for (int i = 0; i < SR; ++i)
+#pragma unroll
+                        for (int j = 0; j < SC; ++j) {
+                            sumR[i][j] += accR[i][j];
+                            sumI[i][j] += accI[i][j];
+                            accR[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+                            accI[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+                        }
+                }
+            }
             __syncthreads();
+        }
+        if constexpr (TWO) {
+#pragma unroll
+            for (int i = 0; i < SR; ++i)
+#pragma unroll
+                for (int j = 0; j < SC; ++j) {
+                    accR[i][j] = sumR[i][j];
+                    accI[i][j] = sumI[i][j];
+                }
         }
 
         // epilogue: lane holds rows 4*lg + v (v = 0..3) of column li of each 16x16 tile
@@ -488,9 +525,12 @@ using CfgF = TileCfg<1, 4, 1, 4>;   //  16 x 256 (lines / few rows)
 using CfgG = TileCfg<2, 11, 8, 1>;  // 256 x 176 (flattened rows x one short axis, e.g. 161)
 using CfgH = TileCfg<1, 11, 8, 1>;  // 128 x 176
 
+// split_only: tiles whose FP32 variant (two accumulator sets) spills so much
+// that it is slower than the 96x256 tile (which spills a little and is still
+// the fastest FP32 tile, measured); the planner does not offer them to FP32.
 static const TileInfo kTiles[] = {
-    {CfgA::RT, CfgA::CT, CfgA::NT, 0}, {CfgB::RT, CfgB::CT, CfgB::NT, 0}, {CfgC::RT, CfgC::CT, CfgC::NT, 0},
-    {CfgD::RT, CfgD::CT, CfgD::NT, 0}, {CfgE::RT, CfgE::CT, CfgE::NT, 0}, {CfgF::RT, CfgF::CT, CfgF::NT, 0},
+    {CfgA::RT, CfgA::CT, CfgA::NT, 1}, {CfgB::RT, CfgB::CT, CfgB::NT, 0}, {CfgC::RT, CfgC::CT, CfgC::NT, 0},
+    {CfgD::RT, CfgD::CT, CfgD::NT, 1}, {CfgE::RT, CfgE::CT, CfgE::NT, 0}, {CfgF::RT, CfgF::CT, CfgF::NT, 0},
     {CfgG::RT, CfgG::CT, CfgG::NT, 1}, {CfgH::RT, CfgH::CT, CfgH::NT, 1},
 };
 

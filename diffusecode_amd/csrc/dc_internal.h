@@ -56,7 +56,7 @@ __device__ inline int istl_point(const Grid &g, int i, int j, int l) {
 
 struct TileInfo {
     int rt, ct, nt;
-    int split_only;         // register-heavy tile used by the split schemes only (FP32 spills)
+    int split_only;         // tile offered to the split schemes only (its FP32 variant spills)
 };
 
 struct ReduceArgs {

@@ -99,15 +99,26 @@ __global__ void reduce_lots_kernel(ReduceArgs a) {
     for (int l = 0; l < a.nlots; ++l) {
         const int send = a.lot_slice[l + 1];
         double fr = 0.0, fi = 0.0;
-        for (; s < send; ++s) {
-            const float2 v = a.partial[(long long)(s - a.slice0) * a.g.plane + po];
+        while (s < send) {
+            // slices of a lot come in runs of one type: sum the run's planes,
+            // then apply that type's form factor once
+            const int t = a.slice_type ? a.slice_type[s] : 0;
+            int e = s + 1;
+            while (e < send && (a.slice_type ? a.slice_type[e] : 0) == t) ++e;
+            double sr = 0.0, si = 0.0;
+#pragma unroll 4
+            for (; s < e; ++s) {
+                const float2 v = a.partial[(long long)(s - a.slice0) * a.g.plane + po];
+                sr += (double)v.x;
+                si += (double)v.y;
+            }
             if (a.lform) {
-                const double2 cf = a.cfact[(long long)a.slice_type[s] * a.ncfact + bin];
-                fr += cf.x * (double)v.x - cf.y * (double)v.y;
-                fi += cf.x * (double)v.y + cf.y * (double)v.x;
+                const double2 cf = a.cfact[(long long)t * a.ncfact + bin];
+                fr += cf.x * sr - cf.y * si;
+                fi += cf.x * si + cf.y * sr;
             } else {
-                fr += (double)v.x;
-                fi += (double)v.y;
+                fr += sr;
+                fi += si;
             }
         }
         if (a.wm) {

@@ -1,0 +1,149 @@
+"""GPU edge cases through the C ABI, each against the oracle or the documented
+status code: empty and single-atom inputs, a type without atoms, a slice
+boundary (KCHUNK_MAX + 1 atoms of one type), a long 1D line, the largest
+sin(theta)/lambda bin, and the DISCUS error paths (-3, -14, -132, -162, -198)
+that must return before any kernel runs and leave the outputs untouched."""
+import numpy as np
+import pytest
+
+from diffusecode_amd import _lib
+from diffusecode_amd import crystal as X
+from diffusecode_amd import fourier as FR
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+BAR = {"fp32": 1e-6, "bf16x3": 1e-5}
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = _lib.Context()
+    yield c
+    c.close()
+
+
+def rel(F, Fref):
+    I, Iref = np.abs(F) ** 2, np.abs(Fref) ** 2
+    return float(np.abs(I - Iref).max() / Iref.max())
+
+
+def layer(shape, lo=(-2.0, -1.5, 0.0), step=0.1):
+    hi = [l + (n - 1) * step for l, n in zip(lo, shape)]
+    return FR.grid_block(lo, hi, shape)
+
+
+def test_layer_call_without_atoms_is_zero(ctx):
+    eck, vi, num = layer((13, 7, 3))
+    st, lmn, off = O.fourier_lmn(eck, vi, num)
+    F = ctx.strucf(np.zeros((0, 3)), vi[:, 0], vi[:, 1], vi[:, 2], off, lmn, num)
+    assert F.shape == tuple(num) and np.all(F == 0)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_single_atom_and_slice_boundary(ctx, prec):
+    """1 atom, and 16,385 atoms of one type (one full 16,384-atom slice plus a
+    1-atom slice), on a grid through integer hkl.  A single atom has no
+    averaging over atoms, so BF16x3 is held to its per-product bound: the
+    dropped lo*lo term and the rounding of lo leave ~2^-17 per real product,
+    two products per component, |dF| <= 2e-5 and |dI|/max(I) <= 4e-5."""
+    eck, vi, num = layer((21, 11, 1), lo=(-1.0, -0.5, 0.0))
+    st, lmn, off = O.fourier_lmn(eck, vi, num)
+    c = X.rocksalt(n=13, vacancy=0.0, seed=2)
+    for pos in (np.array([[0.3, 0.1, 0.7]]), c.pos[:16385]):
+        ctx.set_precision(prec)
+        try:
+            F = ctx.strucf(pos, vi[:, 0], vi[:, 1], vi[:, 2], off, lmn, num)
+        finally:
+            ctx.set_precision("fp32")
+        corner = sum(int(m) * np.asarray(v) for m, v in zip(lmn, [vi[:, 0], vi[:, 1], vi[:, 2], off[:, 0], off[:, 1], off[:, 2]]))
+        Fref = O.strucf_exact(pos, corner, vi[:, 0], vi[:, 1], vi[:, 2], num)
+        err = rel(F, Fref)
+        print(f"  {prec} {pos.shape[0]} atoms: {err:.3e}")
+        assert err <= (4e-5 if (prec == "bf16x3" and pos.shape[0] == 1) else BAR[prec])
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_type_without_atoms_and_long_line(ctx, prec):
+    """A scattering type with no atoms in the crystal, on a centred 1D line of
+    2001 points through the Bragg peaks h00 (Friedel-halved to 1001)."""
+    c = X.rocksalt(n=4, seed=3)
+    c = X.Crystal(c.pos, c.iscat, c.icc, c.ncatoms, c.lattice, np.vstack([c.scat, c.scat[:1]]),
+                  np.append(c.occ, 1.0), np.append(c.dw, 0.0), name="rocksalt+empty type")
+    eck, vi, inc = FR.grid_block((-5.0, 0.0, 0.0), (5.0, 0.0, 0.0), (2001, 1, 1))
+    cf = FR.formtab(c)
+    ctx.crystal_set(c)
+    ctx.set_precision(prec)
+    try:
+        desc, keep = ctx.run_desc(eck, vi, inc, c.rten(), cf)
+        gpu = ctx.four_run(desc)
+    finally:
+        ctx.set_precision("fp32")
+    ref = O.four_run(c, eck, vi, inc, cf)
+    err = float(np.abs(gpu["dsi"] - ref["dsi"]).max() / ref["dsi"].max())
+    print(f"  {prec}: {err:.3e}")
+    assert err <= BAR[prec]
+
+
+def test_largest_sin_theta_over_lambda_bin(ctx):
+    """A corner at s just below CFPKT * 0.001 (bin 9999) runs; one step past it is -3."""
+    c = X.rocksalt(n=2, seed=4)
+    ctx.crystal_set(c)
+    cf = FR.formtab(c)
+    a = 5.64
+    hmax = 2 * 9.999 * a / np.sqrt(3.0)       # s = |h| / (2a) on the cubic diagonal
+    for h, ok in ((hmax * 0.9995, True), (hmax * 1.001, False)):
+        eck, vi, inc = FR.grid_block((h - 0.02, h - 0.02, h - 0.02), (h, h, h), (3, 3, 3))
+        desc, keep = ctx.run_desc(eck, vi, inc, c.rten(), cf)
+        if ok:
+            gpu = ctx.four_run(desc)
+            ref = O.four_run(c, eck, vi, inc, cf)
+            assert float(np.abs(gpu["dsi"] - ref["dsi"]).max() / ref["dsi"].max()) <= 1e-6
+        else:
+            with pytest.raises(_lib.DCError) as e:
+                ctx.four_run(desc)
+            assert e.value.ier_num == _lib.DC_ERR_ISTL
+
+
+def test_error_paths_return_before_any_kernel(ctx):
+    c = X.rocksalt(n=3, seed=5)
+    cf = FR.formtab(c)
+    eck, vi, inc = FR.grid_block((-2, -2, 0), (2, 2, 0), (21, 21, 1))
+    # -132: zero increment along an axis with more than one point
+    ctx.crystal_set(c)
+    vz = vi.copy()
+    vz[:, 1] = 0.0
+    desc, keep = ctx.run_desc(eck, vz, inc, c.rten(), cf)
+    with pytest.raises(_lib.DCError) as e:
+        ctx.four_run(desc)
+    assert e.value.ier_num == _lib.DC_ERR_ZEROVEC
+    # -162: lots on while the crystal is not cells x sites
+    short = X.Crystal(c.pos[:-1], c.iscat[:-1], c.icc, c.ncatoms, c.lattice, c.scat, c.occ, c.dw)
+    ctx.crystal_set(short)
+    desc, keep = ctx.run_desc(eck, vi, inc, c.rten(), cf, ilots=_lib.LOT["box"], ls=(2, 2, 2),
+                              lbeg=np.ones((2, 3), np.int32))
+    with pytest.raises(_lib.DCError) as e:
+        ctx.four_run(desc)
+    assert e.value.ier_num == _lib.DC_ERR_LOTS
+    # -198: no atoms
+    empty = X.Crystal(np.zeros((0, 3)), np.zeros(0, np.int32), (1, 1, 1), 1, c.lattice, c.scat, c.occ, c.dw)
+    ctx.crystal_set(empty)
+    desc, keep = ctx.run_desc(eck, vi, inc, c.rten(), cf)
+    with pytest.raises(_lib.DCError) as e:
+        ctx.four_run(desc)
+    assert e.value.ier_num == _lib.DC_ERR_NOATOMS
+    # -14: the cancel flag leaves the outputs untouched
+    ctx.crystal_set(c)
+    flag = np.ones(1, np.int32)
+    ctx.set_cancel_flag(flag)
+    try:
+        desc, keep = ctx.run_desc(eck, vi, inc, c.rten(), cf)
+        with pytest.raises(_lib.DCError) as e:
+            ctx.four_run(desc)
+        assert e.value.ier_num == _lib.DC_ERR_CANCEL
+    finally:
+        ctx.set_cancel_flag(None)
+    # and the context still works afterwards
+    gpu = ctx.four_run(desc)
+    ref = O.four_run(c, eck, vi, inc, cf)
+    assert float(np.abs(gpu["dsi"] - ref["dsi"]).max() / ref["dsi"].max()) <= 1e-6
