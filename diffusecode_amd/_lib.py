@@ -37,7 +37,14 @@ EXPORTS = [
     "dc_crystal_set", "dc_four_run", "dc_four_run_dev", "dc_last_stats", "dc_synchronize",
     "dc_model_intensity", "dc_model_vjp", "dc_model_chi2", "dc_model_jvp",
     "dc_batch_chi2", "dc_set_autotune", "dc_set_graphs", "dc_batch_info",
+    "dc_four_post", "dc_four_post_dev", "dc_four_fill", "dc_four_fill_dev", "dc_four_accumulate_dev",
+    "dc_qval", "dc_qval_dev",
 ]
+
+DC_ERR_NOSUM = -175
+ACCUM = {"init": -1, "single": 0, "accum": 1, "finished": 2}
+QVAL = {"inten": 1, "ampli": 2, "phase": 3, "real": 4, "imag": 5, "sq": 7, "f2aver": 9, "faver2": 10,
+        "faver": 11, "iq": 13, "f2averb": 17, "faver2b": 18, "faverb": 19}
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -71,6 +78,26 @@ class BatchDesc(C.Structure):
     _fields_ = [("nmembers", C.c_int), ("atom_beg", C.POINTER(C.c_int64)), ("pos", _dp), ("type", _ip),
                 ("occ", _dp), ("bdw", _dp), ("nscat", C.c_int), ("ncfact", C.c_int), ("cfact", _dp),
                 ("rten", _dp), ("eck", C.c_double * 12), ("vi", C.c_double * 9), ("inc", C.c_int * 3)]
+
+
+class PostDesc(C.Structure):
+    _fields_ = [("num", C.c_int * 3), ("eck", C.c_double * 12), ("vi", C.c_double * 9), ("rten", C.c_double * 9),
+                ("vr", C.c_double), ("fave", C.c_double), ("nlots", C.c_int), ("icc", C.c_int * 3),
+                ("ls", C.c_int * 3), ("lots_off", C.c_int), ("diff_scale", C.c_double),
+                ("res_sigma", C.c_double * 3), ("res_tr", C.c_double * 9), ("filter", C.c_int),
+                ("rscale", C.c_double), ("damp", C.c_double), ("width", C.c_int)]
+
+
+class SymmDesc(C.Structure):
+    _fields_ = [("num", C.c_int * 3), ("eck", C.c_double * 12), ("vi", C.c_double * 9), ("fave", C.c_double),
+                ("four_symm", C.c_int), ("nops", C.c_int), ("mats", _dp)]
+
+
+class QvalDesc(C.Structure):
+    _fields_ = [("num", C.c_int * 3), ("eck", C.c_double * 12), ("vi", C.c_double * 9), ("rten", C.c_double * 9),
+                ("value", C.c_int), ("laver", C.c_int), ("cfact", _dp), ("cfact_pure", _dp), ("ncfact", C.c_int),
+                ("nscat", C.c_int), ("amount", _dp), ("n_real_atoms", C.c_double), ("u2aver", C.c_double),
+                ("faver_sign", C.c_double)]
 
 
 _lib = None
@@ -113,6 +140,14 @@ def lib():
     L.dc_set_autotune.argtypes = [C.c_void_p, C.c_int]
     L.dc_set_graphs.argtypes = [C.c_void_p, C.c_int]
     L.dc_batch_info.argtypes = [C.c_void_p, _ip, _ip, _ip, _dp]
+    L.dc_four_post.argtypes = [C.c_void_p, C.POINTER(PostDesc), _dp, _dp, _dp]
+    L.dc_four_post_dev.argtypes = [C.c_void_p, C.POINTER(PostDesc), C.c_void_p, C.c_void_p, C.c_void_p]
+    L.dc_four_fill.argtypes = [C.c_void_p, C.POINTER(SymmDesc), _dp, _dp]
+    L.dc_four_fill_dev.argtypes = [C.c_void_p, C.POINTER(SymmDesc), C.c_void_p, C.c_void_p]
+    L.dc_four_accumulate_dev.argtypes = [C.c_void_p, C.c_int, C.POINTER(SymmDesc), C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_void_p]
+    L.dc_qval.argtypes = [C.c_void_p, C.POINTER(QvalDesc), _dp, _dp, _dp, _dp]
+    L.dc_qval_dev.argtypes = [C.c_void_p, C.POINTER(QvalDesc), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     _lib = L
     return L
 
@@ -445,3 +480,111 @@ class Context:
         self.check(lib().dc_batch_chi2(self.h, C.byref(desc), _d(t), None if w is None else _d(w), _d(chi),
                                        None if ic is None else _d(ic), None if g is None else _d(g)))
         return chi, ic, None if g is None else g.reshape(n, 5)
+
+    # -- post-processing (four_run tail, four_accumulate, qval) ---------------
+    @staticmethod
+    def post_desc(num, eck, vi, rten, vr, fave=0.0, nlots=1, icc=(1, 1, 1), ls=(1, 1, 1), lots_off=True,
+                  diff_scale=1.0, res_sigma=(0.0, 0.0, 0.0), res_tr=None, lanczos=False, rscale=1.0, damp=0.5,
+                  width=4):
+        d = PostDesc()
+        d.num[:] = [int(v) for v in num]
+        d.eck[:] = list(_f64(eck, 12))
+        d.vi[:] = list(_f64(vi, 9))
+        d.rten[:] = list(_f64(rten, 9))
+        d.vr = float(vr)
+        d.fave = float(fave)
+        d.nlots = int(nlots)
+        d.icc[:] = [int(v) for v in icc]
+        d.ls[:] = [int(v) for v in ls]
+        d.lots_off = int(bool(lots_off))
+        d.diff_scale = float(diff_scale)
+        d.res_sigma[:] = [float(v) for v in res_sigma]
+        d.res_tr[:] = list(_f64(np.eye(3) if res_tr is None else res_tr, 9))
+        d.filter = 1 if lanczos else 0
+        d.rscale, d.damp, d.width = float(rscale), float(damp), int(width)
+        return d
+
+    def four_post(self, desc, csf, dsi, want_dsi3d=True):
+        """Host arrays (shape num, Fortran order); returns new (csf, dsi, dsi3d)."""
+        num = tuple(desc.num)
+        c = None if csf is None else np.ascontiguousarray(np.asarray(csf, np.complex128).ravel(order="F")).view(np.float64).copy()
+        d = np.ascontiguousarray(np.asarray(dsi, np.float64).ravel(order="F")).copy()
+        d3 = np.zeros_like(d) if want_dsi3d else None
+        self.check(lib().dc_four_post(self.h, C.byref(desc), _d(c), _d(d), _d(d3)))
+        out_c = None if c is None else c.view(np.complex128).reshape(num, order="F")
+        return out_c, d.reshape(num, order="F"), None if d3 is None else d3.reshape(num, order="F")
+
+    def four_post_dev(self, desc, csf, dsi, dsi3d=None):
+        """Device variant: torch CUDA tensors (float64; complex as view_as_real) or addresses."""
+        def p(t):
+            return None if t is None else (t if isinstance(t, int) else t.data_ptr())
+        return self.check(lib().dc_four_post_dev(self.h, C.byref(desc), p(csf), p(dsi), p(dsi3d)))
+
+    @staticmethod
+    def symm_desc(num, eck, vi, fave=0.0, four_symm=False, ops=None):
+        """ops: list of 3x3 operations of the point-group part, identity first
+        (DISCUS's spc_mat(1:3,1:3,1:spc_n/n_center)); nops = len(ops)."""
+        d = SymmDesc()
+        keep = []
+        d.num[:] = [int(v) for v in num]
+        d.eck[:] = list(_f64(eck, 12))
+        d.vi[:] = list(_f64(vi, 9))
+        d.fave = float(fave)
+        d.four_symm = int(bool(four_symm))
+        if ops is not None and len(ops) > 0:
+            d.nops = len(ops)
+            m = np.ascontiguousarray(np.stack([np.asarray(o, float) for o in ops[1:]] or [np.eye(3)], axis=2).ravel(order="F"))
+            keep.append(m)
+            d.mats = _d(m)
+        else:
+            d.nops = 1
+        return d, keep
+
+    def four_fill(self, desc, csf, dsi):
+        num = tuple(desc.num)
+        c = None if csf is None else np.ascontiguousarray(np.asarray(csf, np.complex128).ravel(order="F")).view(np.float64).copy()
+        d = None if dsi is None else np.ascontiguousarray(np.asarray(dsi, np.float64).ravel(order="F")).copy()
+        self.check(lib().dc_four_fill(self.h, C.byref(desc), _d(c), _d(d)))
+        return (None if c is None else c.view(np.complex128).reshape(num, order="F"),
+                None if d is None else d.reshape(num, order="F"))
+
+    def four_accumulate_dev(self, mode, desc, csf, dsi, csf_sum=None, dsi_sum=None):
+        def p(t):
+            return None if t is None else (t if isinstance(t, int) else t.data_ptr())
+        m = ACCUM[mode] if isinstance(mode, str) else int(mode)
+        return self.check(lib().dc_four_accumulate_dev(self.h, m, C.byref(desc), p(csf), p(dsi), p(csf_sum), p(dsi_sum)))
+
+    @staticmethod
+    def qval_desc(num, eck, vi, rten, value, laver=False, cfact=None, cfact_pure=None, amount=None,
+                  n_real_atoms=1.0, u2aver=0.0, faver_sign=1.0):
+        d = QvalDesc()
+        keep = []
+        d.num[:] = [int(v) for v in num]
+        d.eck[:] = list(_f64(eck, 12))
+        d.vi[:] = list(_f64(vi, 9))
+        d.rten[:] = list(_f64(rten, 9))
+        d.value = QVAL[value] if isinstance(value, str) else int(value)
+        d.laver = int(bool(laver))
+        if cfact is not None:
+            cf = np.asarray(cfact, np.complex128)
+            cp = np.asarray(cfact if cfact_pure is None else cfact_pure, np.complex128)
+            a = np.ascontiguousarray(cf.ravel(order="F")).view(np.float64)
+            b = np.ascontiguousarray(cp.ravel(order="F")).view(np.float64)
+            am = np.ascontiguousarray(np.asarray(amount, np.float64).ravel())
+            keep += [a, b, am]
+            d.cfact, d.cfact_pure, d.amount = _d(a), _d(b), _d(am)
+            d.ncfact, d.nscat = cf.shape[0], cf.shape[1]
+        d.n_real_atoms = float(n_real_atoms)
+        d.u2aver = float(u2aver)
+        d.faver_sign = float(faver_sign)
+        return d, keep
+
+    def qval(self, desc, csf=None, acsf=None, dsi=None):
+        num = tuple(desc.num)
+        def cz(x):
+            return None if x is None else np.ascontiguousarray(np.asarray(x, np.complex128).ravel(order="F")).view(np.float64)
+        c, a = cz(csf), cz(acsf)
+        d = None if dsi is None else np.ascontiguousarray(np.asarray(dsi, np.float64).ravel(order="F"))
+        out = np.zeros(int(np.prod(num)))
+        self.check(lib().dc_qval(self.h, C.byref(desc), _d(c), _d(a), _d(d), _d(out)))
+        return out.reshape(num, order="F")

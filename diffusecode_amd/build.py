@@ -13,7 +13,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdiffuse_turbo.so")
-SOURCES = ["dc_gemm.hip", "dc_lot.hip", "dc_api.hip", "dc_adjoint.hip", "dc_refine.hip", "dc_batch.hip"]
+SOURCES = ["dc_gemm.hip", "dc_lot.hip", "dc_api.hip", "dc_adjoint.hip", "dc_refine.hip", "dc_batch.hip", "dc_post.hip"]
+ROCM_LIB = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
 ARCH = os.environ.get("DC_OFFLOAD_ARCH", "gfx950")
 HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
             # packed FP32 VALU ops stay off (wrong results seen on this hardware pool)
@@ -54,7 +55,8 @@ def build(jobs: int = 4, verbose: bool = False) -> str:
             sys.stderr.write(out + err)
             raise RuntimeError("build failed: " + " ".join(cmd))
     tmp = LIB + ".tmp"
-    _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs])
+    _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs,
+          "-L", ROCM_LIB, "-lhipfft", "-Wl,-rpath," + ROCM_LIB])
     os.replace(tmp, LIB)
     return LIB
 

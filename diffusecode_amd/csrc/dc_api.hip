@@ -265,6 +265,7 @@ void dc_destroy(dc_ctx *c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->st);
     batch_release(c);
+    post_release(c);
     for (DevBuf *b : c->buffers())
         if (b->p) (void)hipFree(b->p);
     for (auto &e : c->ev)

@@ -9,6 +9,7 @@
 #include "dc_internal.h"
 
 struct BatchState;   // dc_batch.hip
+struct PostState;    // dc_post.hip
 
 struct DevBuf {
     void *p = nullptr;
@@ -23,6 +24,7 @@ struct dc_ctx {
     int autotune = 1;               // batch: time tile shapes on the first call of a shape
     int graphs = 1;                 // batch: capture the population step as a hipGraph
     BatchState *batch = nullptr;    // cached plan / buffers / graph of the last batch shape
+    PostState *post = nullptr;      // FFT plans and buffers of the post-processing calls
     std::string err;
     int err_typ = DC_ER_NONE;
     const volatile int *cancel = nullptr;
@@ -132,6 +134,7 @@ int kchunk_for(const Plan &pl, long long natoms_batch, int prec);
 void add_segments(SliceSet &s, const std::vector<std::pair<int, int>> &segs, int kchunk, int &atom_off);
 int run_gemm(dc_ctx *c, const Plan &pl, int nslice, float2 *partial, const float *sq = nullptr);
 void batch_release(dc_ctx *c);   // frees the cached batch plan, buffers and graph
+void post_release(dc_ctx *c);    // frees the FFT plans and buffers of the post-processing calls
 PrepArgs base_prep(const Grid &g, int mapping);
 
 }  // namespace dc

@@ -278,6 +278,100 @@ int dc_set_graphs(dc_ctx *ctx, int on);
  * (0 = one per work item), whether a graph replays it, autotuned GEMM ms. */
 int dc_batch_info(const dc_ctx *ctx, int *tile, int *nwg, int *graph, double *tune_ms);
 
+/* ------------------------------------------------------------------------
+ * Post-processing of a finished run: the tail of four_run
+ * (fourier_sup.f90:279-292) and four_accumulate / qval.
+ *
+ * Arrays are DISCUS's: num(1)*num(2)*num(3) elements, abscissa fastest,
+ * complex as interleaved (re, im) doubles.  The *_dev entry points take
+ * device pointers (e.g. the outputs of dc_four_run_dev) and do not
+ * synchronise the host; the others take host pointers and return when the
+ * outputs are written.  Every step runs on the GPU (hipFFT Z2Z transforms and
+ * the library's own kernels).
+ * ------------------------------------------------------------------------ */
+typedef struct dc_post_desc {
+    int num[3];                 /* grid points along abs, ord, top            */
+    double eck[12];             /* corners, as dc_run_desc                    */
+    double vi[9];               /* increments abs, ord, top = uin, vin, win   */
+    double rten[9];             /* reciprocal metric cr_rten                  */
+    double vr;                  /* reciprocal cell volume cr_vr               */
+    double fave;                /* 'set aver'                                 */
+    int nlots;
+    int icc[3];                 /* crystal size in unit cells cr_icc          */
+    int ls[3];                  /* lot size ls_xyz                            */
+    int lots_off;               /* ilots == LOT_OFF: four_conv convolves csf  */
+    double diff_scale;          /* already applied to dsi and csf             */
+    double res_sigma[3];        /* diff_res(1, 1:3); all < 1e-6: no four_conv */
+    double res_tr[9];           /* diff_tr(3,3), column-major                 */
+    int filter;                 /* 0 off, 1 Lanczos (four_filter)             */
+    double rscale;              /* four_rscale (nint must be 1)               */
+    double damp;                /* four_damp                                  */
+    int width;                  /* four_width                                 */
+} dc_post_desc;
+
+/* four_weight (fourier_sup.f90:1428-1644) into dsi3d, do_four_filter on dsi
+ * (:1646-1805, lib_lanczos.f90:258-458), four_conv on csf/dsi/dsi3d
+ * (fourier_conv.f90:7-423), do_four_filter on dsi3d when fave == 0: the
+ * sequence of fourier_sup.f90:279-290.  dsi3d may be NULL. */
+int dc_four_post(dc_ctx *ctx, const dc_post_desc *post, double *csf, double *dsi, double *dsi3d);
+int dc_four_post_dev(dc_ctx *ctx, const dc_post_desc *post, double *d_csf, double *d_dsi, double *d_dsi3d);
+
+typedef struct dc_symm_desc {
+    int num[3];
+    double eck[12];
+    double vi[9];
+    double fave;                /* < 0: fill Bragg pixels from neighbours      */
+    int four_symm;              /* 'set symm' point-group average              */
+    int nops;                   /* spc_n / n_center                            */
+    const double *mats;         /* (3,3,nops-1): spc_mat(1:3,1:3,2:nops), host */
+} dc_symm_desc;
+
+/* four_fill_csf + four_fill_dsi (fourier_sup.f90:3844-4046): point-group
+ * average (four_symm_csf/_dsi :4048-4293) when four_symm, then the Bragg
+ * fill when fave < 0; in place.  Operations must be integer matrices with
+ * determinant +-1 (DC_ERR_ARG otherwise). */
+int dc_four_fill(dc_ctx *ctx, const dc_symm_desc *symm, double *csf, double *dsi);
+int dc_four_fill_dev(dc_ctx *ctx, const dc_symm_desc *symm, double *d_csf, double *d_dsi);
+
+#define DC_ACCUM_INIT (-1)
+#define DC_ACCUM_SINGLE 0
+#define DC_ACCUM_ACCUM 1
+#define DC_ACCUM_FINISHED 2
+#define DC_ERR_NOSUM (-175)     /* four_accumulate finished without sums (fourier_sup.f90:3827-3830) */
+
+/* four_accumulate (fourier_sup.f90:3784-3842) with caller-owned device sums
+ * csf_sum/dsi_sum (NULL where DISCUS's are not allocated).  INIT does
+ * nothing here (the caller drops its sums); ACCUM adds; FINISHED fills the
+ * sums (dc_four_fill) and copies them into csf/dsi; SINGLE applies the fill
+ * to csf/dsi when four_symm. */
+int dc_four_accumulate_dev(dc_ctx *ctx, int mode, const dc_symm_desc *symm, double *d_csf, double *d_dsi,
+                           double *d_csf_sum, double *d_dsi_sum);
+
+/* qval (qval.f90:65-286) over the whole grid.  value: 1 intensity,
+ * 2 amplitude, 3 phase (degrees), 4 real, 5 imaginary, 7 S(Q), 9 <f^2>,
+ * 10 <f>^2, 11 <f>, 13 I(Q), 17 <f^2> with DBW, 18 <f>^2 with DBW, 19 <f>
+ * with DBW.  6 (random phase), 8, 14-16 (PDF) return DC_ERR_ARG. */
+typedef struct dc_qval_desc {
+    int num[3];
+    double eck[12];
+    double vi[9];
+    double rten[9];             /* for istl (four_stltab)                     */
+    int value;
+    int laver;                  /* take acsf instead of csf                   */
+    const double *cfact;        /* (ncfact, nscat) complex, host              */
+    const double *cfact_pure;   /* (ncfact, nscat) complex, host              */
+    int ncfact, nscat;
+    const double *amount;       /* cr_amount(1:nscat), host                   */
+    double n_real_atoms;        /* cr_n_real_atoms                            */
+    double u2aver;              /* cr_u2aver                                  */
+    double faver_sign;          /* sign <f> takes (DISCUS: left by the last
+                                   S(Q) / <f>^2 evaluation; +1 in a fresh run) */
+} dc_qval_desc;
+int dc_qval(dc_ctx *ctx, const dc_qval_desc *q, const double *csf, const double *acsf, const double *dsi,
+            double *out);
+int dc_qval_dev(dc_ctx *ctx, const dc_qval_desc *q, const double *d_csf, const double *d_acsf,
+                const double *d_dsi, double *d_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -51,6 +51,11 @@ def test_null_context_arguments():
     assert L.dc_strucf(None, None, 0, 0, None, None, None, None, None, None, None, None, 0, None, None, None, 0,
                        None) == _lib.DC_ERR_ARG
     assert L.dc_create(None, 0) == _lib.DC_ERR_ARG
+    assert L.dc_four_post(None, None, None, None, None) == _lib.DC_ERR_ARG
+    assert L.dc_four_post_dev(None, None, None, None, None) == _lib.DC_ERR_ARG
+    assert L.dc_four_fill(None, None, None, None) == _lib.DC_ERR_ARG
+    assert L.dc_four_accumulate_dev(None, 0, None, None, None, None, None) == _lib.DC_ERR_ARG
+    assert L.dc_qval(None, None, None, None, None, None) == _lib.DC_ERR_ARG
 
 
 def _gpu_present():
