@@ -152,6 +152,41 @@ def bench_adjoint(_lib, X, FR):
     return out
 
 
+def bench_post(_lib, FR, torch, steps=5):
+    """Post-processing tail of four_run on the headline grid (161^3): four_weight
+    into dsi3d, Lanczos filter of dsi, four_conv of csf/dsi/dsi3d, Lanczos
+    filter of dsi3d (fourier_sup.f90:279-290), all on device buffers."""
+    inc = (161, 161, 161)
+    eck, vi, _ = FR.grid_block((-4, -4, -4), (4, 4, 4), inc)
+    n = int(np.prod(inc))
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device="cpu").manual_seed(3)
+    base_dsi = torch.rand(n, dtype=torch.float64, generator=g).to(dev)
+    base_csf = torch.randn(2 * n, dtype=torch.float64, generator=g).to(dev)
+    d_dsi, d_csf = base_dsi.clone(), base_csf.clone()
+    d_d3 = torch.empty(n, dtype=torch.float64, device=dev)
+    rten = np.eye(3) / 16.0
+    ctx = _lib.Context(device=0)
+    desc = ctx.post_desc(inc, eck, vi, rten, 1.0 / 64.0, fave=0.0, nlots=1, icc=(40, 40, 40), lots_off=True,
+                         res_sigma=(1.5, 1.5, 1.5), res_tr=np.eye(3) * 0.5, lanczos=True, damp=0.5, width=4)
+    torch.cuda.synchronize()
+    ctx.four_post_dev(desc, d_csf, d_dsi, d_d3)          # plans, buffers
+    ctx.synchronize()
+    t = 0.0
+    for _ in range(steps):
+        d_dsi.copy_(base_dsi)
+        d_csf.copy_(base_csf)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.four_post_dev(desc, d_csf, d_dsi, d_d3)
+        ctx.synchronize()
+        t += time.perf_counter() - t0
+    ctx.close()
+    ms = t / steps * 1e3
+    return {"ms": round(ms, 3), "grid": list(inc), "pixels_per_s": n / (ms * 1e-3),
+            "steps": "four_weight + Lanczos(dsi) + four_conv(csf, dsi, dsi3d) + Lanczos(dsi3d), FP64, hipFFT Z2Z"}
+
+
 def bench_batch(_lib, X, FR, prec):
     from diffusecode_amd.population import Population
     crs = X.population()
@@ -183,6 +218,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--extras", default="lots,adjoint,batch,post",
+                    help="comma list of the extra measurements to run")
     ap.add_argument("--cpu-layers", type=int, default=40)
     ap.add_argument("--calibrate-copy", type=int, default=0)
     args = ap.parse_args()
@@ -269,13 +306,19 @@ def main():
     ctx.close()
 
     extras = {}
-    if not args.no_extras:
+    want = set() if args.no_extras else {e.strip() for e in args.extras.split(",") if e.strip()}
+    if "lots" in want:
         extras["lots"] = bench_lots(_lib, X, FR, torch, args.prec)
         log(f"lots: {extras['lots']['ms']:.2f} ms, {extras['lots']['value'] / 1e12:.2f} Tpairs/s")
+    if "adjoint" in want:
         extras["adjoint"] = bench_adjoint(_lib, X, FR)
         log(f"adjoint: {json.dumps(extras['adjoint'])}")
+    if "batch" in want:
         extras["batch"] = bench_batch(_lib, X, FR, args.prec)
         log(f"batch: {json.dumps(extras['batch'])}")
+    if "post" in want:
+        extras["post"] = bench_post(_lib, FR, torch)
+        log(f"post: {json.dumps(extras['post'])}")
 
     cpu = None
     if not args.no_cpu:

@@ -14,7 +14,7 @@
 // with lf = (m-1) mod 2 everywhere, and lr = lf (3D, 2D Lanczos), 0 (1D), or
 // the other axis's parity (four_conv_2D when num(2) > num(1), where
 // fftfdtoline_22D overwrites l in natural order and reads it in FFT order,
-// map_1dtofield.f90:577-582).  The transforms run on natural axes: a DFT
+// map_1dtofield.f90:575-582).  The transforms run on natural axes: a DFT
 // over all axes does not depend on their order.
 //
 // Layout: a field is `batch` sub-fields of dims m[0..2] (m[0] fastest) at
@@ -981,7 +981,7 @@ int dc_four_fill_dev(dc_ctx *c, const dc_symm_desc *d, double *csf, double *dsi)
     if (!d) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "null symmetry descriptor");
     int st = check_grid(c, d->num);
     if (st) return st;
-    if (d->fave == 0.0 && !d->four_symm) return DC_OK;   // fourier_sup.f90:3869
+    if (d->fave == 0.0 && !d->four_symm) return DC_OK;   // fourier_sup.f90:3862, 3964
     HIPCHK(hipSetDevice(c->device));
     PostState &ps = post_state(c);
     SymmArgs sa{};

@@ -52,6 +52,54 @@ module diffuse_turbo_mod
       real(c_double)     :: dim_lo(3)
    end type dc_crystal_desc
 
+   ! dc_post_desc / dc_symm_desc / dc_qval_desc of include/diffuse_turbo.h
+   type, bind(C), public :: dc_post_desc
+      integer(c_int)     :: num(3)
+      real(c_double)     :: eck(12)
+      real(c_double)     :: vi(9)
+      real(c_double)     :: rten(9)
+      real(c_double)     :: vr
+      real(c_double)     :: fave
+      integer(c_int)     :: nlots
+      integer(c_int)     :: icc(3)
+      integer(c_int)     :: ls(3)
+      integer(c_int)     :: lots_off
+      real(c_double)     :: diff_scale
+      real(c_double)     :: res_sigma(3)
+      real(c_double)     :: res_tr(9)
+      integer(c_int)     :: filter
+      real(c_double)     :: rscale
+      real(c_double)     :: damp
+      integer(c_int)     :: width
+   end type dc_post_desc
+
+   type, bind(C), public :: dc_symm_desc
+      integer(c_int)     :: num(3)
+      real(c_double)     :: eck(12)
+      real(c_double)     :: vi(9)
+      real(c_double)     :: fave
+      integer(c_int)     :: four_symm
+      integer(c_int)     :: nops
+      type(c_ptr)        :: mats
+   end type dc_symm_desc
+
+   type, bind(C), public :: dc_qval_desc
+      integer(c_int)     :: num(3)
+      real(c_double)     :: eck(12)
+      real(c_double)     :: vi(9)
+      real(c_double)     :: rten(9)
+      integer(c_int)     :: value
+      integer(c_int)     :: laver
+      type(c_ptr)        :: cfact
+      type(c_ptr)        :: cfact_pure
+      integer(c_int)     :: ncfact
+      integer(c_int)     :: nscat
+      type(c_ptr)        :: amount
+      real(c_double)     :: n_real_atoms
+      real(c_double)     :: u2aver
+      real(c_double)     :: faver_sign
+   end type dc_qval_desc
+
    interface
       integer(c_int) function dc_create(ctx, device) bind(C, name='dc_create')
          import :: c_ptr, c_int
@@ -103,10 +151,31 @@ module diffuse_turbo_mod
          type(dc_run_desc), intent(in) :: run
          type(c_ptr), value :: csf, acsf, dsi, fmean, dvar
       end function dc_four_run
+
+      integer(c_int) function dc_four_post(ctx, post, csf, dsi, dsi3d) bind(C, name='dc_four_post')
+         import :: c_ptr, c_int, dc_post_desc
+         type(c_ptr), value :: ctx
+         type(dc_post_desc), intent(in) :: post
+         type(c_ptr), value :: csf, dsi, dsi3d
+      end function dc_four_post
+
+      integer(c_int) function dc_four_fill(ctx, symm, csf, dsi) bind(C, name='dc_four_fill')
+         import :: c_ptr, c_int, dc_symm_desc
+         type(c_ptr), value :: ctx
+         type(dc_symm_desc), intent(in) :: symm
+         type(c_ptr), value :: csf, dsi
+      end function dc_four_fill
+
+      integer(c_int) function dc_qval(ctx, q, csf, acsf, dsi, out) bind(C, name='dc_qval')
+         import :: c_ptr, c_int, dc_qval_desc
+         type(c_ptr), value :: ctx
+         type(dc_qval_desc), intent(in) :: q
+         type(c_ptr), value :: csf, acsf, dsi, out
+      end function dc_qval
    end interface
 
    public :: dc_create, dc_destroy, dc_set_precision, dc_last_error_type, dc_strucf
-   public :: dc_crystal_set, dc_four_run
+   public :: dc_crystal_set, dc_four_run, dc_four_post, dc_four_fill, dc_qval
    public :: dc_four_strucf, dc_shutdown
 
    type(c_ptr), save :: the_ctx = c_null_ptr

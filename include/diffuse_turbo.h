@@ -337,9 +337,9 @@ int dc_four_fill_dev(dc_ctx *ctx, const dc_symm_desc *symm, double *d_csf, doubl
 #define DC_ACCUM_SINGLE 0
 #define DC_ACCUM_ACCUM 1
 #define DC_ACCUM_FINISHED 2
-#define DC_ERR_NOSUM (-175)     /* four_accumulate finished without sums (fourier_sup.f90:3827-3830) */
+#define DC_ERR_NOSUM (-175)     /* four_accumulate finished without sums (fourier_sup.f90:3826-3830) */
 
-/* four_accumulate (fourier_sup.f90:3784-3842) with caller-owned device sums
+/* four_accumulate (fourier_sup.f90:3784-3840) with caller-owned device sums
  * csf_sum/dsi_sum (NULL where DISCUS's are not allocated).  INIT does
  * nothing here (the caller drops its sums); ACCUM adds; FINISHED fills the
  * sums (dc_four_fill) and copies them into csf/dsi; SINGLE applies the fill
