@@ -152,6 +152,60 @@ def bench_adjoint(_lib, X, FR):
     return out
 
 
+def bench_nufft(_lib, torch, c, steps=3):
+    """The NUFFT technique (four_run_nufft's per-type transforms,
+    fourier_finufft.f90:156-245) on the headline workload: fcc 40^3 cells,
+    positions 2 pi * 2 * pos / 40 (four_getatm_anis_iam with scales = 2 for a
+    0.05 step), 161^3 modes per type, tol 1e-7 as DISCUS sets it.  Accuracy
+    of type 1 against the exact FP64 sum on two subsampled l-layers."""
+    from oracle import oracle as O
+    inc = [161, 161, 161]
+    ls = [1, 1, 1]
+    n = int(np.prod(inc))
+    dev = torch.device("cuda:0")
+    types = [t for t in range(1, c.nscat + 1) if np.any(c.iscat == t)]
+    d_pos = {}
+    for t in types:
+        p = c.pos[c.iscat == t]
+        d_pos[t] = [torch.from_numpy(np.ascontiguousarray(2 * np.pi * 2.0 * p[:, d] / 40.0)).to(dev) for d in range(3)]
+    d_out = torch.empty(2 * n, dtype=torch.float64, device=dev)
+    ii = np.array(inc, np.int32)
+    ll = np.array(ls, np.int32)
+    L = _lib.lib()
+    ctx = _lib.Context(device=0)
+    torch.cuda.synchronize()
+
+    def run_type(t):
+        x, y, z = d_pos[t]
+        ctx.check(L.dc_four_strucf_nufft_dev(ctx.h, 3, x.numel(), x.data_ptr(), y.data_ptr(), z.data_ptr(), 1.0,
+                                             _lib._i(ii), _lib._i(ll), d_out.data_ptr()))
+    for t in types:
+        run_type(t)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for t in types:
+            run_type(t)
+    ctx.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    run_type(types[0])
+    ctx.synchronize()
+    f = d_out.cpu().numpy().view(np.complex128).reshape(inc, order="F")
+    ctx.close()
+    p = c.pos[c.iscat == types[0]]
+    errs = []
+    for lval, kidx in ((0.0, 80), (1.0, 100)):
+        ref = O.strucf_exact(p, [-4.0, -4.0, lval], [0.2, 0, 0], [0, 0.2, 0], [0, 0, 0.05], [41, 41, 1])[:, :, 0]
+        got = f[::4, ::4, kidx]
+        errs.append((np.abs(got - ref).max(), np.abs(ref).max()))
+    err = float(max(e for e, _ in errs) / max(m for _, m in errs))
+    natoms = int(sum(len(d_pos[t][0]) for t in types))
+    return {"ms": round(ms, 3), "value": natoms * n / (ms * 1e-3), "unit": "atom·hkl pairs/s (full grid)",
+            "tol": 1e-7, "max_rel_err_sampled_F": err, "types": len(types),
+            "note": "per-type NUFFT transforms only (spread, 324^3 FFT, deconvolution); form factors and |F|^2 "
+                    "are the shared post-processing"}
+
+
 def bench_post(_lib, FR, torch, steps=5):
     """Post-processing tail of four_run on the headline grid (161^3): four_weight
     into dsi3d, Lanczos filter of dsi, four_conv of csf/dsi/dsi3d, Lanczos
@@ -218,7 +272,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--extras", default="lots,adjoint,batch,post",
+    ap.add_argument("--extras", default="lots,adjoint,batch,post,nufft",
                     help="comma list of the extra measurements to run")
     ap.add_argument("--cpu-layers", type=int, default=40)
     ap.add_argument("--calibrate-copy", type=int, default=0)
@@ -306,6 +360,10 @@ def main():
     ctx.close()
 
     extras = {}
+    want_nufft = not args.no_extras and "nufft" in {e.strip() for e in args.extras.split(",")}
+    if want_nufft:
+        extras["nufft"] = bench_nufft(_lib, torch, c)
+        log(f"nufft: {json.dumps(extras['nufft'])}")
     want = set() if args.no_extras else {e.strip() for e in args.extras.split(",") if e.strip()}
     if "lots" in want:
         extras["lots"] = bench_lots(_lib, X, FR, torch, args.prec)

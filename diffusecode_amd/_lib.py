@@ -38,7 +38,7 @@ EXPORTS = [
     "dc_model_intensity", "dc_model_vjp", "dc_model_chi2", "dc_model_jvp",
     "dc_batch_chi2", "dc_set_autotune", "dc_set_graphs", "dc_batch_info",
     "dc_four_post", "dc_four_post_dev", "dc_four_fill", "dc_four_fill_dev", "dc_four_accumulate_dev",
-    "dc_qval", "dc_qval_dev",
+    "dc_qval", "dc_qval_dev", "dc_nufft1", "dc_nufft1_dev", "dc_four_strucf_nufft", "dc_four_strucf_nufft_dev",
 ]
 
 DC_ERR_NOSUM = -175
@@ -148,6 +148,13 @@ def lib():
                                          C.c_void_p, C.c_void_p]
     L.dc_qval.argtypes = [C.c_void_p, C.POINTER(QvalDesc), _dp, _dp, _dp, _dp]
     L.dc_qval_dev.argtypes = [C.c_void_p, C.POINTER(QvalDesc), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.dc_nufft1.argtypes = [C.c_void_p, C.c_int, C.c_int64, _dp, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int64,
+                            C.c_int64, C.c_int64, _dp]
+    L.dc_nufft1_dev.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_int, C.c_double, C.c_int64, C.c_int64, C.c_int64, C.c_void_p]
+    L.dc_four_strucf_nufft.argtypes = [C.c_void_p, C.c_int, C.c_int64, _dp, _dp, _dp, C.c_double, _ip, _ip, _dp]
+    L.dc_four_strucf_nufft_dev.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.c_double, _ip, _ip, C.c_void_p]
     _lib = L
     return L
 
@@ -588,3 +595,28 @@ class Context:
         out = np.zeros(int(np.prod(num)))
         self.check(lib().dc_qval(self.h, C.byref(desc), _d(c), _d(a), _d(d), _d(out)))
         return out.reshape(num, order="F")
+
+    # -- NUFFT technique ----------------------------------------------------------
+    def nufft1(self, pts, c, iflag, tol, N):
+        """Type-1 NUFFT (finufft{1,2,3}d1): pts list of dim coordinate arrays,
+        c complex (M,), N mode counts; returns fk of shape N (k1 fastest)."""
+        dim = len(pts)
+        xs = [np.ascontiguousarray(np.asarray(p, np.float64).ravel()) for p in pts]
+        cc = np.ascontiguousarray(np.asarray(c, np.complex128).ravel()).view(np.float64)
+        NN = [int(n) for n in N][:dim] + [1] * (3 - dim)
+        fk = np.zeros(2 * int(np.prod(NN)))
+        xyz = xs + [None] * (3 - dim)
+        self.check(lib().dc_nufft1(self.h, dim, len(xs[0]), _d(xyz[0]), _d(xyz[1]), _d(xyz[2]), _d(cc), int(iflag),
+                                   float(tol), NN[0], NN[1], NN[2], _d(fk)))
+        return fk.view(np.complex128).reshape(tuple(NN[:dim]), order="F")
+
+    def four_strucf_nufft(self, pts, occ, inc, lscales):
+        dim = len(pts)
+        xs = [np.ascontiguousarray(np.asarray(p, np.float64).ravel()) for p in pts]
+        xyz = xs + [None] * (3 - dim)
+        ii = np.ascontiguousarray(np.array(list(inc)[:dim] + [1] * (3 - dim), np.int32))
+        ls = np.ascontiguousarray(np.array(list(lscales)[:dim] + [1] * (3 - dim), np.int32))
+        out = np.zeros(2 * int(np.prod(ii)))
+        self.check(lib().dc_four_strucf_nufft(self.h, dim, len(xs[0]), _d(xyz[0]), _d(xyz[1]), _d(xyz[2]), float(occ),
+                                              _i(ii), _i(ls), _d(out)))
+        return out.view(np.complex128).reshape(tuple(int(v) for v in ii[:dim]), order="F")

@@ -266,6 +266,7 @@ void dc_destroy(dc_ctx *c) {
     (void)hipStreamSynchronize(c->st);
     batch_release(c);
     post_release(c);
+    nufft_release(c);
     for (DevBuf *b : c->buffers())
         if (b->p) (void)hipFree(b->p);
     for (auto &e : c->ev)

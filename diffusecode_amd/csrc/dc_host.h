@@ -10,6 +10,7 @@
 
 struct BatchState;   // dc_batch.hip
 struct PostState;    // dc_post.hip
+struct NufftState;   // dc_nufft.hip
 
 struct DevBuf {
     void *p = nullptr;
@@ -25,6 +26,7 @@ struct dc_ctx {
     int graphs = 1;                 // batch: capture the population step as a hipGraph
     BatchState *batch = nullptr;    // cached plan / buffers / graph of the last batch shape
     PostState *post = nullptr;      // FFT plans and buffers of the post-processing calls
+    NufftState *nufft = nullptr;    // fine grid, plans and buffers of the NUFFT technique
     std::string err;
     int err_typ = DC_ER_NONE;
     const volatile int *cancel = nullptr;
@@ -135,6 +137,7 @@ void add_segments(SliceSet &s, const std::vector<std::pair<int, int>> &segs, int
 int run_gemm(dc_ctx *c, const Plan &pl, int nslice, float2 *partial, const float *sq = nullptr);
 void batch_release(dc_ctx *c);   // frees the cached batch plan, buffers and graph
 void post_release(dc_ctx *c);    // frees the FFT plans and buffers of the post-processing calls
+void nufft_release(dc_ctx *c);   // frees the NUFFT fine grid, plans and buffers
 PrepArgs base_prep(const Grid &g, int mapping);
 
 }  // namespace dc

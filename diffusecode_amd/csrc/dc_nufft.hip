@@ -1,0 +1,459 @@
+// dc_nufft.hip -- type-1 non-uniform FFT on the GPU: DISCUS's second Fourier
+// technique (four_run_nufft, fourier_sup.f90:327-690), which calls FINUFFT's
+// finufft1d1/2d1/3d1 through four_strucf_1d/2d/3d (fourier_finufft.f90:19-245):
+//
+//   fk(k) = sum_j c_j exp(iflag * i * k . x_j),   k_d = -N_d/2 .. (N_d-1)/2
+//
+// Method: the exponential-of-semicircle kernel of Barnett, Magland and
+// af Klinteberg (SIAM J. Sci. Comput. 41 (2019) C479-C504), written here:
+//   phi(z) = exp(beta (sqrt(1 - z^2) - 1)), |z| <= 1, width w grid points,
+//   w = ceil(log10(1/tol)) + 1 rounded up to even (2..16), beta = 2.30 w,
+//   oversampling 2;
+// 1. spread: every point adds c_j phi(2 (l - u_j)/w) to the w^d fine-grid
+//    points l around u_j = x_j n / (2 pi) (periodic), FP64 vector atomics;
+// 2. transform the fine grid (hipFFT Z2Z, backward for iflag > 0);
+// 3. deconvolve: fk(k) = B(k mod n) / prod_d phihat_d(k_d), phihat from a
+//    Gauss-Legendre rule (Newton on the three-term recurrence) on the host.
+// Everything is FP64; the atomics make the last bits order-dependent.
+#include <hipfft/hipfft.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "dc_host.h"
+
+using namespace dc;
+
+namespace {
+
+constexpr int MAXW = 16;
+
+struct NuParams {
+    int dim;
+    int w;
+    double beta;
+    long long n[3];      // fine grid
+    long long N[3];      // modes
+};
+
+struct SpreadArgs {
+    int dim, w;
+    double beta, half_w, two_over_w;
+    long long n[3];
+    double scale[3];     // n / (2 pi)
+};
+
+__device__ inline long long wrapi(long long l, long long n) {
+    long long r = l % n;
+    return r < 0 ? r + n : r;
+}
+
+// One thread per point: kernel values per axis (W compile-time, in
+// registers), then the W^DIM products added with FP64 vector atomics.
+template <int DIM, int W>
+__global__ __launch_bounds__(256) void spread_kernel(SpreadArgs a, long long M, const double *__restrict__ x,
+                                                     const double *__restrict__ y, const double *__restrict__ z,
+                                                     const double2 *__restrict__ c, double *__restrict__ grid) {
+    for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < M; j += (long long)gridDim.x * blockDim.x) {
+        // axes 1 and 2 keep their W kernel values in registers; the third
+        // axis evaluates its value inside its (not unrolled) loop
+        constexpr int KD = DIM < 3 ? DIM : 2;
+        double ker[KD][W];
+        long long l0[DIM];
+        double uu[DIM], lov[DIM];
+        const double *xs[3] = {x, y, z};
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            double u = xs[d][j] * a.scale[d];
+            u = u - floor(u / (double)a.n[d]) * (double)a.n[d];     // into [0, n)
+            const double lo = ceil(u - a.half_w);
+            l0[d] = (long long)lo;
+            uu[d] = u;
+            lov[d] = lo;
+            if (d < KD) {
+#pragma unroll
+                for (int i = 0; i < W; ++i) {
+                    const double zz = (lo + (double)i - u) * a.two_over_w;
+                    const double s = 1.0 - zz * zz;
+                    ker[d < KD ? d : 0][i] = s > 0.0 ? exp(a.beta * (sqrt(s) - 1.0)) : 0.0;
+                }
+            }
+        }
+        const double2 cj = c[j];
+        if constexpr (DIM == 1) {
+#pragma unroll
+            for (int i = 0; i < W; ++i) {
+                const long long g = wrapi(l0[0] + i, a.n[0]);
+                unsafeAtomicAdd(grid + 2 * g, cj.x * ker[0][i]);
+                unsafeAtomicAdd(grid + 2 * g + 1, cj.y * ker[0][i]);
+            }
+        } else if constexpr (DIM == 2) {
+            long long g1[W];
+#pragma unroll
+            for (int i = 0; i < W; ++i) g1[i] = wrapi(l0[0] + i, a.n[0]);
+            for (int i2 = 0; i2 < W; ++i2) {
+                const long long g2 = wrapi(l0[1] + i2, a.n[1]) * a.n[0];
+                const double k2r = cj.x * ker[1][i2], k2i = cj.y * ker[1][i2];
+#pragma unroll
+                for (int i1 = 0; i1 < W; ++i1) {
+                    const long long g = g2 + g1[i1];
+                    unsafeAtomicAdd(grid + 2 * g, k2r * ker[0][i1]);
+                    unsafeAtomicAdd(grid + 2 * g + 1, k2i * ker[0][i1]);
+                }
+            }
+        } else {
+            long long g1[W];
+#pragma unroll
+            for (int i = 0; i < W; ++i) g1[i] = wrapi(l0[0] + i, a.n[0]);
+            for (int i3 = 0; i3 < W; ++i3) {
+                const long long g3 = wrapi(l0[DIM - 1] + i3, a.n[2]) * a.n[1];
+                const double z3 = (lov[DIM - 1] + (double)i3 - uu[DIM - 1]) * a.two_over_w;
+                const double s3 = 1.0 - z3 * z3;
+                const double k3 = s3 > 0.0 ? exp(a.beta * (sqrt(s3) - 1.0)) : 0.0;
+#pragma unroll
+                for (int i2 = 0; i2 < W; ++i2) {
+                    const long long g2 = (g3 + wrapi(l0[1] + i2, a.n[1])) * a.n[0];
+                    const double k23 = k3 * ker[1][i2];
+                    const double vr = cj.x * k23, vi = cj.y * k23;
+#pragma unroll
+                    for (int i1 = 0; i1 < W; ++i1) {
+                        const long long g = g2 + g1[i1];
+                        unsafeAtomicAdd(grid + 2 * g, vr * ker[0][i1]);
+                        unsafeAtomicAdd(grid + 2 * g + 1, vi * ker[0][i1]);
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <int DIM>
+void launch_spread(int w, int grid, hipStream_t st, const SpreadArgs &a, long long M, const double *x, const double *y,
+                   const double *z, const double2 *c, double *g) {
+    switch (w) {
+    case 2: spread_kernel<DIM, 2><<<grid, 256, 0, st>>>(a, M, x, y, z, c, g); break;
+    case 4: spread_kernel<DIM, 4><<<grid, 256, 0, st>>>(a, M, x, y, z, c, g); break;
+    case 6: spread_kernel<DIM, 6><<<grid, 256, 0, st>>>(a, M, x, y, z, c, g); break;
+    case 8: spread_kernel<DIM, 8><<<grid, 256, 0, st>>>(a, M, x, y, z, c, g); break;
+    case 10: spread_kernel<DIM, 10><<<grid, 256, 0, st>>>(a, M, x, y, z, c, g); break;
+    case 12: spread_kernel<DIM, 12><<<grid, 256, 0, st>>>(a, M, x, y, z, c, g); break;
+    case 14: spread_kernel<DIM, 14><<<grid, 256, 0, st>>>(a, M, x, y, z, c, g); break;
+    default: spread_kernel<DIM, 16><<<grid, 256, 0, st>>>(a, M, x, y, z, c, g); break;
+    }
+}
+
+// fk(m) = B(k mod n) * corr1(m1) corr2(m2) corr3(m3), CMCL order (k1 fastest).
+struct DeconvArgs {
+    long long N[3], n[3];
+    long long sub[3];        // output stride per axis (every sub-th mode kept)
+    long long out[3];        // output counts
+};
+
+__global__ void deconv_kernel(DeconvArgs a, const double2 *__restrict__ B, const double *__restrict__ c1,
+                              const double *__restrict__ c2, const double *__restrict__ c3, double2 *__restrict__ fk) {
+    const long long total = a.out[0] * a.out[1] * a.out[2];
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long long)gridDim.x * blockDim.x) {
+        const long long o1 = e % a.out[0];
+        const long long o2 = (e / a.out[0]) % a.out[1];
+        const long long o3 = e / (a.out[0] * a.out[1]);
+        const long long m1 = o1 * a.sub[0], m2 = o2 * a.sub[1], m3 = o3 * a.sub[2];
+        const long long k1 = m1 - a.N[0] / 2, k2 = m2 - a.N[1] / 2, k3 = m3 - a.N[2] / 2;
+        const long long g = wrapi(k1, a.n[0]) + a.n[0] * (wrapi(k2, a.n[1]) + a.n[1] * wrapi(k3, a.n[2]));
+        const double f = c1[m1] * c2[m2] * c3[m3];
+        const double2 b = B[g];
+        fk[e] = make_double2(b.x * f, b.y * f);
+    }
+}
+
+__global__ void zero_kernel(long long n, double *__restrict__ p) {
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) p[e] = 0.0;
+}
+
+__global__ void occ_weights_kernel(long long M, double occ, double2 *__restrict__ c) {
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < M; e += (long long)gridDim.x * blockDim.x)
+        c[e] = make_double2(occ, 0.0);
+}
+
+inline int grid_for(long long n) {
+    long long g = (n + 255) / 256;
+    if (g > 16384) g = 16384;
+    return (int)(g < 1 ? 1 : g);
+}
+
+// smallest even 2^a 3^b 5^c >= n
+long long next_smooth(long long n) {
+    if (n < 2) n = 2;
+    for (long long m = n + (n & 1);; m += 2) {
+        long long r = m;
+        for (long long p : {2LL, 3LL, 5LL})
+            while (r % p == 0) r /= p;
+        if (r == 1) return m;
+    }
+}
+
+// Gauss-Legendre nodes/weights on [-1, 1]: Newton on P_q from the recurrence.
+void gauss_legendre(int q, std::vector<double> &xs, std::vector<double> &ws) {
+    xs.assign(q, 0.0);
+    ws.assign(q, 0.0);
+    const double pi = 3.14159265358979323846;
+    for (int i = 0; i < (q + 1) / 2; ++i) {
+        double x = std::cos(pi * (i + 0.75) / (q + 0.5));
+        double dp = 0.0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = x;
+            for (int k = 2; k <= q; ++k) {
+                const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = q * (x * p1 - p0) / (x * x - 1.0);
+            const double dx = p1 / dp;
+            x -= dx;
+            if (std::fabs(dx) < 1e-16) break;
+        }
+        double p0 = 1.0, p1 = x;
+        for (int k = 2; k <= q; ++k) {
+            const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+            p0 = p1;
+            p1 = p2;
+        }
+        dp = q * (x * p1 - p0) / (x * x - 1.0);
+        const double wgt = 2.0 / ((1.0 - x * x) * dp * dp);
+        xs[i] = -x;
+        xs[q - 1 - i] = x;
+        ws[i] = wgt;
+        ws[q - 1 - i] = wgt;
+    }
+}
+
+// 1 / phihat(k) for k = -N/2 .. (N-1)/2, phihat(k) = int_{-w/2}^{w/2} phi(2t/w) cos(2 pi k t / n) dt
+std::vector<double> correction(long long N, long long n, int w, double beta) {
+    const int q = 4 * w + 24;
+    std::vector<double> gx, gw;
+    gauss_legendre(q, gx, gw);
+    const double h = 0.5 * w;
+    const double pi = 3.14159265358979323846;
+    std::vector<double> out(N);
+    for (long long m = 0; m < N; ++m) {
+        const double k = (double)(m - N / 2);
+        double s = 0.0;
+        for (int i = 0; i < q; ++i) {
+            const double t = h * gx[i];                       // t in [-w/2, w/2]
+            const double zz = gx[i];
+            const double ph = std::exp(beta * (std::sqrt(std::max(0.0, 1.0 - zz * zz)) - 1.0));
+            s += gw[i] * ph * std::cos(2.0 * pi * k * t / (double)n);
+        }
+        out[m] = 1.0 / (s * h);
+    }
+    return out;
+}
+
+}  // namespace
+
+struct NufftState {
+    std::map<std::tuple<long long, long long, long long>, hipfftHandle> plans;
+    DevBuf grid, pts[3], c, fk, corr[3];
+};
+
+namespace dc {
+void nufft_release(dc_ctx *c) {
+    if (!c->nufft) return;
+    for (auto &kv : c->nufft->plans) (void)hipfftDestroy(kv.second);
+    NufftState *s = c->nufft;
+    for (DevBuf *b : {&s->grid, &s->pts[0], &s->pts[1], &s->pts[2], &s->c, &s->fk, &s->corr[0], &s->corr[1], &s->corr[2]})
+        if (b->p) (void)hipFree(b->p);
+    delete s;
+    c->nufft = nullptr;
+}
+}  // namespace dc
+
+namespace {
+
+NufftState &nu_state(dc_ctx *c) {
+    if (!c->nufft) c->nufft = new NufftState();
+    return *c->nufft;
+}
+
+int nu_params(dc_ctx *c, int dim, double tol, const long long N[3], NuParams &p) {
+    if (dim < 1 || dim > 3) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "NUFFT dimension must be 1, 2 or 3");
+    if (!(tol > 0.0)) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "NUFFT tolerance must be > 0");
+    p.dim = dim;
+    p.w = (int)std::ceil(std::log10(1.0 / tol)) + 1;
+    p.w = std::min(MAXW, std::max(2, p.w));
+    p.w += p.w & 1;                       // even widths only (compiled kernels); accuracy only improves
+    p.beta = 2.30 * p.w;
+    for (int d = 0; d < 3; ++d) {
+        p.N[d] = d < dim ? N[d] : 1;
+        if (p.N[d] < 1) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "NUFFT mode counts must be >= 1");
+        p.n[d] = d < dim ? next_smooth(std::max(2 * p.N[d], (long long)(2 * p.w))) : 1;
+    }
+    if ((double)p.n[0] * p.n[1] * p.n[2] > 2.0e9) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "NUFFT fine grid too large");
+    return 0;
+}
+
+// Spread, transform, deconvolve; points/weights already on the device.
+int nufft_run(dc_ctx *c, const NuParams &p, long long M, const double *dx, const double *dy, const double *dz,
+              const double2 *dc, int iflag, const long long sub[3], const long long out[3], double2 *dfk) {
+    NufftState &s = nu_state(c);
+    const long long ng = p.n[0] * p.n[1] * p.n[2];
+    int st = ensure(c, s.grid, sizeof(double2) * (size_t)ng);
+    if (st) return st;
+    for (int d = 0; d < 3; ++d) {
+        std::vector<double> corr = d < p.dim ? correction(p.N[d], p.n[d], p.w, p.beta) : std::vector<double>(1, 1.0);
+        if ((st = upload(c, s.corr[d], corr.data(), corr.size()))) return st;
+        HIPCHK(hipStreamSynchronize(c->st));   // corr is a host temporary
+    }
+    double *g = ptr<double>(s.grid);
+    zero_kernel<<<grid_for(2 * ng), 256, 0, c->st>>>(2 * ng, g);
+    SpreadArgs a{};
+    a.dim = p.dim;
+    a.w = p.w;
+    a.beta = p.beta;
+    a.half_w = 0.5 * p.w;
+    a.two_over_w = 2.0 / p.w;
+    for (int d = 0; d < 3; ++d) {
+        a.n[d] = p.n[d];
+        a.scale[d] = (double)p.n[d] / (2.0 * 3.14159265358979323846);
+    }
+    if (M > 0) {
+        if (p.dim == 1)
+            launch_spread<1>(p.w, grid_for(M), c->st, a, M, dx, dy, dz, dc, g);
+        else if (p.dim == 2)
+            launch_spread<2>(p.w, grid_for(M), c->st, a, M, dx, dy, dz, dc, g);
+        else
+            launch_spread<3>(p.w, grid_for(M), c->st, a, M, dx, dy, dz, dc, g);
+    }
+    HIPCHK(hipGetLastError());
+    // fine-grid transform (row-major dims: slowest first)
+    auto key = std::make_tuple(p.n[0], p.n[1], p.n[2]);
+    hipfftHandle h;
+    auto it = s.plans.find(key);
+    if (it == s.plans.end()) {
+        int dims[3];
+        for (int d = 0; d < p.dim; ++d) dims[d] = (int)p.n[p.dim - 1 - d];
+        hipfftResult r = hipfftPlanMany(&h, p.dim, dims, nullptr, 1, 0, nullptr, 1, 0, HIPFFT_Z2Z, 1);
+        if (r != HIPFFT_SUCCESS) return set_err(c, DC_ERR_DEVICE, DC_ER_LIB, "hipfftPlanMany failed (" + std::to_string((int)r) + ")");
+        s.plans[key] = h;
+    } else {
+        h = it->second;
+    }
+    if (hipfftSetStream(h, c->st) != HIPFFT_SUCCESS) return set_err(c, DC_ERR_DEVICE, DC_ER_LIB, "hipfftSetStream failed");
+    hipfftResult r = hipfftExecZ2Z(h, reinterpret_cast<hipfftDoubleComplex *>(g), reinterpret_cast<hipfftDoubleComplex *>(g),
+                                   iflag > 0 ? HIPFFT_BACKWARD : HIPFFT_FORWARD);
+    if (r != HIPFFT_SUCCESS) return set_err(c, DC_ERR_DEVICE, DC_ER_LIB, "hipfftExecZ2Z failed (" + std::to_string((int)r) + ")");
+    DeconvArgs da{};
+    for (int d = 0; d < 3; ++d) {
+        da.N[d] = p.N[d];
+        da.n[d] = p.n[d];
+        da.sub[d] = sub[d];
+        da.out[d] = out[d];
+    }
+    deconv_kernel<<<grid_for(out[0] * out[1] * out[2]), 256, 0, c->st>>>(da, reinterpret_cast<const double2 *>(g), ptr<double>(s.corr[0]),
+                                                                          ptr<double>(s.corr[1]), ptr<double>(s.corr[2]), dfk);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int check_points(dc_ctx *c, int dim, long long M, const double *x, const double *y, const double *z) {
+    if (M < 0) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "negative point count");
+    if (M > 0 && (!x || (dim > 1 && !y) || (dim > 2 && !z))) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "null coordinate array");
+    if (M > (1LL << 31)) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "more than 2^31 points");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dc_nufft1_dev(dc_ctx *c, int dim, int64_t M, const double *x, const double *y, const double *z, const double *cw,
+                  int iflag, double tol, int64_t n1, int64_t n2, int64_t n3, double *fk) {
+    if (!c) return DC_ERR_ARG;
+    int st = check_points(c, dim, M, x, y, z);
+    if (st) return st;
+    if (!fk || (M > 0 && !cw)) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "null weights or output");
+    const long long N[3] = {n1, n2, n3};
+    NuParams p;
+    if ((st = nu_params(c, dim, tol, N, p))) return st;
+    HIPCHK(hipSetDevice(c->device));
+    const long long sub[3] = {1, 1, 1};
+    const long long out[3] = {p.N[0], p.N[1], p.N[2]};
+    return nufft_run(c, p, M, x, y, z, reinterpret_cast<const double2 *>(cw), iflag, sub, out, reinterpret_cast<double2 *>(fk));
+}
+
+int dc_nufft1(dc_ctx *c, int dim, int64_t M, const double *x, const double *y, const double *z, const double *cw,
+              int iflag, double tol, int64_t n1, int64_t n2, int64_t n3, double *fk) {
+    if (!c) return DC_ERR_ARG;
+    int st = check_points(c, dim, M, x, y, z);
+    if (st) return st;
+    if (!fk || (M > 0 && !cw)) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "null weights or output");
+    const long long N[3] = {n1, n2, n3};
+    NuParams p;
+    if ((st = nu_params(c, dim, tol, N, p))) return st;
+    HIPCHK(hipSetDevice(c->device));
+    NufftState &s = nu_state(c);
+    const double *xs[3] = {x, y, z};
+    for (int d = 0; d < dim; ++d)
+        if ((st = upload(c, s.pts[d], xs[d], (size_t)M))) return st;
+    if ((st = upload(c, s.c, cw, 2 * (size_t)M))) return st;
+    const long long nout = p.N[0] * p.N[1] * p.N[2];
+    if ((st = ensure(c, s.fk, sizeof(double2) * (size_t)nout))) return st;
+    const long long sub[3] = {1, 1, 1};
+    const long long out[3] = {p.N[0], p.N[1], p.N[2]};
+    if ((st = nufft_run(c, p, M, ptr<double>(s.pts[0]), dim > 1 ? ptr<double>(s.pts[1]) : nullptr,
+                        dim > 2 ? ptr<double>(s.pts[2]) : nullptr, ptr<double2>(s.c), iflag, sub, out, ptr<double2>(s.fk))))
+        return st;
+    HIPCHK(hipMemcpyAsync(fk, s.fk.p, sizeof(double2) * (size_t)nout, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return DC_OK;
+}
+
+int dc_four_strucf_nufft_dev(dc_ctx *c, int dim, int64_t nat, const double *xpos, const double *ypos, const double *zpos,
+                             double occ, const int inc[3], const int lscales[3], double *tcsf) {
+    if (!c) return DC_ERR_ARG;
+    int st = check_points(c, dim, nat, xpos, ypos, zpos);
+    if (st) return st;
+    if (!inc || !lscales || !tcsf) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "null inc, lscales or tcsf");
+    long long N[3], sub[3], out[3];
+    for (int d = 0; d < 3; ++d) {
+        if (d < dim && (inc[d] < 1 || lscales[d] < 1)) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "inc and lscales must be >= 1");
+        sub[d] = d < dim ? lscales[d] : 1;
+        out[d] = d < dim ? inc[d] : 1;
+        N[d] = d < dim ? (long long)(inc[d] - 1) * lscales[d] + 1 : 1;   // fourier_finufft.f90:56, 113-114, 199-201
+    }
+    NuParams p;
+    const double tol = dim == 3 ? 1.0e-7 : 1.0e-9;                      // fourier_finufft.f90:59, 121, 207
+    if ((st = nu_params(c, dim, tol, N, p))) return st;
+    HIPCHK(hipSetDevice(c->device));
+    NufftState &s = nu_state(c);
+    if ((st = ensure(c, s.c, sizeof(double2) * (size_t)std::max<int64_t>(nat, 1)))) return st;
+    occ_weights_kernel<<<grid_for(nat), 256, 0, c->st>>>(nat, occ, ptr<double2>(s.c));
+    HIPCHK(hipGetLastError());
+    return nufft_run(c, p, nat, xpos, ypos, zpos, ptr<double2>(s.c), 1, sub, out, reinterpret_cast<double2 *>(tcsf));
+}
+
+int dc_four_strucf_nufft(dc_ctx *c, int dim, int64_t nat, const double *xpos, const double *ypos, const double *zpos,
+                         double occ, const int inc[3], const int lscales[3], double *tcsf) {
+    if (!c) return DC_ERR_ARG;
+    int st = check_points(c, dim, nat, xpos, ypos, zpos);
+    if (st) return st;
+    if (!inc || !lscales || !tcsf) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "null inc, lscales or tcsf");
+    HIPCHK(hipSetDevice(c->device));
+    NufftState &s = nu_state(c);
+    const double *xs[3] = {xpos, ypos, zpos};
+    for (int d = 0; d < dim; ++d)
+        if ((st = upload(c, s.pts[d], xs[d], (size_t)nat))) return st;
+    long long nout = 1;
+    for (int d = 0; d < dim; ++d) nout *= inc[d];
+    if ((st = ensure(c, s.fk, sizeof(double2) * (size_t)nout))) return st;
+    if ((st = dc_four_strucf_nufft_dev(c, dim, nat, ptr<double>(s.pts[0]), dim > 1 ? ptr<double>(s.pts[1]) : nullptr,
+                                       dim > 2 ? ptr<double>(s.pts[2]) : nullptr, occ, inc, lscales, ptr<double>(s.fk))))
+        return st;
+    HIPCHK(hipMemcpyAsync(tcsf, s.fk.p, sizeof(double2) * (size_t)nout, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return DC_OK;
+}
+
+}  // extern "C"

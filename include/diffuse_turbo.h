@@ -372,6 +372,33 @@ int dc_qval(dc_ctx *ctx, const dc_qval_desc *q, const double *csf, const double 
 int dc_qval_dev(dc_ctx *ctx, const dc_qval_desc *q, const double *d_csf, const double *d_acsf,
                 const double *d_dsi, double *d_out);
 
+/* ------------------------------------------------------------------------
+ * NUFFT technique (four_run_nufft, fourier_sup.f90:327-690).
+ *
+ * dc_nufft1 replaces FINUFFT's type-1 calls finufft1d1 / finufft2d1 /
+ * finufft3d1 as four_strucf_1d/2d/3d make them (fourier_finufft.f90:62,
+ * 124, 210, default options):
+ *   fk(k1,k2,k3) = sum_j c_j exp(iflag * i * (k1 x_j + k2 y_j + k3 z_j)),
+ *   k_d = -N_d/2 .. (N_d-1)/2 in increasing order, k1 fastest,
+ * x_j, y_j, z_j in [-3 pi, 3 pi) (periodic), c and fk complex interleaved,
+ * relative accuracy about tol.  dim 1..3; y/z (and n2/n3) unused below dim.
+ * ------------------------------------------------------------------------ */
+int dc_nufft1(dc_ctx *ctx, int dim, int64_t M, const double *x, const double *y, const double *z,
+              const double *c, int iflag, double tol, int64_t n1, int64_t n2, int64_t n3, double *fk);
+int dc_nufft1_dev(dc_ctx *ctx, int dim, int64_t M, const double *x, const double *y, const double *z,
+                  const double *c, int iflag, double tol, int64_t n1, int64_t n2, int64_t n3, double *fk);
+
+/* four_strucf_1d / _2d / _3d (fourier_finufft.f90:19-245): nat atoms of one
+ * type at the scaled positions of four_getatm_anis_iam (2 pi scales pos /
+ * ncells), weight occ each, NUFFT on (inc-1)*lscales+1 modes per axis with
+ * iflag = +1 and DISCUS's tolerance (1e-9 in 1D/2D, 1e-7 in 3D); every
+ * lscales-th mode is kept: tcsf(inc(1), inc(2), inc(3)) complex, abscissa
+ * fastest (inc(d) = 1 beyond dim). */
+int dc_four_strucf_nufft(dc_ctx *ctx, int dim, int64_t nat, const double *xpos, const double *ypos,
+                         const double *zpos, double occ, const int inc[3], const int lscales[3], double *tcsf);
+int dc_four_strucf_nufft_dev(dc_ctx *ctx, int dim, int64_t nat, const double *xpos, const double *ypos,
+                             const double *zpos, double occ, const int inc[3], const int lscales[3], double *tcsf);
+
 #ifdef __cplusplus
 }
 #endif

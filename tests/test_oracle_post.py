@@ -281,3 +281,56 @@ def test_qval_closed_forms():
     assert np.allclose(P.qval(P.VAL_F2AVER, False, *args), np.abs(cp[i1, 0]) ** 2)
     assert np.allclose(P.qval(P.VAL_FAVER2, False, *args), np.abs(cp[i1, 0]) ** 2)
     assert np.allclose(P.qval(P.VAL_FAVERB, False, *args), np.abs(cf[i1, 0]))
+
+
+# ---------------------------------------------------------------- NUFFT checker
+from oracle import nufft as NU
+
+
+def test_nufft_direct_single_point():
+    x = np.array([0.37])
+    f = NU.nufft1_direct([x], [2.0 + 0j], 1, [7])
+    k = np.arange(7) - 3
+    assert np.allclose(f, 2.0 * np.exp(1j * k * 0.37), rtol=1e-15)
+
+
+def test_nufft_direct_lattice_dirichlet():
+    L = 8
+    m = np.arange(L)
+    x = 2 * np.pi * m / L * 0.3          # points 2 pi (0.3 m / L)
+    f = NU.nufft1_direct([x], np.ones(L, complex), 1, [9])
+    k = np.arange(9) - 4
+    t = 0.3 * k / L
+    want = np.where(np.abs(np.sin(np.pi * t)) < 1e-14, L,
+                    np.exp(1j * np.pi * t * (L - 1)) * np.sin(np.pi * L * t) / np.where(np.sin(np.pi * t) == 0, 1, np.sin(np.pi * t)))
+    assert np.allclose(f, want, rtol=1e-12, atol=1e-12)
+
+
+def test_nufft_direct_conjugate_and_bruteforce():
+    import cmath
+    rng = np.random.default_rng(3)
+    pts = [rng.uniform(-np.pi, np.pi, 3) for _ in range(3)]
+    c = rng.random(3) + 0j
+    fp = NU.nufft1_direct(pts, c, 1, [4, 3, 5])
+    fm = NU.nufft1_direct(pts, c, -1, [4, 3, 5])
+    assert np.allclose(fp, np.conj(fm), rtol=1e-13)                # real weights
+    assert np.allclose(fp[1:, :, :][::-1, ::-1, ::-1], fm[1:, :, :], rtol=1e-13)   # f+(-k) = f-(k)
+    for (a, b, cc) in [(0, 0, 0), (3, 2, 4), (1, 1, 2)]:
+        k = (a - 2, b - 1, cc - 2)
+        s = sum(c[j] * cmath.exp(1j * (k[0] * pts[0][j] + k[1] * pts[1][j] + k[2] * pts[2][j])) for j in range(3))
+        assert abs(fp[a, b, cc] - s) < 1e-14
+
+
+def test_strucf_nufft_is_structure_factor():
+    """positions 2 pi s x / n (four_getatm_anis_iam) -> F(h = k s / n)."""
+    rng = np.random.default_rng(4)
+    frac = rng.random((50, 3)) * 6.0
+    s, n = 2.0, 6.0
+    pts = [2 * np.pi * s * frac[:, d] / n for d in range(3)]
+    inc, ls = (5, 4, 3), (2, 1, 3)
+    t = NU.strucf_nufft(3, pts, 0.5, inc, ls)
+    for idx in [(0, 0, 0), (4, 3, 2), (2, 1, 1)]:
+        k = [idx[d] * ls[d] - ((inc[d] - 1) * ls[d] + 1) // 2 for d in range(3)]
+        h = np.array(k) * s / n
+        want = 0.5 * np.exp(2j * np.pi * frac @ h).sum()
+        assert abs(t[idx] - want) < 1e-11
