@@ -46,6 +46,12 @@ struct SpreadArgs {
     double scale[3];     // n / (2 pi)
 };
 
+inline int grid_for(long long n) {
+    long long g = (n + 255) / 256;
+    if (g > 16384) g = 16384;
+    return (int)(g < 1 ? 1 : g);
+}
+
 __device__ inline long long wrapi(long long l, long long n) {
     long long r = l % n;
     return r < 0 ? r + n : r;
@@ -130,9 +136,77 @@ __global__ __launch_bounds__(256) void spread_kernel(SpreadArgs a, long long M, 
     }
 }
 
+// 3D with coalesced atomics: a team of 2W lanes spreads one point; lane t
+// takes abscissa offset t/2 and component t%2, so each atomic instruction of
+// the team covers 2W consecutive doubles (one or two 128-byte lines) instead
+// of one double per lane.  64/(2W) points per wave.
+template <int W>
+__global__ __launch_bounds__(256) void spread3_team_kernel(SpreadArgs a, long long M, const double *__restrict__ x,
+                                                           const double *__restrict__ y, const double *__restrict__ z,
+                                                           const double2 *__restrict__ c, double *__restrict__ grid) {
+    constexpr int T = 2 * W;
+    constexpr int PER = 64 / T;
+    const int lane = threadIdx.x & 63;
+    const int team = lane / T;
+    const int t = lane - team * T;
+    const int i1 = t >> 1, comp = t & 1;
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+    for (long long base = wave * PER; base < M; base += nwaves * PER) {
+        const long long j = base + team;
+        if (team >= PER || j >= M) continue;
+        double u[3], lo[3];
+        const double *xs[3] = {x, y, z};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            double v = xs[d][j] * a.scale[d];
+            v = v - floor(v / (double)a.n[d]) * (double)a.n[d];
+            u[d] = v;
+            lo[d] = ceil(v - a.half_w);
+        }
+        double k2[W], k3[W];
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            const double z2 = (lo[1] + (double)i - u[1]) * a.two_over_w;
+            const double z3 = (lo[2] + (double)i - u[2]) * a.two_over_w;
+            const double s2 = 1.0 - z2 * z2, s3 = 1.0 - z3 * z3;
+            k2[i] = s2 > 0.0 ? exp(a.beta * (sqrt(s2) - 1.0)) : 0.0;
+            k3[i] = s3 > 0.0 ? exp(a.beta * (sqrt(s3) - 1.0)) : 0.0;
+        }
+        const double z1 = (lo[0] + (double)i1 - u[0]) * a.two_over_w;
+        const double s1 = 1.0 - z1 * z1;
+        const double k1 = s1 > 0.0 ? exp(a.beta * (sqrt(s1) - 1.0)) : 0.0;
+        const double cv = (comp ? c[j].y : c[j].x) * k1;
+        const long long g1 = wrapi((long long)lo[0] + i1, a.n[0]);
+        const long long l1 = (long long)lo[1], l2 = (long long)lo[2];
+        for (int i3 = 0; i3 < W; ++i3) {
+            const long long g3 = wrapi(l2 + i3, a.n[2]) * a.n[1];
+            const double v3 = cv * k3[i3];
+#pragma unroll
+            for (int i2 = 0; i2 < W; ++i2) {
+                const long long g = (g3 + wrapi(l1 + i2, a.n[1])) * a.n[0] + g1;
+                unsafeAtomicAdd(grid + 2 * g + comp, v3 * k2[i2]);
+            }
+        }
+    }
+}
+
 template <int DIM>
 void launch_spread(int w, int grid, hipStream_t st, const SpreadArgs &a, long long M, const double *x, const double *y,
                    const double *z, const double2 *c, double *g) {
+    if constexpr (DIM == 3) {
+        const int tg = grid_for(M * 2 * w);   // ~2w threads per point
+        switch (w) {
+        case 2: spread3_team_kernel<2><<<tg, 256, 0, st>>>(a, M, x, y, z, c, g); return;
+        case 4: spread3_team_kernel<4><<<tg, 256, 0, st>>>(a, M, x, y, z, c, g); return;
+        case 6: spread3_team_kernel<6><<<tg, 256, 0, st>>>(a, M, x, y, z, c, g); return;
+        case 8: spread3_team_kernel<8><<<tg, 256, 0, st>>>(a, M, x, y, z, c, g); return;
+        case 10: spread3_team_kernel<10><<<tg, 256, 0, st>>>(a, M, x, y, z, c, g); return;
+        case 12: spread3_team_kernel<12><<<tg, 256, 0, st>>>(a, M, x, y, z, c, g); return;
+        case 14: spread3_team_kernel<14><<<tg, 256, 0, st>>>(a, M, x, y, z, c, g); return;
+        default: spread3_team_kernel<16><<<tg, 256, 0, st>>>(a, M, x, y, z, c, g); return;
+        }
+    }
     switch (w) {
     case 2: spread_kernel<DIM, 2><<<grid, 256, 0, st>>>(a, M, x, y, z, c, g); break;
     case 4: spread_kernel<DIM, 4><<<grid, 256, 0, st>>>(a, M, x, y, z, c, g); break;
@@ -177,11 +251,6 @@ __global__ void occ_weights_kernel(long long M, double occ, double2 *__restrict_
         c[e] = make_double2(occ, 0.0);
 }
 
-inline int grid_for(long long n) {
-    long long g = (n + 255) / 256;
-    if (g > 16384) g = 16384;
-    return (int)(g < 1 ? 1 : g);
-}
 
 // smallest even 2^a 3^b 5^c >= n
 long long next_smooth(long long n) {
