@@ -39,6 +39,7 @@ EXPORTS = [
     "dc_batch_chi2", "dc_set_autotune", "dc_set_graphs", "dc_batch_info",
     "dc_four_post", "dc_four_post_dev", "dc_four_fill", "dc_four_fill_dev", "dc_four_accumulate_dev",
     "dc_qval", "dc_qval_dev", "dc_nufft1", "dc_nufft1_dev", "dc_four_strucf_nufft", "dc_four_strucf_nufft_dev",
+    "dc_four_run_nufft", "dc_four_run_nufft_dev",
 ]
 
 DC_ERR_NOSUM = -175
@@ -153,6 +154,8 @@ def lib():
     L.dc_nufft1_dev.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                 C.c_int, C.c_double, C.c_int64, C.c_int64, C.c_int64, C.c_void_p]
     L.dc_four_strucf_nufft.argtypes = [C.c_void_p, C.c_int, C.c_int64, _dp, _dp, _dp, C.c_double, _ip, _ip, _dp]
+    L.dc_four_run_nufft.argtypes = [C.c_void_p, C.POINTER(RunDesc), _dp, _dp, _dp]
+    L.dc_four_run_nufft_dev.argtypes = [C.c_void_p, C.POINTER(RunDesc), _dp, C.c_void_p, C.c_void_p]
     L.dc_four_strucf_nufft_dev.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
                                            C.c_double, _ip, _ip, C.c_void_p]
     _lib = L
@@ -620,3 +623,13 @@ class Context:
         self.check(lib().dc_four_strucf_nufft(self.h, dim, len(xs[0]), _d(xyz[0]), _d(xyz[1]), _d(xyz[2]), float(occ),
                                               _i(ii), _i(ls), _d(out)))
         return out.view(np.complex128).reshape(tuple(int(v) for v in ii[:dim]), order="F")
+
+    def four_run_nufft(self, desc, type_occ=None):
+        """four_run_nufft: returns {'csf', 'dsi'} over inc (Fortran order)."""
+        num = tuple(desc.inc)
+        n = int(np.prod(num))
+        csf = np.zeros(2 * n)
+        dsi = np.zeros(n)
+        occ = None if type_occ is None else np.ascontiguousarray(np.asarray(type_occ, np.float64).ravel())
+        self.check(lib().dc_four_run_nufft(self.h, C.byref(desc), _d(occ), _d(csf), _d(dsi)))
+        return {"csf": csf.view(np.complex128).reshape(num, order="F"), "dsi": dsi.reshape(num, order="F")}

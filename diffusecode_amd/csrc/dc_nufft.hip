@@ -18,6 +18,7 @@
 #include <hipfft/hipfft.h>
 
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 #include <map>
 #include <tuple>
@@ -522,6 +523,262 @@ int dc_four_strucf_nufft(dc_ctx *c, int dim, int64_t nat, const double *xpos, co
         return st;
     HIPCHK(hipMemcpyAsync(tcsf, s.fk.p, sizeof(double2) * (size_t)nout, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
+    return DC_OK;
+}
+
+}  // extern "C"
+
+// ---- run level: four_run_nufft (fourier_sup.f90:327-690), isotropic IAM ------
+namespace {
+
+__global__ void gather_scaled_kernel(long long n, const int *__restrict__ idx, const double *__restrict__ pos, double fx,
+                                     double fy, double fz, double *__restrict__ x, double *__restrict__ y,
+                                     double *__restrict__ z) {
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+        const long long a = idx[e];
+        x[e] = pos[3 * a] * fx;
+        y[e] = pos[3 * a + 1] * fy;
+        z[e] = pos[3 * a + 2] * fz;
+    }
+}
+
+// csf (+)= tcsf * cfact(istl(i,j,h), t)  (tcsf_form_iso, fourier_form.f90)
+__global__ void form_acc_kernel(Grid g, int t, int ncfact, int first, const double2 *__restrict__ cfact,
+                                const double2 *__restrict__ fk, double2 *__restrict__ csf) {
+    const long long n = (long long)g.n[0] * g.n[1] * g.n[2];
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(e % g.n[0]);
+        const int j = (int)((e / g.n[0]) % g.n[1]);
+        const int l = (int)(e / ((long long)g.n[0] * g.n[1]));
+        const double2 cf = cfact[(long long)t * ncfact + istl_point(g, i, j, l)];
+        const double2 f = fk[e];
+        const double2 v = make_double2(f.x * cf.x - f.y * cf.y, f.x * cf.y + f.y * cf.x);
+        csf[e] = first ? v : make_double2(csf[e].x + v.x, csf[e].y + v.y);
+    }
+}
+
+// four_aver_finufft_n (fourier_sup.f90:1074-1190): F = 0 at the pixels of
+// integer hkl reached through vi^-1 (skipping pixels outside the grid).
+struct AverArgs {
+    int n[3];
+    int h0[3], nh[3];
+    double e[3];
+    double inv[9];      // row-major
+};
+
+__global__ void aver_zero_kernel(AverArgs a, double2 *__restrict__ csf) {
+    const long long tot = (long long)a.nh[0] * a.nh[1] * a.nh[2];
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (long long)gridDim.x * blockDim.x) {
+        const int h = a.h0[0] + (int)(e % a.nh[0]);
+        const int k = a.h0[1] + (int)((e / a.nh[0]) % a.nh[1]);
+        const int l = a.h0[2] + (int)(e / ((long long)a.nh[0] * a.nh[1]));
+        const double d[3] = {h - a.e[0], k - a.e[1], l - a.e[2]};
+        long long ii[3];
+        bool ok = true;
+        for (int r = 0; r < 3; ++r) {
+            const double v = a.inv[3 * r] * d[0] + a.inv[3 * r + 1] * d[1] + a.inv[3 * r + 2] * d[2] + 1.0;
+            const double nv = v >= 0.0 ? floor(v + 0.5) : -floor(-v + 0.5);
+            ok = ok && fabs(v - nv) < 1.0e-9;
+            ii[r] = (long long)nv - 1;
+            ok = ok && ii[r] >= 0 && ii[r] < a.n[r];
+        }
+        if (ok) csf[ii[0] + a.n[0] * (ii[1] + (long long)a.n[1] * ii[2])] = make_double2(0.0, 0.0);
+    }
+}
+
+__global__ void nufft_finish_kernel(long long n, double sq, double ds, const double2 *__restrict__ f, double *__restrict__ csf,
+                                    double *__restrict__ dsi) {
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+        const double2 v = f[e];
+        if (dsi) dsi[e] = fmax(0.0, v.x * v.x + v.y * v.y) * ds;
+        if (csf) {
+            csf[2 * e] = v.x * sq;
+            csf[2 * e + 1] = v.y * sq;
+        }
+    }
+}
+
+double fint(double x) { return x < 0 ? std::ceil(x) : std::floor(x); }                 // Fortran INT
+long long fnint(double x) { return (long long)(x >= 0 ? std::floor(x + 0.5) : -std::floor(-x + 0.5)); }
+
+// Per-axis transform parameters of four_run_nufft (fourier_sup.f90:386-451).
+int nufft_run_params(dc_ctx *c, const dc_run_desc *r, double scales[3], int lscales[3], int ncells[3]) {
+    const double EPS = 1.0e-8, EPS2 = 1.0e-4, MAXSCALE = 3.0;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int64_t i = 0; i < c->natoms; ++i)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = std::min(lo[d], c->h_pos[3 * i + d]);
+            hi[d] = std::max(hi[d], c->h_pos[3 * i + d]);
+        }
+    int is_dim = 0;
+    for (int i = 0; i < 3; ++i) {
+        const double vii = r->vi[4 * i];
+        for (int j = 0; j < 3; ++j)
+            if (j != i && r->vi[i + 3 * j] != 0.0)
+                return set_err(c, DC_ERR_ARG, DC_ER_LIB, "NUFFT technique: increment vectors must run along a*, b*, c*");
+        if ((hi[i] - lo[i] > 1.0e-8 && c->icc[i] > 1) || std::fabs(vii) > 1.0e-8) ++is_dim;
+        if (c->icc[i] == 1 && std::fabs(vii) > 1.0e-8)
+            return set_err(c, DC_ERR_ARG, DC_ER_LIB, "NUFFT technique: crystals one cell thick along an axis are not supported");
+    }
+    if (is_dim != 3) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "NUFFT technique: only 3D crystals are supported");
+    for (int j = 0; j < 3; ++j) {
+        scales[j] = fint(std::fabs(r->vi[4 * j] * (hi[j] - lo[j])) - 0.01) + 1.0;
+        lscales[j] = 1;
+        if (scales[j] > MAXSCALE) {
+            const long long ii = fnint(scales[j]);
+            for (long long l = 2; l <= ii; ++l) {
+                const double q = scales[j] / (double)l;
+                if (q <= MAXSCALE && std::fabs(q - fint(q)) < 0.01) {
+                    lscales[j] = (int)l;
+                    scales[j] = q;
+                    break;
+                }
+            }
+        }
+        ncells[j] = c->icc[j];
+    }
+    if (r->fave != 0.0) {
+        static const char *axes[3] = {"abscissa", "ordinate", "top axis"};
+        for (int i = 0; i < 3; ++i) {
+            const double vii = r->vi[4 * i];
+            const double t = std::fabs(vii * (double)c->icc[i]);
+            if (std::fabs(t - (double)fnint(t)) > 0.0) {
+                if (std::fabs(scales[i] - (double)fnint(scales[i])) > EPS)
+                    return set_err(c, -186, DC_ER_APPL, std::string("Increment vector along ") + axes[i]);
+                if (std::fabs(1.0 / vii - (double)fnint(1.0 / vii)) > EPS2)
+                    return set_err(c, -187, DC_ER_APPL, std::string("Increment vector along ") + axes[i]);
+                const int jj = r->inc[i] % 2 == 1 ? 1 : 0;
+                if (std::fabs(r->eck[i] + (double)((r->inc[i] - jj) / 2) * vii) > EPS)
+                    return set_err(c, -188, DC_ER_APPL, std::string("Increment vector along ") + axes[i]);
+            }
+        }
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dc_four_run_nufft_dev(dc_ctx *c, const dc_run_desc *r, const double *type_occ, double *d_csf, double *d_dsi) {
+    if (!c) return DC_ERR_ARG;
+    if (!r || !r->cfact || r->ncfact < 1) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "dc_four_run_nufft: bad descriptor");
+    if (!c->have_crystal) return set_err(c, DC_ERR_STATE, DC_ER_LIB, "dc_four_run_nufft: no crystal (dc_crystal_set)");
+    if (c->natoms < 1) return set_err(c, DC_ERR_NOATOMS, DC_ER_APPL, "No fourier will be calculated");
+    for (int k = 0; k < 3; ++k)
+        if (r->inc[k] < 1) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "dc_four_run_nufft: inc < 1");
+    double scales[3];
+    int lscales[3], ncells[3];
+    int st = nufft_run_params(c, r, scales, lscales, ncells);
+    if (st) return st;
+    Grid g{};
+    for (int k = 0; k < 3; ++k) {
+        g.n[k] = r->inc[k];
+        g.f[k] = r->inc[k];
+        g.xm[k] = r->eck[k];
+        g.uin[k] = r->vi[k];
+        g.vin[k] = r->vi[3 + k];
+        g.win[k] = r->vi[6 + k];
+    }
+    for (int k = 0; k < 9; ++k) g.rten[k] = r->rten[k];
+    if (max_corner_bin(g) > std::min(CFPKT, r->ncfact - 1))
+        return set_err(c, DC_ERR_ISTL, DC_ER_FOUR, "sin(theta)/lambda beyond the form factor table");
+    HIPCHK(hipSetDevice(c->device));
+    std::memset(&c->stats, 0, sizeof(c->stats));
+    HIPCHK(hipEventRecord(c->ev[0], c->st));
+    NufftState &s = nu_state(c);
+    const long long ntot = (long long)g.n[0] * g.n[1] * g.n[2];
+    if ((st = upload(c, c->cfact, r->cfact, 2 * (size_t)r->ncfact * c->nscat))) return st;
+    if ((st = ensure(c, s.fk, sizeof(double2) * (size_t)ntot))) return st;
+    if ((st = ensure(c, c->csf, sizeof(double2) * (size_t)ntot))) return st;   // the sum over types
+    double2 *acc = ptr<double2>(c->csf);
+    const double pi = 3.14159265358979323846;
+    double fac[3];
+    for (int d = 0; d < 3; ++d) fac[d] = 2.0 * pi * scales[d] / (double)ncells[d];   // four_getatm_anis_iam
+    long long N[3];
+    for (int d = 0; d < 3; ++d) N[d] = (long long)(r->inc[d] - 1) * lscales[d] + 1;
+    NuParams p;
+    if ((st = nu_params(c, 3, 1.0e-7, N, p))) return st;                 // fourier_finufft.f90:207
+    const long long sub[3] = {lscales[0], lscales[1], lscales[2]};
+    const long long out[3] = {r->inc[0], r->inc[1], r->inc[2]};
+    bool first = true;
+    double pairs = 0.0;
+    for (int t = 0; t < c->nscat; ++t) {
+        const long long nat = c->type_beg[t + 1] - c->type_beg[t];
+        if (nat == 0) continue;
+        if (cancelled(c)) return set_err(c, DC_ERR_CANCEL, DC_ER_COMM, "cancelled");
+        for (int d = 0; d < 3; ++d)
+            if ((st = ensure(c, s.pts[d], sizeof(double) * (size_t)nat))) return st;
+        if ((st = ensure(c, s.c, sizeof(double2) * (size_t)nat))) return st;
+        gather_scaled_kernel<<<grid_for(nat), 256, 0, c->st>>>(nat, ptr<int>(c->type_idx) + c->type_beg[t], ptr<double>(c->pos),
+                                                             fac[0], fac[1], fac[2], ptr<double>(s.pts[0]), ptr<double>(s.pts[1]),
+                                                             ptr<double>(s.pts[2]));
+        occ_weights_kernel<<<grid_for(nat), 256, 0, c->st>>>(nat, type_occ ? type_occ[t] : 1.0, ptr<double2>(s.c));
+        HIPCHK(hipGetLastError());
+        if ((st = nufft_run(c, p, nat, ptr<double>(s.pts[0]), ptr<double>(s.pts[1]), ptr<double>(s.pts[2]), ptr<double2>(s.c), 1,
+                            sub, out, ptr<double2>(s.fk))))
+            return st;
+        form_acc_kernel<<<grid_for(ntot), 256, 0, c->st>>>(g, t, r->ncfact, first ? 1 : 0, ptr<double2>(c->cfact),
+                                                          ptr<double2>(s.fk), acc);
+        HIPCHK(hipGetLastError());
+        first = false;
+        pairs += (double)nat * (double)ntot;
+    }
+    if (first) zero_kernel<<<grid_for(2 * ntot), 256, 0, c->st>>>(2 * ntot, reinterpret_cast<double *>(acc));
+    if (r->fave > 0.0) {
+        // four_aver_finufft_n: vi with unit diagonal on flat axes, integer hkl box from INT(eck)
+        double m[3][3];
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) m[a][b] = r->vi[a + 3 * b];
+        for (int a = 0; a < 3; ++a)
+            if (r->inc[a] == 1) m[a][a] = 1.0;
+        const double det = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) - m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+                           m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+        if (det == 0.0) return set_err(c, DC_ERR_ARG, DC_ER_APPL, "Increment vectors cannot be inverted");
+        AverArgs aa{};
+        aa.inv[0] = (m[1][1] * m[2][2] - m[1][2] * m[2][1]) / det;
+        aa.inv[1] = (m[0][2] * m[2][1] - m[0][1] * m[2][2]) / det;
+        aa.inv[2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) / det;
+        aa.inv[3] = (m[1][2] * m[2][0] - m[1][0] * m[2][2]) / det;
+        aa.inv[4] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) / det;
+        aa.inv[5] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) / det;
+        aa.inv[6] = (m[1][0] * m[2][1] - m[1][1] * m[2][0]) / det;
+        aa.inv[7] = (m[0][1] * m[2][0] - m[0][0] * m[2][1]) / det;
+        aa.inv[8] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) / det;
+        const int corner[3] = {1, 2, 3};   // eck(:,2) lr, eck(:,3) ul, eck(:,4) tl
+        for (int a = 0; a < 3; ++a) {
+            aa.n[a] = r->inc[a];
+            aa.e[a] = r->eck[a];
+            const long long h0 = fnint(fint(r->eck[a])), h1 = fnint(fint(r->eck[a + 3 * corner[a]]));
+            aa.h0[a] = (int)h0;
+            aa.nh[a] = (int)std::max(0LL, h1 - h0 + 1);
+        }
+        if ((long long)aa.nh[0] * aa.nh[1] * aa.nh[2] > 0)
+            aver_zero_kernel<<<grid_for((long long)aa.nh[0] * aa.nh[1] * aa.nh[2]), 256, 0, c->st>>>(aa, acc);
+        HIPCHK(hipGetLastError());
+    }
+    nufft_finish_kernel<<<grid_for(ntot), 256, 0, c->st>>>(ntot, std::sqrt(r->diff_scale), r->diff_scale, acc, d_csf, d_dsi);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev[1], c->st));
+    c->stats.pairs = pairs;
+    return DC_OK;
+}
+
+int dc_four_run_nufft(dc_ctx *c, const dc_run_desc *r, const double *type_occ, double *csf, double *dsi) {
+    if (!c) return DC_ERR_ARG;
+    if (!r) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "dc_four_run_nufft: bad descriptor");
+    HIPCHK(hipSetDevice(c->device));
+    const size_t n = (size_t)std::max(1, r->inc[0]) * std::max(1, r->inc[1]) * std::max(1, r->inc[2]);
+    int st = 0;
+    if ((st = ensure(c, c->out[0], sizeof(double) * 2 * n))) return st;
+    if ((st = ensure(c, c->out[1], sizeof(double) * n))) return st;
+    if ((st = dc_four_run_nufft_dev(c, r, type_occ, csf ? ptr<double>(c->out[0]) : nullptr, dsi ? ptr<double>(c->out[1]) : nullptr)))
+        return st;
+    if (csf) HIPCHK(hipMemcpyAsync(csf, c->out[0].p, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, c->st));
+    if (dsi) HIPCHK(hipMemcpyAsync(dsi, c->out[1].p, sizeof(double) * n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) c->stats.total_ms = ms;
     return DC_OK;
 }
 

@@ -84,3 +84,76 @@ def test_strucf_nufft_low_dim(ctx, dim):
     l2, mx = errs(got, ref)
     print(f"  strucf {dim}D rel l2 {l2:.2e}  rel max {mx:.2e}")
     assert l2 <= 1e-8 and mx <= 1e-8
+
+
+# ---------------------------------------------------------------- run level
+from diffusecode_amd import fourier as FR
+
+
+def _run(ctx, cr, lo, hi, n, fave=0.0, type_occ=None, sel=None):
+    eck, vi, inc = FR.grid_block(lo, hi, n)
+    cf = FR.formtab(cr)
+    ctx.crystal_set(cr)
+    kw = {}
+    if fave:
+        kw = dict(fave=fave, sel_cell=np.ones(int(np.prod(cr.icc)), np.uint8) if sel is None else sel)
+    desc, keep = ctx.run_desc(eck, vi, inc, cr.rten(), cf, **kw)
+    return ctx.four_run_nufft(desc, type_occ), (eck, vi, inc, cf)
+
+
+@pytest.mark.parametrize("step,half", [(1.0 / 3.0, 2.0), (1.0, 2.0)])
+def test_run_nufft_matches_exact_run(ctx, step, half):
+    """step 1/3: scales 2, lscales 1; step 1: scales 6 -> lscales 2, scales 3.
+    dsi against the exact FP64 oracle run (lots off, Friedel off in the oracle
+    sense: the full grid)."""
+    print(f"case run_nufft step={step}", flush=True)
+    cr = X.rocksalt(n=6, seed=7)
+    n = int(round(2 * half / step)) + 1
+    got, (eck, vi, inc, cf) = _run(ctx, cr, (-half,) * 3, (half,) * 3, (n, n, n))
+    ref = O.four_run(cr, eck, vi, inc, cf, four_friedel=False)
+    err = float(np.abs(got["dsi"] - ref["dsi"]).max() / ref["dsi"].max())
+    print(f"  run_nufft max|dI|/max(I) {err:.2e}")
+    assert err <= 1e-6
+
+
+def test_run_nufft_type_occ_weights(ctx):
+    """DISCUS weights each atom by cr_occ in the transform and again in cfact:
+    type_occ scales type t's F by occ_t."""
+    cr = X.rocksalt(n=4, seed=8)
+    a, _ = _run(ctx, cr, (-1.0,) * 3, (1.0,) * 3, (7, 7, 7))
+    b, _ = _run(ctx, cr, (-1.0,) * 3, (1.0,) * 3, (7, 7, 7), type_occ=[0.5, 0.5])
+    assert np.allclose(b["csf"], 0.5 * a["csf"], rtol=1e-12, atol=1e-12 * np.abs(a["csf"]).max())
+
+
+def test_run_nufft_fave_zeroes_bragg(ctx):
+    cr = X.rocksalt(n=6, seed=9)
+    got, (eck, vi, inc, cf) = _run(ctx, cr, (-2.0,) * 3, (2.0,) * 3, (13, 13, 13), fave=50.0)
+    bragg = np.zeros(inc, bool)
+    bragg[::3, ::3, ::3] = True
+    assert np.all(got["csf"][bragg] == 0) and np.all(got["dsi"][bragg] == 0)
+    assert np.count_nonzero(got["dsi"][~bragg]) == np.count_nonzero(~bragg)
+
+
+def test_run_nufft_errors(ctx):
+    cr = X.rocksalt(n=6, seed=10)
+    # a step 0.3 with fave: 1/0.3 is not an integer -> -187 (scales 2 is integer)
+    with pytest.raises(_lib.DCError) as e:
+        _run(ctx, cr, (-1.5,) * 3, (1.5,) * 3, (11, 11, 11), fave=50.0)
+    assert e.value.ier_num in (-186, -187)
+    # step 0.25 (0.25 * 6 cells is not an integer, 1/0.25 is): an off-centre corner -> -188
+    with pytest.raises(_lib.DCError) as e:
+        _run(ctx, cr, (-1.75, -2.0, -2.0), (2.25, 2.0, 2.0), (17, 17, 17), fave=50.0)
+    assert e.value.ier_num == -188
+    # the centred grid passes the checks
+    got, _ = _run(ctx, cr, (-2.0,) * 3, (2.0,) * 3, (17, 17, 17), fave=50.0)
+    assert got["dsi"].shape == (17, 17, 17)
+    # oblique increments are refused
+    eck, vi, inc = FR.grid_block((-1, -1, 0), (1, 1, 0), (9, 9, 1))
+    vi = vi.copy()
+    vi[1, 0] = 0.05
+    cf = FR.formtab(cr)
+    ctx.crystal_set(cr)
+    desc, keep = ctx.run_desc(eck, vi, inc, cr.rten(), cf)
+    with pytest.raises(_lib.DCError) as e:
+        ctx.four_run_nufft(desc)
+    assert e.value.ier_num == _lib.DC_ERR_ARG
