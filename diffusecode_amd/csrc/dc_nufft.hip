@@ -601,7 +601,7 @@ __global__ void nufft_finish_kernel(long long n, double sq, double ds, const dou
 double fint(double x) { return x < 0 ? std::ceil(x) : std::floor(x); }                 // Fortran INT
 long long fnint(double x) { return (long long)(x >= 0 ? std::floor(x + 0.5) : -std::floor(-x + 0.5)); }
 
-// Per-axis transform parameters of four_run_nufft (fourier_sup.f90:386-451).
+// Per-axis transform parameters of four_run_nufft (fourier_sup.f90:390-483).
 int nufft_run_params(dc_ctx *c, const dc_run_desc *r, double scales[3], int lscales[3], int ncells[3]) {
     const double EPS = 1.0e-8, EPS2 = 1.0e-4, MAXSCALE = 3.0;
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};

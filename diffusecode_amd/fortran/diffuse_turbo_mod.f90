@@ -172,10 +172,38 @@ module diffuse_turbo_mod
          type(dc_qval_desc), intent(in) :: q
          type(c_ptr), value :: csf, acsf, dsi, out
       end function dc_qval
+
+      integer(c_int) function dc_nufft1(ctx, dim, m, x, y, z, c, iflag, tol, n1, n2, n3, fk) bind(C, name='dc_nufft1')
+         import :: c_ptr, c_int, c_int64_t, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: dim, iflag
+         integer(c_int64_t), value :: m, n1, n2, n3
+         type(c_ptr), value :: x, y, z, c, fk
+         real(c_double), value :: tol
+      end function dc_nufft1
+
+      integer(c_int) function dc_four_strucf_nufft(ctx, dim, nat, xpos, ypos, zpos, occ, inc, lscales, tcsf) &
+            bind(C, name='dc_four_strucf_nufft')
+         import :: c_ptr, c_int, c_int64_t, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: dim
+         integer(c_int64_t), value :: nat
+         type(c_ptr), value :: xpos, ypos, zpos, tcsf
+         real(c_double), value :: occ
+         integer(c_int), intent(in) :: inc(3), lscales(3)
+      end function dc_four_strucf_nufft
+
+      integer(c_int) function dc_four_run_nufft(ctx, run, type_occ, csf, dsi) bind(C, name='dc_four_run_nufft')
+         import :: c_ptr, c_int, dc_run_desc
+         type(c_ptr), value :: ctx
+         type(dc_run_desc), intent(in) :: run
+         type(c_ptr), value :: type_occ, csf, dsi
+      end function dc_four_run_nufft
    end interface
 
    public :: dc_create, dc_destroy, dc_set_precision, dc_last_error_type, dc_strucf
    public :: dc_crystal_set, dc_four_run, dc_four_post, dc_four_fill, dc_qval
+   public :: dc_nufft1, dc_four_strucf_nufft, dc_four_run_nufft
    public :: dc_four_strucf, dc_shutdown
 
    type(c_ptr), save :: the_ctx = c_null_ptr

@@ -402,13 +402,13 @@ int dc_four_strucf_nufft_dev(dc_ctx *ctx, int dim, int64_t nat, const double *xp
 /* four_run_nufft (fourier_sup.f90:327-690) for 3D crystals, isotropic IAM:
  * the crystal of dc_crystal_set, per type the NUFFT of four_strucf_3d with
  * DISCUS's scaling (four_getatm_anis_iam, scales / lscales from
- * fourier_sup.f90:386-451, weight type_occ[t] = cr_occ, NULL = 1), times
+ * fourier_sup.f90:390-483, weight type_occ[t] = cr_occ, NULL = 1), times
  * cfact(istl) (tcsf_form_iso), summed over types; fave > 0 zeros the integer
  * hkl pixels (four_aver_finufft_n :1074-1190); dsi = |F|^2 * diff_scale, csf
  * = F * sqrt(diff_scale).  Increment vectors must run along a*, b*, c*.
  * Like DISCUS, the transform is taken on the modes k * scales / ncells,
  * centred on hkl = 0; eck(:,1) enters only the istl bins and the fave checks
- * (-186, -187, -188, :409-436).  The rest of the tail is dc_four_post. */
+ * (-186, -187, -188, :452-483).  The rest of the tail is dc_four_post. */
 int dc_four_run_nufft(dc_ctx *ctx, const dc_run_desc *run, const double *type_occ, double *csf, double *dsi);
 int dc_four_run_nufft_dev(dc_ctx *ctx, const dc_run_desc *run, const double *type_occ, double *d_csf, double *d_dsi);
 
