@@ -325,7 +325,9 @@ std::vector<double> correction(long long N, long long n, int w, double beta) {
 
 struct NufftState {
     std::map<std::tuple<long long, long long, long long>, hipfftHandle> plans;
-    DevBuf grid, pts[3], c, fk, corr[3];
+    std::map<std::tuple<long long, long long, int>, DevBuf> corr;   // (N, n, w) -> 1 / phihat on the device
+    std::vector<std::vector<double>> corr_host;                      // sources of the uploads, kept alive
+    DevBuf grid, pts[3], c, fk, one;
 };
 
 namespace dc {
@@ -333,8 +335,10 @@ void nufft_release(dc_ctx *c) {
     if (!c->nufft) return;
     for (auto &kv : c->nufft->plans) (void)hipfftDestroy(kv.second);
     NufftState *s = c->nufft;
-    for (DevBuf *b : {&s->grid, &s->pts[0], &s->pts[1], &s->pts[2], &s->c, &s->fk, &s->corr[0], &s->corr[1], &s->corr[2]})
+    for (DevBuf *b : {&s->grid, &s->pts[0], &s->pts[1], &s->pts[2], &s->c, &s->fk, &s->one})
         if (b->p) (void)hipFree(b->p);
+    for (auto &kv : s->corr)
+        if (kv.second.p) (void)hipFree(kv.second.p);
     delete s;
     c->nufft = nullptr;
 }
@@ -371,10 +375,17 @@ int nufft_run(dc_ctx *c, const NuParams &p, long long M, const double *dx, const
     const long long ng = p.n[0] * p.n[1] * p.n[2];
     int st = ensure(c, s.grid, sizeof(double2) * (size_t)ng);
     if (st) return st;
+    const double *corr[3];
     for (int d = 0; d < 3; ++d) {
-        std::vector<double> corr = d < p.dim ? correction(p.N[d], p.n[d], p.w, p.beta) : std::vector<double>(1, 1.0);
-        if ((st = upload(c, s.corr[d], corr.data(), corr.size()))) return st;
-        HIPCHK(hipStreamSynchronize(c->st));   // corr is a host temporary
+        auto key = std::make_tuple(d < p.dim ? p.N[d] : 1LL, d < p.dim ? p.n[d] : 1LL, d < p.dim ? p.w : 0);
+        auto it = s.corr.find(key);
+        if (it == s.corr.end()) {
+            s.corr_host.push_back(d < p.dim ? correction(p.N[d], p.n[d], p.w, p.beta) : std::vector<double>(1, 1.0));
+            DevBuf b;
+            if ((st = upload(c, b, s.corr_host.back().data(), s.corr_host.back().size()))) return st;
+            it = s.corr.emplace(key, b).first;
+        }
+        corr[d] = reinterpret_cast<const double *>(it->second.p);
     }
     double *g = ptr<double>(s.grid);
     zero_kernel<<<grid_for(2 * ng), 256, 0, c->st>>>(2 * ng, g);
@@ -421,8 +432,8 @@ int nufft_run(dc_ctx *c, const NuParams &p, long long M, const double *dx, const
         da.sub[d] = sub[d];
         da.out[d] = out[d];
     }
-    deconv_kernel<<<grid_for(out[0] * out[1] * out[2]), 256, 0, c->st>>>(da, reinterpret_cast<const double2 *>(g), ptr<double>(s.corr[0]),
-                                                                          ptr<double>(s.corr[1]), ptr<double>(s.corr[2]), dfk);
+    deconv_kernel<<<grid_for(out[0] * out[1] * out[2]), 256, 0, c->st>>>(da, reinterpret_cast<const double2 *>(g), corr[0], corr[1],
+                                                                          corr[2], dfk);
     HIPCHK(hipGetLastError());
     return 0;
 }

@@ -34,7 +34,7 @@ using namespace dc;
 
 namespace {
 
-constexpr int SUM_BLOCKS = 64;
+constexpr int SUM_BLOCKS = 512;
 constexpr int SUM_THREADS = 256;
 
 struct Field {
@@ -206,20 +206,31 @@ __global__ void lanczos_table_kernel(TabArgs t, double2 *__restrict__ out) {
     }
 }
 
-// FP64 sums of real sub-fields, fixed order: block g of sub-field b strides
-// over elements g, g + SUM_BLOCKS*T, ...; tree in LDS; part[b][g].
-__global__ __launch_bounds__(SUM_THREADS) void sum_part_kernel(Field f, const double *__restrict__ x, double *__restrict__ part) {
+// FP64 sums of real sub-fields, fixed order: block g of sub-field b takes
+// the contiguous chunk [g*chunk, (g+1)*chunk) of its elements (threads stride
+// inside it); tree in LDS; part[b][g].  Contiguous sub-fields skip the
+// index arithmetic.
+__global__ __launch_bounds__(SUM_THREADS) void sum_part_kernel(Field f, int contiguous, const double *__restrict__ x,
+                                                               double *__restrict__ part) {
     __shared__ double s[SUM_THREADS];
     const int b = blockIdx.y;
     const long long sub = f.sub();
+    const long long chunk = (sub + SUM_BLOCKS - 1) / SUM_BLOCKS;
+    const long long e0 = (long long)blockIdx.x * chunk;
+    const long long e1 = e0 + chunk < sub ? e0 + chunk : sub;
     double acc = 0.0;
-    for (long long e = (long long)blockIdx.x * SUM_THREADS + threadIdx.x; e < sub; e += (long long)SUM_BLOCKS * SUM_THREADS) {
-        long long r = e;
-        const int i = (int)(r % f.m[0]);
-        r /= f.m[0];
-        const int j = (int)(r % f.m[1]);
-        const int k = (int)(r / f.m[1]);
-        acc += x[nat_off(f, b, i, j, k)];
+    if (contiguous) {
+        const double *xb = x + (long long)b * f.bs;
+        for (long long e = e0 + threadIdx.x; e < e1; e += SUM_THREADS) acc += xb[e];
+    } else {
+        for (long long e = e0 + threadIdx.x; e < e1; e += SUM_THREADS) {
+            long long r = e;
+            const int i = (int)(r % f.m[0]);
+            r /= f.m[0];
+            const int j = (int)(r % f.m[1]);
+            const int k = (int)(r / f.m[1]);
+            acc += x[nat_off(f, b, i, j, k)];
+        }
     }
     s[threadIdx.x] = acc;
     __syncthreads();
@@ -567,7 +578,8 @@ int field_sums(dc_ctx *c, const Field &f, const double *x, double *out) {
     PostState &ps = post_state(c);
     int st = ensure(c, ps.part, sizeof(double) * SUM_BLOCKS * (size_t)f.batch);
     if (st) return st;
-    sum_part_kernel<<<dim3(SUM_BLOCKS, f.batch), SUM_THREADS, 0, c->st>>>(f, x, ptr<double>(ps.part));
+    const int contiguous = f.s[0] == 1 && f.s[1] == f.m[0] && (f.m[2] == 1 || f.s[2] == (long long)f.m[0] * f.m[1]);
+    sum_part_kernel<<<dim3(SUM_BLOCKS, f.batch), SUM_THREADS, 0, c->st>>>(f, contiguous, x, ptr<double>(ps.part));
     sum_finish_kernel<<<(f.batch + 63) / 64, 64, 0, c->st>>>(f.batch, ptr<double>(ps.part), out);
     HIPCHK(hipGetLastError());
     return 0;
