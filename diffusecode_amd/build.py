@@ -66,6 +66,8 @@ FORTRAN_SRC = os.path.join(HERE, "fortran", "diffuse_turbo_mod.f90")
 
 FORTRAN_EXAMPLE = os.path.join(HERE, "fortran", "example_strucf.f90")
 FORTRAN_EXE = os.path.join(OBJ, "fortran", "example_strucf")
+FORTRAN_EXAMPLE_POST = os.path.join(HERE, "fortran", "example_post.f90")
+FORTRAN_EXE_POST = os.path.join(OBJ, "fortran", "example_post")
 
 
 def build_fortran() -> str | None:
@@ -78,8 +80,9 @@ def build_fortran() -> str | None:
     os.makedirs(out, exist_ok=True)
     obj = os.path.join(out, "diffuse_turbo_mod.o")
     _run([fc, "-c", "-O2", "-fPIC", "-module-dir", out, FORTRAN_SRC, "-o", obj])
-    _run([fc, "-O2", "-module-dir", out, "-I", out, FORTRAN_EXAMPLE, obj, "-o", FORTRAN_EXE,
-          "-L", HERE, "-ldiffuse_turbo", "-Wl,-rpath,$ORIGIN/../..", "-Wl,-rpath," + HERE])
+    for src, exe in ((FORTRAN_EXAMPLE, FORTRAN_EXE), (FORTRAN_EXAMPLE_POST, FORTRAN_EXE_POST)):
+        _run([fc, "-O2", "-module-dir", out, "-I", out, src, obj, "-o", exe,
+              "-L", HERE, "-ldiffuse_turbo", "-Wl,-rpath,$ORIGIN/../..", "-Wl,-rpath," + HERE])
     return FORTRAN_EXE
 
 

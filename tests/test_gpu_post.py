@@ -268,3 +268,17 @@ def test_end_to_end_run_then_post(ctx):
     r3 = P.four_filter(r3, num, 1.0, 0.5, 4)
     # the run carries the FP32 path's error (<= 1e-6 of max I; F to ~1e-6 of max |F|)
     assert rel(s, rs) <= 2e-6 and rel(d3, r3) <= 2e-6 and rel(c, rc) <= 1e-5
+
+
+def test_fortran_host_post_and_nufft():
+    """A Fortran host fills the bind(C) descriptors of diffuse_turbo_mod and
+    calls dc_nufft1, dc_four_post, dc_four_fill and dc_qval (closed forms)."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "diffusecode_amd", "_build", "fortran", "example_post")
+    assert os.path.exists(exe), "build the Fortran examples first (__graft_entry__.build)"
+    print("case fortran post/nufft", flush=True)
+    p = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(p.stdout, p.stderr, flush=True)
+    assert p.returncode == 0 and "PASS" in p.stdout
