@@ -206,6 +206,33 @@ __global__ void lanczos_table_kernel(TabArgs t, double2 *__restrict__ out) {
     }
 }
 
+// Separable circular stencil along one axis of a (batched) field:
+// dst(p) = sum_u src(p + shift - r_u) v_u, indices mod m along the axis.
+constexpr int MAXTAP = 64;
+struct TapArgs {
+    Field f;
+    int axis, ntap, shift;
+    int r[MAXTAP];
+    double v[MAXTAP];
+};
+
+__global__ void axis_conv_kernel(TapArgs t, const double *__restrict__ src, double *__restrict__ dst) {
+    const long long n = t.f.total();
+    const int m = t.f.m[t.axis];
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+        int b, c3[3];
+        unflat(t.f, e, b, c3[0], c3[1], c3[2]);
+        const int q = c3[t.axis];
+        double acc = 0.0;
+        for (int u = 0; u < t.ntap; ++u) {
+            int cc[3] = {c3[0], c3[1], c3[2]};
+            cc[t.axis] = pmod(q + t.shift - t.r[u], m);
+            acc += src[nat_off(t.f, b, cc[0], cc[1], cc[2])] * t.v[u];
+        }
+        dst[nat_off(t.f, b, c3[0], c3[1], c3[2])] = acc;
+    }
+}
+
 // FP64 sums of real sub-fields, fixed order: block g of sub-field b takes
 // the contiguous chunk [g*chunk, (g+1)*chunk) of its elements (threads stride
 // inside it); tree in LDS; part[b][g].  Contiguous sub-fields skip the
@@ -693,8 +720,69 @@ int run_conv(dc_ctx *c, const dc_post_desc *d, double *csf, double *dsi, double 
 }
 
 // do_lanczos_*_fft over a batch of equal sub-fields (lib_lanczos.f90:258-458)
+// Per-axis factor of the Lanczos table of do_lanczos_*_fft as circular taps
+// at FFT-index offsets.  The 2D/3D table wsinc * lqgm is a product over the
+// axes of sinc(pi qg rcut) sinc(pi qg / m) [|qg| < m] (wsinc_1d squares its
+// sinc), each clipped to the natural range 1..N like the table.  A tap at
+// FFT index r of the table convolves as the natural offset r (the maps are
+// shifts); the 1D reverse map adds shift = lf - lr.  false: more taps than
+// MAXTAP (the transform path is used).
+bool lanczos_taps(const Field &f, int nd, double nscale, double rcut, int m, int axis, TapArgs &t) {
+    const int N = f.m[axis];
+    const int lf = (N - 1) % 2, lr = nd == 1 ? 0 : lf;
+    const int cen = (N + 1) / 2;
+    const int lo = (int)std::lround(-nscale * m - 1.0) + cen, hi = (int)std::lround(nscale * m + 1.0) + cen;
+    const double pi = 3.14159265358979323846264338328;
+    auto sinc = [](double v) { return std::fabs(v) < 1.0e-7 ? 1.0 : std::sin(v) / v; };
+    t.f = f;
+    t.axis = axis;
+    t.shift = lf - lr;
+    t.ntap = 0;
+    for (int i = std::max(1, lo); i <= std::min(N, hi); ++i) {
+        const double qg = (double)cen / nscale - (double)i / nscale;
+        if (!(std::fabs(qg) < (double)m)) continue;
+        double w = sinc(pi * qg * rcut);
+        if (nd == 1) w = w * w;
+        const double v = w * sinc(pi * qg / (double)m);
+        if (v == 0.0) continue;
+        if (t.ntap == MAXTAP) return false;
+        t.r[t.ntap] = ((i + N / 2 - lf) % N + N) % N;
+        t.v[t.ntap] = v;
+        ++t.ntap;
+    }
+    return true;
+}
+
 int lanczos_batch(dc_ctx *c, const Field &f, int nd, double nscale, double rcut, int m, double *x) {
     PostState &ps = post_state(c);
+    {
+        // separable direct form: one short circular stencil per axis
+        TapArgs taps[3];
+        bool ok = true;
+        for (int a = 0; a < nd; ++a) ok = ok && lanczos_taps(f, nd, nscale, rcut, m, a, taps[a]);
+        if (ok) {
+            int st = ensure(c, ps.sums, sizeof(double) * 2 * (size_t)f.batch);
+            if (st) return st;
+            const long long span = (long long)(f.batch - 1) * f.bs + (long long)(f.m[0] - 1) * f.s[0] +
+                                   (long long)(f.m[1] - 1) * f.s[1] + (long long)(f.m[2] - 1) * f.s[2] + 1;
+            if ((st = ensure(c, ps.copy, sizeof(double) * (size_t)span))) return st;
+            double *sums = ptr<double>(ps.sums);
+            if ((st = field_sums(c, f, x, sums))) return st;                    // sum(infield)
+            double *bufs[2] = {x, ptr<double>(ps.copy)};
+            int cur = 0;
+            const int gr = grid_for(f.total());
+            for (int a = 0; a < nd; ++a) {
+                axis_conv_kernel<<<gr, 256, 0, c->st>>>(taps[a], bufs[cur], bufs[1 - cur]);
+                cur = 1 - cur;
+            }
+            HIPCHK(hipGetLastError());
+            if (cur == 1) HIPCHK(hipMemcpyAsync(x, ps.copy.p, sizeof(double) * (size_t)span, hipMemcpyDeviceToDevice, c->st));
+            if ((st = field_sums(c, f, x, sums + f.batch))) return st;         // sum(outfield)
+            rescale_kernel<false, true><<<gr, 256, 0, c->st>>>(f, sums, sums + f.batch, nscale, x);
+            HIPCHK(hipGetLastError());
+            return 0;
+        }
+    }
     int st = ensure(c, ps.spec, sizeof(double2) * (size_t)f.sub());
     if (st) return st;
     if ((st = ensure(c, ps.sums, sizeof(double) * 2 * (size_t)f.batch))) return st;
