@@ -89,6 +89,19 @@ class Fourier:
         self.ltop_c = False
         self.ltop_n = False
         self.rng = np.random.default_rng(seed)
+        # 'set technique/filter/scale/damp/width/symm/mode' (fourier.f90:1090-1200, fourier_reset.f90:100-105)
+        self.four_tech = "turbo"
+        self.four_filter = False
+        self.four_rscale = 1.0
+        self.four_damp = 0.5
+        self.four_width = 4
+        self.four_symm = False
+        self.symm_ops = None            # point-group operations (identity first) for 'symm apply'
+        self.four_accum = "single"
+        self.res_sigma = [0.0, 0.0, 0.0]   # diff_res(1, 1:3)
+        self.res_tr = np.zeros((3, 3))     # diff_tr
+        self.type_occ = None               # cr_occ per type, the NUFFT weight
+        self.want_dsi3d = False            # four_weight's dsi3d (for 3D-PDF output)
         self.ctx = ctx
         self.precision = precision
         self.crystal = None
@@ -195,6 +208,55 @@ class Fourier:
     def neutron(self):
         self.lxray = False
 
+    def set_options(self, technique=None, filter=None, scale=None, damp=None, width=None, symm=None, mode=None):
+        """Optional parameters of 'set' (fourier.f90:1090-1200): technique
+        turbo|nufft, filter lanczos|off, scale (four_rscale with a filter,
+        else diff_scale), damp, width, symm apply|friedel|ignore, mode
+        single|init|add|finished."""
+        if symm is not None:
+            m = symm.lower()
+            if m not in ("apply", "friedel", "ignore"):
+                raise FourierError(-6, ER_COMM, "Symmetry paramter must be 'apply', 'friedel' or 'ignore'")
+            self.four_symm = m == "apply"
+            self.four_friedel = m != "ignore"
+        if technique is not None:
+            if technique.lower() not in ("turbo", "nufft"):
+                raise FourierError(-6, ER_COMM, "Fourier technique must be 'turbo' or 'nufft'")
+            self.four_tech = technique.lower()
+        if filter is not None:
+            self.four_filter = filter.lower() == "lanczos"
+        if scale is not None:
+            if filter is not None and filter.lower() == "lanczos":
+                self.four_rscale = float(scale)
+            elif filter is None:
+                self.diff_scale = float(scale)
+        if damp is not None:
+            self.four_damp = float(damp)
+        if width is not None:
+            self.four_width = int(round(width))
+        if mode is not None:
+            m = mode.lower()
+            if m not in ("single", "init", "add", "finished"):
+                raise FourierError(-6, ER_COMM, "mode must be single, init, add or finished")
+            self.four_accum = m
+
+    def resolution(self, sigabs=0.0, sigord=0.0, sigtop=0.0, tr=None):
+        """Resolution of four_conv ('run' options sigabs/sigord/sigtop, four_resolution,
+        fourier_conv.f90:426-650) with the default resolution vectors (the
+        increment vectors): simat's columns are then vi(:,k) sigma_k / |vi(:,k)|,
+        so diff_tr = simat^-1 vi = diag(|vi(:,k)| / sigma_k) on the non-flat axes.
+        tr: a full diff_tr (3x3) for other resolution vectors."""
+        self.res_sigma = [float(sigabs), float(sigord), float(sigtop)]
+        if tr is not None:
+            self.res_tr = np.asarray(tr, float).reshape(3, 3)
+            return
+        g = self.crystal.rten() if self.crystal is not None else np.eye(3)
+        self.res_tr = np.zeros((3, 3))
+        for k in range(3):
+            if self.inc[k] > 1 and self.res_sigma[k] > 0.0:
+                v = self.vi[:, k]
+                self.res_tr[k, k] = np.sqrt(v @ g @ v) / self.res_sigma[k]
+
     def set_precision(self, precision):
         self.precision = precision
         if self.ctx is not None:
@@ -240,6 +302,8 @@ class Fourier:
         self._prepare()
         c = self.crystal
         cfact = formtab(c, lxray=self.lxray, ldbw=self.ldbw, ano=self.ano)
+        if self.four_tech == "nufft" and device_out is None:
+            return self._run_nufft(cfact)
         if self.ilots != 1 and lbeg is None:
             lbeg = self.lot_origins()
         if self.fave != 0.0 and sel_cell is None:
@@ -261,7 +325,51 @@ class Fourier:
         self.csf, self.acsf, self.dsi = out["csf"], out["acsf"], out["dsi"]
         if want_var:
             self.fmean, self.dvar = out["fmean"], out["dvar"]
+        self._tail(out)
         return out
+
+    def _run_nufft(self, cfact):
+        """four_run_nufft (fourier_sup.f90:327-690) up to dsi, then the tail."""
+        c = self.crystal
+        desc, keep = _lib.Context.run_desc(self.eck, self.vi, self.inc, c.rten(), cfact, fave=self.fave,
+                                           diff_scale=self.diff_scale)
+        try:
+            out = self.ctx.four_run_nufft(desc, self.type_occ)
+        except _lib.DCError as e:
+            raise FourierError(e.ier_num, e.ier_typ, str(e)) from None
+        out["status"] = 0
+        out["acsf"] = np.zeros_like(out["csf"])
+        self.status = 0
+        self.csf, self.acsf, self.dsi = out["csf"], out["acsf"], out["dsi"]
+        self._tail(out)
+        return out
+
+    def _tail(self, out):
+        """four_weight, filter, four_conv, filter of dsi3d (fourier_sup.f90:279-290)
+        and, in 'single' mode with 'symm apply', the point-group average
+        (four_accumulate :3784-3840)."""
+        c = self.crystal
+        conv = any(sv >= 1.0e-6 for sv in self.res_sigma)
+        self.dsi3d = None
+        if self.four_filter or conv or self.want_dsi3d:
+            vr = float(np.sqrt(np.linalg.det(c.rten())))
+            pd = _lib.Context.post_desc(self.inc, self.eck, self.vi, c.rten(), vr, fave=self.fave, nlots=self.nlots,
+                                        icc=c.icc, ls=self.ls_xyz, lots_off=self.ilots == 1,
+                                        diff_scale=self.diff_scale, res_sigma=self.res_sigma,
+                                        res_tr=self.res_tr.ravel(order="F"), lanczos=self.four_filter,
+                                        rscale=self.four_rscale, damp=self.four_damp, width=self.four_width)
+            try:
+                csf, dsi, d3 = self.ctx.four_post(pd, out["csf"], out["dsi"], want_dsi3d=self.want_dsi3d)
+            except _lib.DCError as e:
+                raise FourierError(e.ier_num, e.ier_typ, str(e)) from None
+            out["csf"], out["dsi"], out["dsi3d"] = csf, dsi, d3
+            self.csf, self.dsi, self.dsi3d = csf, dsi, d3
+        if self.four_accum == "single" and self.four_symm and self.symm_ops is not None:
+            sd, keep = _lib.Context.symm_desc(self.inc, self.eck, self.vi, fave=self.fave, four_symm=True,
+                                              ops=self.symm_ops)
+            csf, dsi = self.ctx.four_fill(sd, out["csf"], out["dsi"])
+            out["csf"], out["dsi"] = csf, dsi
+            self.csf, self.dsi = csf, dsi
 
     def calc(self, h, k, l):
         """Single reflection (fourier.f90:243-294 -> calc_000): a 1-point grid."""

@@ -282,3 +282,37 @@ def test_fortran_host_post_and_nufft():
     p = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     print(p.stdout, p.stderr, flush=True)
     assert p.returncode == 0 and "PASS" in p.stdout
+
+
+def test_fourier_menu_filter_resolution_and_nufft(ctx):
+    """The fourier menu mirror: 'set technique,nufft', 'set filter,lanczos',
+    a resolution, and 'run' against the oracle run followed by the oracle tail."""
+    from diffusecode_amd.fourier import Fourier
+    cr = X.rocksalt(n=4, seed=12)
+    f = Fourier(cr, ctx=ctx)
+    f.ll(-2.0, -2.0, 0.0)
+    f.lr(2.0, -2.0, 0.0)
+    f.ul(-2.0, 2.0, 0.0)
+    f.nabs(25)
+    f.nord(25)
+    f.set_symm("ignore")
+    f.set_options(filter="lanczos", damp=0.5, width=4)
+    f.resolution(0.02, 0.03)
+    out = f.run()
+    eck, vi, inc = f.eck, f.vi, f.inc
+    cf = FR.formtab(cr)
+    ref = O.four_run(cr, eck, vi, inc, cf, four_friedel=False)
+    rs = P.four_filter(ref["dsi"], inc, 1.0, 0.5, 4)
+    rc, rs, _ = P.four_conv(inc, f.res_sigma, f.res_tr.ravel(order="F"), ref["csf"], rs, lots_off=True)
+    assert rel(out["dsi"], rs) <= 2e-6
+    # the NUFFT technique on a block grid of the same crystal (step 1/4 = scales 1 for 4 cells)
+    g = Fourier(cr, ctx=ctx)
+    g.ll(-2.0, -2.0, 0.0)
+    g.lr(2.0, -2.0, 0.0)
+    g.ul(-2.0, 2.0, 0.0)
+    g.nabs(17)
+    g.nord(17)
+    g.set_options(technique="nufft")
+    out = g.run()
+    ref = O.four_run(cr, g.eck, g.vi, g.inc, cf, four_friedel=False)
+    assert rel(out["dsi"], ref["dsi"]) <= 1e-6
