@@ -319,6 +319,11 @@ def main():
     t32, g32, _ = run_steps("fp32", fp32_steps, 1)
     v32 = pairs / t32
     log(f"fp32: {t32 * 1e3:.2f} ms/step, {v32 / 1e12:.2f} Tpairs/s, gemm {g32:.2f} ms/launch")
+    # the third scheme, FP16x2 (A split, B single FP16), on the same workload
+    t16, g16, _ = run_steps("fp16x2", fp32_steps, 1)
+    v16 = pairs / t16
+    dsi_16 = d_dsi.view(inc[2], inc[1], inc[0]).cpu().numpy()
+    log(f"fp16x2: {t16 * 1e3:.2f} ms/step, {v16 / 1e12:.2f} Tpairs/s, gemm {g16:.2f} ms/launch")
 
     # dominant kernel: the phase GEMM; algorithmic MFMA flops = 8 x k_split per
     # computed atom.hkl pair (the GEMM computes the Friedel-reduced region)
@@ -331,12 +336,17 @@ def main():
                 "flops_per_launch": 8.0 * KSPLIT[args.prec] * pairs_computed_per_launch,
                 "fp32_kernel": {"avg_launch_ms": round(g32, 3),
                                 "achieved": round(8.0 * pairs_computed_per_launch / (g32 * 1e-3) / 1e12, 2),
-                                "peak": PEAK["fp32"]}}
+                                "peak": PEAK["fp32"]},
+                "fp16x2_kernel": {"avg_launch_ms": round(g16, 3),
+                                  "achieved": round(16.0 * pairs_computed_per_launch / (g16 * 1e-3) / 1e12, 2),
+                                  "peak": PEAK["fp16x2"]}}
 
     # bounded sample of the GPU output for the check in the CPU leg: two
     # l-layers (l = 0 and l = 1) subsampled every 4th point (h, k step 0.2)
     head_samples = {0.0: dsi_split[80, ::4, ::4].T.copy(), 1.0: dsi_split[100, ::4, ::4].T.copy()}
+    fp16_samples = {0.0: dsi_16[80, ::4, ::4].T.copy(), 1.0: dsi_16[100, ::4, ::4].T.copy()}
     max_err = None
+    err16 = None
     ctx.close()
 
     extras = {}
@@ -375,6 +385,8 @@ def main():
                 return float(max(e for e, _ in errs) / max(m for _, m in errs))
             max_err = sampled_err(head_samples)
             log(f"sampled max|dI|/max(I) vs FP64 oracle: {max_err:.3e}")
+            err16 = sampled_err(fp16_samples)
+            log(f"fp16x2 sampled max|dI|/max(I): {err16:.3e}")
             if nufft_samples is not None:
                 extras["nufft"]["max_rel_err_sampled"] = sampled_err(nufft_samples)
                 log(f"nufft sampled max|dI|/max(I): {extras['nufft']['max_rel_err_sampled']:.3e}")
@@ -405,6 +417,7 @@ def main():
                                "3D block h,k,l in [-4,4] at 161^3 points, lots off, Friedel on",
                    "atoms": int(c.natoms), "grid": inc, "pairs_per_step": pairs, "precision": args.prec},
         "fp32_value": v32, "fp32_ms_per_step": t32 * 1e3, "speedup_vs_fp32": value / v32,
+        "fp16x2_value": v16, "fp16x2_ms_per_step": t16 * 1e3, "fp16x2_max_rel_err_sampled": err16,
         "max_rel_err_sampled": max_err,
         "roofline": roofline, "cpu_baseline": cpu,
     }
