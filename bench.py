@@ -93,6 +93,9 @@ def calibrate_copy(nbytes):
     log(f"copied {nbytes} bytes x 3")
 
 
+LOTS_CHECK = {}   # sampled output and inputs of the lots run, checked in the CPU leg
+
+
 def bench_lots(_lib, X, FR, torch, prec):
     c = X.fcc_correlated(n=100, a=4.0, subst=0.10, sigma=0.08, seed=3)
     eck, vi, inc = FR.grid_block((-10.0, -10.0, 0.0), (10.0, 10.0, 3.5), (401, 401, 8))
@@ -113,6 +116,9 @@ def bench_lots(_lib, X, FR, torch, prec):
         if it:
             stats.append(ctx.stats())
     ctx.close()
+    # bounded sample for the check in the CPU leg: layer l = 0, every 10th point (h, k step 0.5)
+    dsi = bufs["dsi"].cpu().numpy().reshape(inc, order="F")
+    LOTS_CHECK.update(sample=dsi[::10, ::10, 0].copy(), crystal=c, lbeg=lbeg, ls=ls)
     ms = float(np.median([s["total_ms"] for s in stats]))
     gemm = float(np.median([s["gemm_ms"] for s in stats]))
     red = float(np.median([s["reduce_ms"] for s in stats]))
@@ -387,6 +393,13 @@ def main():
             log(f"sampled max|dI|/max(I) vs FP64 oracle: {max_err:.3e}")
             err16 = sampled_err(fp16_samples)
             log(f"fp16x2 sampled max|dI|/max(I): {err16:.3e}")
+            if LOTS_CHECK and "lots" in extras:
+                cl = LOTS_CHECK["crystal"]
+                e3, v3, i3 = FR.grid_block((-10.0, -10.0, 0.0), (10.0, 10.0, 0.0), (41, 41, 1))
+                ref = O.four_run(cl, e3, v3, i3, FR.formtab(cl), ilots=O.LOT_BOX, ls=LOTS_CHECK["ls"],
+                                 lbeg=LOTS_CHECK["lbeg"], lperiod=True)["dsi"][:, :, 0]
+                extras["lots"]["max_rel_err_sampled"] = float(np.abs(LOTS_CHECK["sample"] - ref).max() / ref.max())
+                log(f"lots sampled max|dI|/max(I): {extras['lots']['max_rel_err_sampled']:.3e}")
             if nufft_samples is not None:
                 extras["nufft"]["max_rel_err_sampled"] = sampled_err(nufft_samples)
                 log(f"nufft sampled max|dI|/max(I): {extras['nufft']['max_rel_err_sampled']:.3e}")
