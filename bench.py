@@ -157,7 +157,7 @@ def bench_adjoint(_lib, X, FR):
         fwd = float(np.median([s["forward_ms"] for s in rows]))
         adj = float(np.median([s["adjoint_ms"] for s in rows]))
         out[prec] = {"forward_ms": fwd, "adjoint_ms": adj, "adjoint_over_forward": adj / fwd,
-                     "pairs_per_forward": rows[0]["pairs"]}
+                     "pairs_per_forward": rows[0]["pairs"], "adjoint_walk": int(rows[0]["adjoint_walk"])}
     ctx.close()
     return out
 

@@ -33,7 +33,7 @@ CFPKT = 9999
 
 EXPORTS = [
     "dc_create", "dc_destroy", "dc_last_error", "dc_last_error_type", "dc_set_precision",
-    "dc_get_precision", "dc_set_gemm_tile", "dc_gemm_num_tiles", "dc_gemm_tile_shape", "dc_set_cancel_flag", "dc_version", "dc_device_info", "dc_strucf",
+    "dc_get_precision", "dc_set_gemm_tile", "dc_set_adjoint_walk", "dc_gemm_num_tiles", "dc_gemm_tile_shape", "dc_set_cancel_flag", "dc_version", "dc_device_info", "dc_strucf",
     "dc_crystal_set", "dc_four_run", "dc_four_run_dev", "dc_last_stats", "dc_synchronize",
     "dc_model_intensity", "dc_model_vjp", "dc_model_chi2", "dc_model_jvp",
     "dc_batch_chi2", "dc_set_autotune", "dc_set_graphs", "dc_batch_info",
@@ -66,7 +66,7 @@ class RunDesc(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("pairs", C.c_double), ("gemm_ms", C.c_double), ("total_ms", C.c_double),
                 ("reduce_ms", C.c_double), ("gemm_launches", C.c_int64), ("slices", C.c_int64),
-                ("forward_ms", C.c_double), ("adjoint_ms", C.c_double)]
+                ("forward_ms", C.c_double), ("adjoint_ms", C.c_double), ("adjoint_walk", C.c_int64)]
 
 
 class ModelDesc(C.Structure):
@@ -121,6 +121,7 @@ def lib():
     L.dc_set_precision.argtypes = [C.c_void_p, C.c_int]
     L.dc_get_precision.argtypes = [C.c_void_p]
     L.dc_set_gemm_tile.argtypes = [C.c_void_p, C.c_int]
+    L.dc_set_adjoint_walk.argtypes = [C.c_void_p, C.c_int]
     L.dc_gemm_tile_shape.argtypes = [C.c_int, _ip, _ip, _ip]
     L.dc_set_cancel_flag.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
     L.dc_version.restype = C.c_char_p
@@ -236,6 +237,11 @@ class Context:
 
     def set_gemm_tile(self, tile):
         self.check(lib().dc_set_gemm_tile(self.h, int(tile)))
+
+    def set_adjoint_walk(self, walk):
+        """Column tiles per work-group of the split reverse pass on short
+        grids: 0 = planned, 1..32 forced."""
+        self.check(lib().dc_set_adjoint_walk(self.h, int(walk)))
 
     @staticmethod
     def gemm_tiles():

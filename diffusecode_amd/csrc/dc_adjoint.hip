@@ -29,6 +29,7 @@
 // registers one stage ahead.  Products hi.hi + hi.lo + lo.hi, FP32
 // accumulation, the same epilogue.
 #include <algorithm>
+#include <atomic>
 
 #include "dc_internal.h"
 
@@ -63,11 +64,13 @@ struct ColTables {
 };
 
 // Row factors, exact two-level: row 16 s + rr of atom n has phase
-// 16 s ra + rr ra, so A = RH[s][n] * RL[n][rr] (grids of up to MAXST stages).
-struct RowTables {
-    float2 rh[MAXST][NA];
+// 16 s ra + rr ra, so A = RH[s][n] * RL[n][rr] (grids of up to S stages).
+template <int S>
+struct RowTablesT {
+    float2 rh[S][NA];
     float2 rl[NA][16];
 };
+using RowTables = RowTablesT<MAXST>;
 
 struct AdSmem {
     float a[2][KP][NA];       // A(k', atom), k' = 2 rr + (0 re, 1 im)
@@ -85,7 +88,8 @@ __device__ __forceinline__ float2 cis(uint64_t ph) {
     return make_float2(c, s);
 }
 
-__device__ __forceinline__ void build_row_tables(RowTables &rt, const AtomRec *at, int nst) {
+template <class RT>
+__device__ __forceinline__ void build_row_tables(RT &rt, const AtomRec *at, int nst) {
     for (int x = threadIdx.x; x < NA * (16 + nst); x += NT) {
         const int n = x % NA, e = x / NA;
         if (e < 16)
@@ -96,8 +100,9 @@ __device__ __forceinline__ void build_row_tables(RowTables &rt, const AtomRec *a
 }
 
 // A element of atom n at row r = 16 s + rr (r < R): row phase and exp(-B s_r^2)
-__device__ __forceinline__ float2 row_element(const RowTables &rt, const AtomRec &ar, int n, int s, int rr, int r,
-                                              bool tab, float s2) {
+template <class RT>
+__device__ __forceinline__ float2 row_element(const RT &rt, const AtomRec &ar, int n, int s, int rr, int r, bool tab,
+                                              float s2) {
     float2 v;
     if (tab) {
         const float2 h = rt.rh[s][n], l = rt.rl[n][rr];
@@ -135,12 +140,10 @@ __device__ __forceinline__ void load_stage(AdSmem &sm, int buf, const AdjArgs &a
 // Epilogue shared by the FP32 and split kernels: lane holds atoms 4 lg + v of
 // column li of every 16x16 tile; contract with the generated column factor
 // and weights, reduce over columns in a fixed order, write the partials.
-__device__ __forceinline__ void adjoint_epilogue(const AdjArgs &a, const float *__restrict__ sq,
-                                                 const AtomRec *at, float (&red)[NWC][NA][6], ColTables &ct,
-                                                 f32x4 (&accR)[3][WA][WC], f32x4 (&accI)[3][WA][WC], int col0,
-                                                 int atom0, int na) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
-    const int wa0 = (wave % NWA) * 16 * WA, wc0 = (wave / NWA) * 16 * WC;
+// Three steps: column tables of the tile at col0 (epi_tables), contraction
+// into per-lane column sums (epi_contract, adds to acc), reduction over lanes
+// and waves into partial set `set` (epi_reduce).
+__device__ __forceinline__ void epi_tables(const AdjArgs &a, const AtomRec *at, ColTables &ct, int col0) {
     // two-level column tables of the work-group's atoms (caller synchronised the LDS)
     constexpr int NH = NC / 16, NE = NH + 32;
     for (int x = threadIdx.x; x < NA * NE; x += NT) {
@@ -156,14 +159,13 @@ __device__ __forceinline__ void adjoint_epilogue(const AdjArgs &a, const float *
             ct.tl[n][d][l] = cis((uint64_t)l * ar.ca + (d ? w : 0ull));
         }
     }
-    __syncthreads();
-    float acc[WA][4][6];
-#pragma unroll
-    for (int i = 0; i < WA; ++i)
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-#pragma unroll
-            for (int k = 0; k < 6; ++k) acc[i][v][k] = 0.0f;
+}
+
+__device__ __forceinline__ void epi_contract(const AdjArgs &a, const float *__restrict__ sq, const AtomRec *at,
+                                             const ColTables &ct, const f32x4 (&accR)[3][WA][WC],
+                                             const f32x4 (&accI)[3][WA][WC], int col0, float (&acc)[WA][4][6]) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
+    const int wa0 = (wave % NWA) * 16 * WA, wc0 = (wave / NWA) * 16 * WC;
 #pragma unroll
     for (int j = 0; j < WC; ++j) {
         const int col = col0 + wc0 + 16 * j + li;
@@ -195,6 +197,25 @@ __device__ __forceinline__ void adjoint_epilogue(const AdjArgs &a, const float *
                 acc[i][v][5] += u2 + sc * tr;
             }
     }
+}
+
+// Sum of the per-wave partials (red, synchronised by the caller) in wave
+// order, written as partial set `set`.
+__device__ __forceinline__ void epi_write(const AdjArgs &a, const float (&red)[NWC][NA][6], int set, int atom0,
+                                          int na) {
+    float *out = a.part + ((long long)set * a.natoms + atom0) * 6;
+    for (int x = threadIdx.x; x < na * 6; x += NT) {
+        float v = 0.0f;
+#pragma unroll
+        for (int w = 0; w < NWC; ++w) v += (&red[w][0][0])[x];
+        out[x] = v;
+    }
+}
+
+__device__ __forceinline__ void epi_reduce(const AdjArgs &a, float (&red)[NWC][NA][6], float (&acc)[WA][4][6],
+                                           int set, int atom0, int na) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
+    const int wa0 = (wave % NWA) * 16 * WA;
     // sum over the 16 columns of a lane group, then over the waves of a row of tiles
 #pragma unroll
     for (int i = 0; i < WA; ++i)
@@ -210,13 +231,66 @@ __device__ __forceinline__ void adjoint_epilogue(const AdjArgs &a, const float *
                 if (li == 0) red[wave / NWA][wa0 + 16 * i + 4 * lg + v][k] = x;
             }
     __syncthreads();
-    float *out = a.part + ((long long)blockIdx.x * a.natoms + atom0) * 6;
-    for (int x = threadIdx.x; x < na * 6; x += NT) {
-        float v = 0.0f;
+    epi_write(a, red, set, atom0, na);
+}
+
+// Row-local lane exchange (data-parallel primitives): lane li of each 16-lane
+// row reads lane src(li) of the same row.
+template <int CTRL>
+__device__ __forceinline__ float dpp(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xF, 0xF, false));
+}
+constexpr int DPP_ROW_MIRROR = 0x140;        // src = 15 - li
+constexpr int DPP_ROW_HALF_MIRROR = 0x141;   // src = (li & 8) | (7 - (li & 7))
+constexpr int DPP_QUAD_XOR2 = 0x4E;          // quad_perm [2,3,0,1]: src = li ^ 2
+constexpr int DPP_QUAD_XOR1 = 0xB1;          // quad_perm [1,0,3,2]: src = li ^ 1
+
+// Sums the 48 per-lane column sums of a 16-lane row over its lanes and
+// scatters the totals: four halving exchanges (partners 15 - li, the mirror
+// inside the half-row, li ^ 2, li ^ 1; bit 8, 4, 2, 1 of li picks the half a
+// lane keeps) leave lane li with the row totals of flat entries 3 li, 3 li + 1,
+// 3 li + 2 of acc (flat = (4 i + v) 6 + k).  Fixed order, so deterministic.
+__device__ __forceinline__ void row_reduce_scatter(const float (&acc)[WA][4][6], float (&out)[3], int li) {
+    static_assert(WA * 4 * 6 == 48, "48 column sums per lane");
+    const float *x = &acc[0][0][0];
+    float y[24], z[12], w[6];
 #pragma unroll
-        for (int w = 0; w < NWC; ++w) v += (&red[w][0][0])[x];
-        out[x] = v;
+    for (int j = 0; j < 24; ++j) {
+        const bool h = (li & 8) != 0;
+        y[j] = (h ? x[24 + j] : x[j]) + dpp<DPP_ROW_MIRROR>(h ? x[j] : x[24 + j]);
     }
+#pragma unroll
+    for (int j = 0; j < 12; ++j) {
+        const bool h = (li & 4) != 0;
+        z[j] = (h ? y[12 + j] : y[j]) + dpp<DPP_ROW_HALF_MIRROR>(h ? y[j] : y[12 + j]);
+    }
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        const bool h = (li & 2) != 0;
+        w[j] = (h ? z[6 + j] : z[j]) + dpp<DPP_QUAD_XOR2>(h ? z[j] : z[6 + j]);
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const bool h = (li & 1) != 0;
+        out[j] = (h ? w[3 + j] : w[j]) + dpp<DPP_QUAD_XOR1>(h ? w[j] : w[3 + j]);
+    }
+}
+
+__device__ __forceinline__ void adjoint_epilogue(const AdjArgs &a, const float *__restrict__ sq,
+                                                 const AtomRec *at, float (&red)[NWC][NA][6], ColTables &ct,
+                                                 f32x4 (&accR)[3][WA][WC], f32x4 (&accI)[3][WA][WC], int col0,
+                                                 int atom0, int na) {
+    epi_tables(a, at, ct, col0);
+    __syncthreads();
+    float acc[WA][4][6];
+#pragma unroll
+    for (int i = 0; i < WA; ++i)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) acc[i][v][k] = 0.0f;
+    epi_contract(a, sq, at, ct, accR, accI, col0, acc);
+    epi_reduce(a, red, acc, blockIdx.x, atom0, na);
 }
 
 __global__ void __launch_bounds__(NT) adjoint_kernel(AdjArgs a) {
@@ -304,8 +378,11 @@ __device__ __forceinline__ void bsplit(float x, float y, uint32_t &hi, uint32_t 
 }
 __device__ __forceinline__ uint32_t rot16(uint32_t w) { return (w >> 16) | (w << 16); }
 
+// One stage of split A in LDS: [form (re,-im) | (im,re)][weight][hi | lo][row group][atom]
+typedef u32x4 StageA[2][3][2][4][NA];
+
 struct AdSmemS {
-    u32x4 a[2][2][3][2][4][NA];   // [buffer][form (re,-im) | (im,re)][weight][hi | lo][row group][atom]
+    StageA a[2];                  // double-buffered stages
     AtomRec at[NA];
     float red[NWC][NA][6];
     ColTables ct;
@@ -314,8 +391,9 @@ struct AdSmemS {
 
 // A of stage st (rows 16 st .. 16 st + 15): thread (atom, row group g, e)
 // makes row 4g + e, all three weights, both forms, hi and lo.
-__device__ __forceinline__ void gen_stage_split(AdSmemS &sm, int buf, const AdjArgs &a, const float *__restrict__ sq,
-                                                int st, int na, bool tab) {
+template <class RT>
+__device__ __forceinline__ void gen_stage_split(StageA &dst, const RT &rt, const AtomRec *ats, const AdjArgs &a,
+                                                const float *__restrict__ sq, int st, int na, bool tab) {
     const int t = threadIdx.x;
     const int at = t & (NA - 1), g = (t / NA) & 3, e = t / (4 * NA);
     const int rr = 4 * g + e, r = st * SKR + rr;
@@ -323,17 +401,66 @@ __device__ __forceinline__ void gen_stage_split(AdSmemS &sm, int buf, const AdjA
     float s2 = 0.0f;
     if (r < a.R && at < na) {
         s2 = sq[r];
-        v = row_element(sm.rt, sm.at[at], at, st, rr, r, tab, s2);
+        v = row_element(rt, ats[at], at, st, rr, r, tab, s2);
     }
     const float wt[3] = {1.0f, (float)r - a.rc, s2};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         uint32_t hi, lo;
         bsplit(v.x * wt[k], v.y * wt[k], hi, lo);
-        reinterpret_cast<uint32_t *>(&sm.a[buf][0][k][0][g][at])[e] = hi ^ 0x80000000u;
-        reinterpret_cast<uint32_t *>(&sm.a[buf][1][k][0][g][at])[e] = rot16(hi);
-        reinterpret_cast<uint32_t *>(&sm.a[buf][0][k][1][g][at])[e] = lo ^ 0x80000000u;
-        reinterpret_cast<uint32_t *>(&sm.a[buf][1][k][1][g][at])[e] = rot16(lo);
+        reinterpret_cast<uint32_t *>(&dst[0][k][0][g][at])[e] = hi ^ 0x80000000u;
+        reinterpret_cast<uint32_t *>(&dst[1][k][0][g][at])[e] = rot16(hi);
+        reinterpret_cast<uint32_t *>(&dst[0][k][1][g][at])[e] = lo ^ 0x80000000u;
+        reinterpret_cast<uint32_t *>(&dst[1][k][1][g][at])[e] = rot16(lo);
+    }
+}
+
+// B operand words of stage st: lane (column li, row group lg) of each of the
+// wave's 16-column tiles starting at column cw (zero past the padded columns).
+__device__ __forceinline__ void load_bwords(const uint32_t *__restrict__ qh, const uint32_t *__restrict__ ql,
+                                            const AdjArgs &a, int cw, int st, u32x4 (&bh)[WC], u32x4 (&bl)[WC]) {
+    const int lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
+#pragma unroll
+    for (int j = 0; j < WC; ++j) {
+        const int col = cw + 16 * j + li;
+        const long long o = ((long long)(st * 4 + lg) * a.C + col) * 4;
+        if (col < a.C) {
+            bh[j] = *reinterpret_cast<const u32x4 *>(qh + o);
+            bl[j] = *reinterpret_cast<const u32x4 *>(ql + o);
+        } else {
+            bh[j] = u32x4{0u, 0u, 0u, 0u};
+            bl[j] = u32x4{0u, 0u, 0u, 0u};
+        }
+    }
+}
+
+// The 72 MFMAs of one stage of one wave: hi.hi + hi.lo + lo.hi for the three
+// row weights, both forms of the real embedding, WA x WC tiles.
+__device__ __forceinline__ void split_mfma_stage(const StageA &sa, const u32x4 (&bh)[WC], const u32x4 (&bl)[WC],
+                                                 f32x4 (&accR)[3][WA][WC], f32x4 (&accI)[3][WA][WC], int wa0) {
+    const int lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
+#pragma unroll
+    for (int i = 0; i < WA; ++i) {
+        u32x4 A[2][3][2];
+#pragma unroll
+        for (int f = 0; f < 2; ++f)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int p = 0; p < 2; ++p) A[f][k][p] = sa[f][k][p][lg][wa0 + 16 * i + li];
+#pragma unroll
+        for (int j = 0; j < WC; ++j)
+#pragma unroll
+            for (int t = 0; t < 3; ++t)          // hi.hi, hi.lo, lo.hi
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const int p = t == 2 ? 1 : 0;
+                    const u32x4 y = t == 1 ? bl[j] : bh[j];
+                    accR[k][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(bf16x8, A[0][k][p]), __builtin_bit_cast(bf16x8, y), accR[k][i][j], 0, 0, 0);
+                    accI[k][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(bf16x8, A[1][k][p]), __builtin_bit_cast(bf16x8, y), accI[k][i][j], 0, 0, 0);
+                }
     }
 }
 
@@ -366,59 +493,21 @@ __global__ void __launch_bounds__(NT) adjoint_split_kernel(AdjArgs a) {
                 accI[k][i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
             }
 
-    // B operand words of one stage: lane (column li, row group lg) of each 16-column tile
-    auto load_b = [&](int st, u32x4 (&bh)[WC], u32x4 (&bl)[WC]) {
-#pragma unroll
-        for (int j = 0; j < WC; ++j) {
-            const int col = col0 + wc0 + 16 * j + li;
-            const long long o = ((long long)(st * 4 + lg) * a.C + col) * 4;
-            if (col < a.C) {
-                bh[j] = *reinterpret_cast<const u32x4 *>(qh + o);
-                bl[j] = *reinterpret_cast<const u32x4 *>(ql + o);
-            } else {
-                bh[j] = u32x4{0u, 0u, 0u, 0u};
-                bl[j] = u32x4{0u, 0u, 0u, 0u};
-            }
-        }
-    };
-
     const int nst = (a.R + SKR - 1) / SKR;
     const bool tab = nst <= MAXST;
     if (tab) build_row_tables(sm.rt, sm.at, nst);
     __syncthreads();
     u32x4 bh[WC], bl[WC], nbh[WC], nbl[WC];
-    load_b(0, bh, bl);
-    gen_stage_split(sm, 0, a, sq, 0, na, tab);
+    load_bwords(qh, ql, a, col0 + wc0, 0, bh, bl);
+    gen_stage_split(sm.a[0], sm.rt, sm.at, a, sq, 0, na, tab);
     __syncthreads();
     for (int s = 0; s < nst; ++s) {
         const int buf = s & 1;
         if (s + 1 < nst) {
-            load_b(s + 1, nbh, nbl);
-            gen_stage_split(sm, buf ^ 1, a, sq, s + 1, na, tab);
+            load_bwords(qh, ql, a, col0 + wc0, s + 1, nbh, nbl);
+            gen_stage_split(sm.a[buf ^ 1], sm.rt, sm.at, a, sq, s + 1, na, tab);
         }
-#pragma unroll
-        for (int i = 0; i < WA; ++i) {
-            u32x4 A[2][3][2];
-#pragma unroll
-            for (int f = 0; f < 2; ++f)
-#pragma unroll
-                for (int k = 0; k < 3; ++k)
-#pragma unroll
-                    for (int p = 0; p < 2; ++p) A[f][k][p] = sm.a[buf][f][k][p][lg][wa0 + 16 * i + li];
-#pragma unroll
-            for (int j = 0; j < WC; ++j)
-#pragma unroll
-                for (int t = 0; t < 3; ++t)          // hi.hi, hi.lo, lo.hi
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        const int p = t == 2 ? 1 : 0;
-                        const u32x4 y = t == 1 ? bl[j] : bh[j];
-                        accR[k][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                            __builtin_bit_cast(bf16x8, A[0][k][p]), __builtin_bit_cast(bf16x8, y), accR[k][i][j], 0, 0, 0);
-                        accI[k][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                            __builtin_bit_cast(bf16x8, A[1][k][p]), __builtin_bit_cast(bf16x8, y), accI[k][i][j], 0, 0, 0);
-                    }
-        }
+        split_mfma_stage(sm.a[buf], bh, bl, accR, accI, wa0);
         if (s + 1 < nst) {
 #pragma unroll
             for (int j = 0; j < WC; ++j) {
@@ -429,6 +518,102 @@ __global__ void __launch_bounds__(NT) adjoint_split_kernel(AdjArgs a) {
         __syncthreads();
     }
     adjoint_epilogue(a, sq, sm.at, sm.red, sm.ct, accR, accI, col0, atom0, na);
+}
+
+// Split adjoint for short grids (R <= 16 RES_ST rows): the work-group makes
+// A for every stage once, keeps it in LDS, and walks a range of a.ctpg column
+// tiles.  The stages of one column tile run without barriers (A is read-only,
+// B streams into registers one stage ahead, also across column tiles); the
+// epilogue's column sums accumulate per lane over the whole range, so the
+// partial set is the column range (blockIdx.x), not the column tile.
+constexpr int RES_ST = 5;
+constexpr int RES_MAXCT = 32;   // column tiles per work-group at most (bounds the FP32 column sums)
+
+struct AdSmemR {
+    StageA a[RES_ST];
+    AtomRec at[NA];
+    float red[NWC][NA][6];
+    ColTables ct;
+    RowTablesT<RES_ST> rt;
+};
+
+__global__ void __launch_bounds__(NT) adjoint_split_res_kernel(AdjArgs a) {
+    __shared__ AdSmemR sm;
+    const int4 tile = a.tiles[blockIdx.y];
+    const int atom0 = tile.x, na = tile.y - tile.x;
+    const float *sq = a.sq + (long long)tile.w * a.sq_stride;
+    const uint32_t *qh = a.qs + (long long)tile.z * 2 * a.Rg * a.C * 4;
+    const uint32_t *ql = qh + (long long)a.Rg * a.C * 4;
+    const int ntc = (a.C + NC - 1) / NC;
+    const int ct0 = blockIdx.x * a.ctpg, ct1 = min(ntc, ct0 + a.ctpg);
+    const int wave = threadIdx.x >> 6;
+    const int wa0 = (wave % NWA) * 16 * WA, wc0 = (wave / NWA) * 16 * WC;
+
+    for (int x = threadIdx.x; x < NA; x += NT) {
+        if (x < na)
+            sm.at[x] = a.rec[atom0 + x];
+        else
+            sm.at[x] = AtomRec{};
+    }
+    __syncthreads();
+    const int nst = (a.R + SKR - 1) / SKR;   // <= RES_ST, checked by the launcher
+    build_row_tables(sm.rt, sm.at, nst);
+    __syncthreads();
+    for (int s = 0; s < nst; ++s) gen_stage_split(sm.a[s], sm.rt, sm.at, a, sq, s, na, true);
+    u32x4 bh[WC], bl[WC], nbh[WC], nbl[WC];
+    if (ct0 < ct1) load_bwords(qh, ql, a, ct0 * NC + wc0, 0, bh, bl);
+    __syncthreads();
+
+    const int li = threadIdx.x & 15, lg = (threadIdx.x & 63) >> 4;
+    float tot[3] = {0.0f, 0.0f, 0.0f};   // row totals of flat entries 3 li .. 3 li + 2, over the column range
+    for (int ct = ct0; ct < ct1; ++ct) {
+        f32x4 accR[3][WA][WC], accI[3][WA][WC];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int i = 0; i < WA; ++i)
+#pragma unroll
+                for (int j = 0; j < WC; ++j) {
+                    accR[k][i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+                    accI[k][i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+                }
+        for (int s = 0; s < nst; ++s) {
+            // next (column tile, stage) of the walk
+            const int nct = s + 1 < nst ? ct : ct + 1, ns = s + 1 < nst ? s + 1 : 0;
+            const bool more = nct < ct1;
+            if (more) load_bwords(qh, ql, a, nct * NC + wc0, ns, nbh, nbl);
+            split_mfma_stage(sm.a[s], bh, bl, accR, accI, wa0);
+            if (more) {
+#pragma unroll
+                for (int j = 0; j < WC; ++j) {
+                    bh[j] = nbh[j];
+                    bl[j] = nbl[j];
+                }
+            }
+        }
+        __syncthreads();   // the previous tile's contraction has read the column tables
+        epi_tables(a, sm.at, sm.ct, ct * NC);
+        __syncthreads();
+        float acc[WA][4][6];
+#pragma unroll
+        for (int i = 0; i < WA; ++i)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+#pragma unroll
+                for (int k = 0; k < 6; ++k) acc[i][v][k] = 0.0f;
+        epi_contract(a, sq, sm.at, sm.ct, accR, accI, ct * NC, acc);
+        float part[3];
+        row_reduce_scatter(acc, part, li);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) tot[q] += part[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const int f = 3 * li + q, i = f / 24, v = (f / 6) % 4, k = f % 6;
+        sm.red[wave / NWA][wa0 + 16 * i + 4 * lg + v][k] = tot[q];
+    }
+    __syncthreads();
+    epi_write(a, sm.red, blockIdx.x, atom0, na);
 }
 
 // FP32 complex Q planes -> split words: for plane t, part (hi, lo), row
@@ -635,12 +820,62 @@ hipError_t launch_fill_zero(double *p, long long n, hipStream_t st) {
 int adjoint_tile_atoms() { return NA; }
 int adjoint_tile_cols() { return NC; }
 
+// Short grids (split scheme, R <= 16 RES_ST) walk column ranges with A
+// resident in LDS; one such work-group fills a CU, so the walk length is
+// chosen to balance rounds of work-groups over the CUs against the per-group
+// prologue (about half a column tile of work).
+int adjoint_max_walk() { return RES_MAXCT; }
+
+int adjoint_plan(AdjArgs &a, int ntiles, bool split, int force) {
+    const int ntc = std::max(1, (a.C + NC - 1) / NC);
+    a.ctpg = 1;
+    if (!split || ntiles <= 0 || a.R <= 0 || (a.R + SKR - 1) / SKR > RES_ST) return ntc;
+    if (force > 0) {
+        a.ctpg = std::min({force, ntc, RES_MAXCT});
+        return (ntc + a.ctpg - 1) / a.ctpg;
+    }
+    // CU count of the current device, queried once per device
+    static std::atomic<int> cus[64];
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < 64) {
+        ncu = cus[dev].load(std::memory_order_relaxed);
+        if (ncu <= 0) {
+            if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
+                ncu = 256;
+            cus[dev].store(ncu, std::memory_order_relaxed);
+        }
+    }
+    double best = 0.0;
+    int bestc = 0;
+    for (int ctpg = std::min(ntc, RES_MAXCT); ctpg >= 1; --ctpg) {
+        const long long groups = (ntc + ctpg - 1) / ctpg;
+        const double rounds = (double)((groups * ntiles + ncu - 1) / ncu);
+        const double cost = rounds * (ctpg + 0.5);
+        if (bestc == 0 || cost < best) {
+            best = cost;
+            bestc = ctpg;
+        }
+    }
+    a.ctpg = bestc;
+    return (ntc + bestc - 1) / bestc;
+}
+
 hipError_t launch_adjoint(const AdjArgs &a, int ntiles, hipStream_t st, bool split) {
     if (ntiles <= 0 || a.R <= 0 || a.C <= 0) return hipSuccess;
-    dim3 grid((a.C + NC - 1) / NC, ntiles);
+    const int ntc = (a.C + NC - 1) / NC;
+    if (a.ctpg < 1 || (a.ctpg > 1 && !split)) return hipErrorInvalidValue;
+    dim3 grid(ntc, ntiles);
     if (split) {
         if (!a.qs || a.Rg * 4 < a.R || a.Rg % 4) return hipErrorInvalidValue;
-        adjoint_split_kernel<<<grid, dim3(NT), 0, st>>>(a);
+        if ((a.R + SKR - 1) / SKR <= RES_ST && a.ctpg > 1) {
+            if (a.ctpg > RES_MAXCT) return hipErrorInvalidValue;
+            grid.x = (ntc + a.ctpg - 1) / a.ctpg;
+            adjoint_split_res_kernel<<<grid, dim3(NT), 0, st>>>(a);
+        } else if (a.ctpg == 1) {
+            adjoint_split_kernel<<<grid, dim3(NT), 0, st>>>(a);
+        } else {
+            return hipErrorInvalidValue;
+        }
     } else {
         adjoint_kernel<<<grid, dim3(NT), 0, st>>>(a);
     }

@@ -296,6 +296,14 @@ int dc_set_gemm_tile(dc_ctx *c, int tile) {
 
 int dc_gemm_num_tiles(void) { return gemm_num_tiles(); }
 
+int dc_set_adjoint_walk(dc_ctx *c, int walk) {
+    if (!c) return DC_ERR_ARG;
+    if (walk < 0 || walk > adjoint_max_walk())
+        return set_err(c, DC_ERR_ARG, DC_ER_LIB, "adjoint walk out of range");
+    c->adj_walk = walk;
+    return DC_OK;
+}
+
 int dc_gemm_tile_shape(int tile, int *rows, int *cols, int *threads) {
     if (tile < 0 || tile >= gemm_num_tiles()) return DC_ERR_ARG;
     TileInfo t = gemm_tile_info(tile);

@@ -33,6 +33,8 @@ struct BatchState {
     int nwg = 0;
     std::vector<int> order;
     int nsorted = 0, nslice = 0, ntiles = 0, ntc = 0, nblk = 0;
+    int walk = 1, nset = 0;              // adjoint column tiles per work-group, partial sets
+    int walk_req = 0;                    // the context's adjoint walk setting at planning
     int64_t natoms = 0;
     DevBuf pos, occ, bdw, idx, rec, sbeg, stype, ssq, msl, partial, cfact, rten, sq, target, weight, icalc, chi,
         q, qs, atiles, part, grad;
@@ -77,7 +79,7 @@ bool same_shape(const BatchState *b, const dc_ctx *c, const dc_batch_desc *d, bo
     if (b->nmembers != d->nmembers || b->nscat != d->nscat || b->ncfact != d->ncfact || b->prec != c->prec ||
         b->tile_forced != c->tile || b->autotune != c->autotune || b->graphs != c->graphs ||
         b->want_grad != want_grad || b->want_icalc != want_icalc || b->have_weight != have_weight ||
-        b->any_b != any_b)
+        b->any_b != any_b || b->walk_req != c->adj_walk)
         return false;
     if (std::memcmp(b->eck, d->eck, sizeof(b->eck)) || std::memcmp(b->vi, d->vi, sizeof(b->vi)) ||
         std::memcmp(b->inc, d->inc, sizeof(b->inc)))
@@ -146,6 +148,7 @@ int build_state(dc_ctx *c, const dc_batch_desc *d, BatchState &b) {
     b.nscat = d->nscat;
     b.ncfact = d->ncfact;
     b.prec = c->prec;
+    b.walk_req = c->adj_walk;
     b.tile_forced = c->tile;
     b.autotune = c->autotune;
     b.graphs = c->graphs;
@@ -205,6 +208,13 @@ int build_state(dc_ctx *c, const dc_batch_desc *d, BatchState &b) {
     b.nslice = (int)ss.type.size();
     b.ntiles = (int)tiles.size() / 4;
     b.ntc = (gg.C + adjoint_tile_cols() - 1) / adjoint_tile_cols();
+    {
+        AdjArgs pa{};
+        pa.R = gg.R;
+        pa.C = gg.C;
+        b.nset = adjoint_plan(pa, b.ntiles, b.prec != DC_PREC_FP32, c->adj_walk);
+        b.walk = pa.ctpg;
+    }
     b.nblk = (int)std::min<long long>((g.P + 255) / 256, 256);
 
     int st;
@@ -301,10 +311,11 @@ int enqueue(dc_ctx *c, BatchState &b, int tile, int nwg, bool prep, bool rest) {
             aa.qs = ptr<uint32_t>(b.qs);
             HIPCHK(launch_q_split(ptr<float2>(b.q), b.nmembers * b.nscat, gg.R, gg.C, gg.P, aa.Rg, ptr<uint32_t>(b.qs), c->st));
         }
+        aa.ctpg = b.walk;
         HIPCHK(launch_adjoint(aa, b.ntiles, c->st, split));
         AdjFinishArgs fa{};
         fa.part = ptr<float>(b.part);
-        fa.ntile_c = b.ntc;
+        fa.ntile_c = b.nset;
         fa.natoms = b.nsorted;
         fa.idx = ptr<int>(b.idx);
         fa.occ = ptr<double>(b.occ);
@@ -472,6 +483,7 @@ int dc_batch_chi2(dc_ctx *c, const dc_batch_desc *d, const double *target, const
     c->stats.slices = b.nslice;
     c->stats.gemm_launches = 1;
     c->stats.gemm_ms = b.tune_best_ms;
+    c->stats.adjoint_walk = want_grad ? b.walk : 0;
     return DC_OK;
 }
 

@@ -24,6 +24,7 @@ struct dc_ctx {
     int tile = -1;                  // forced GEMM tile configuration, -1 = planned
     int autotune = 1;               // batch: time tile shapes on the first call of a shape
     int graphs = 1;                 // batch: capture the population step as a hipGraph
+    int adj_walk = 0;               // split adjoint column walk: 0 planned, else forced (short grids)
     BatchState *batch = nullptr;    // cached plan / buffers / graph of the last batch shape
     PostState *post = nullptr;      // FFT plans and buffers of the post-processing calls
     NufftState *nufft = nullptr;    // fine grid, plans and buffers of the NUFFT technique

@@ -114,13 +114,14 @@ struct AdjArgs {
     const float2 *q;        // Q planes: q[t * plane + c + C * r] (column fastest)
     const float *sq;        // s^2 parts: R rows, then C columns (per member: + tile.w * sq_stride)
     long long sq_stride;
-    float *part;            // [column tile][natoms][6]
+    float *part;            // [partial set][natoms][6]: one set per column tile or column range
     const uint32_t *qs;     // split scheme: Q as BF16 hi/lo words, per plane [hi|lo][row group][C][4 rows]
     int Rg;                 // row groups of 4 rows in qs (R padded to a multiple of 16)
     int R, C, C0, f1;       // rows, padded columns, column period, valid columns per period
     long long plane;        // R * C
     int natoms;
     float rc, jc, mc;       // centres of the row / column index weights
+    int ctpg;               // column tiles per work-group (set by adjoint_plan; 1 = one tile each)
 };
 struct AdjFinishArgs {
     const float *part;
@@ -183,6 +184,12 @@ struct JvpArgs {
 hipError_t launch_jvp_reduce(const JvpArgs &a, hipStream_t st);
 int adjoint_tile_atoms();
 int adjoint_tile_cols();
+// Chooses the adjoint kernel's column walk for a.R, a.C and ntiles atom
+// tiles (sets a.ctpg) and returns the number of partial sets it writes
+// (<= ceil(C / adjoint_tile_cols())), which adjoint_finish sums.  force > 0
+// asks for that many column tiles per work-group where the walk applies.
+int adjoint_plan(AdjArgs &a, int ntiles, bool split, int force = 0);
+int adjoint_max_walk();
 // split = true: BF16x3 MFMA on qs (made by launch_q_split), else FP32 on q.
 hipError_t launch_adjoint(const AdjArgs &a, int ntiles, hipStream_t st, bool split = false);
 // FP32 complex Q planes (c + C r) -> BF16 hi/lo words for the split adjoint.

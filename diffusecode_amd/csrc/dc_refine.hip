@@ -252,11 +252,13 @@ int model_adjoint(dc_ctx *c, const dc_model_desc *m, ModelPlan &mp) {
             aa.qs = ptr<uint32_t>(c->m_qs);
             HIPCHK(launch_q_split(ptr<float2>(c->m_q), nq, gg.R, gg.C, gg.P, aa.Rg, ptr<uint32_t>(c->m_qs), c->st));
         }
+        const int nset = adjoint_plan(aa, ntiles, split, c->adj_walk);
+        c->stats.adjoint_walk = aa.ctpg;
         HIPCHK(launch_adjoint(aa, ntiles, c->st, split));
 
         AdjFinishArgs fa{};
         fa.part = ptr<float>(c->m_part);
-        fa.ntile_c = ntc;
+        fa.ntile_c = nset;
         fa.natoms = mp.nsorted;
         fa.idx = ptr<int>(c->idx);
         fa.occ = ptr<double>(c->m_tgt);

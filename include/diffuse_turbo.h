@@ -173,6 +173,8 @@ typedef struct dc_stats {
     int64_t slices;
     double forward_ms;          /* refinement model: forward (GEMM + reduction)   */
     double adjoint_ms;          /* refinement model: reverse pass (Q, GEMM, sums) */
+    int64_t adjoint_walk;       /* column tiles per work-group of the last reverse
+                                   pass (1: one work-group per column tile)       */
 } dc_stats;
 int dc_last_stats(const dc_ctx *ctx, dc_stats *out);
 int dc_synchronize(dc_ctx *ctx);
@@ -274,6 +276,14 @@ int dc_batch_chi2(dc_ctx *ctx, const dc_batch_desc *batch, const double *target,
 /* Autotuner and hipGraph capture of batch calls (both on by default). */
 int dc_set_autotune(dc_ctx *ctx, int on);
 int dc_set_graphs(dc_ctx *ctx, int on);
+/* Column walk of the BF16x3 / FP16x2 reverse pass on grids of at most 80
+ * abscissa points (dc_model_vjp, dc_model_chi2, dc_batch_chi2): 0 (default)
+ * plans it from the grid and the CU count; 1..32 forces that many column
+ * tiles of 256 per work-group (1 = one work-group per column tile).  A tuning
+ * knob with no counterpart in DISCUS; it changes the FP32 summation order of
+ * the gradient partials, not the result beyond rounding.  Reported back in
+ * dc_stats.adjoint_walk. */
+int dc_set_adjoint_walk(dc_ctx *ctx, int walk);
 /* Choice of the last batch shape: tile configuration, persistent work-groups
  * (0 = one per work item), whether a graph replays it, autotuned GEMM ms. */
 int dc_batch_info(const dc_ctx *ctx, int *tile, int *nwg, int *graph, double *tune_ms);

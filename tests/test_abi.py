@@ -46,6 +46,7 @@ def test_null_context_arguments():
     L = _lib.lib()
     assert L.dc_set_precision(None, 0) == _lib.DC_ERR_ARG
     assert L.dc_set_gemm_tile(None, 0) == _lib.DC_ERR_ARG
+    assert L.dc_set_adjoint_walk(None, 0) == _lib.DC_ERR_ARG
     assert L.dc_crystal_set(None, None) == _lib.DC_ERR_ARG
     assert L.dc_four_run(None, None, None, None, None, None, None) == _lib.DC_ERR_ARG
     assert L.dc_strucf(None, None, 0, 0, None, None, None, None, None, None, None, None, 0, None, None, None, 0,

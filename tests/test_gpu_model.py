@@ -89,6 +89,63 @@ def test_model_vjp_matches_fp64_gradient(ctx, n, prec):
     assert max(e) <= GRAD_BAR_PREC[prec]
 
 
+_WALK_CASE = {}
+
+
+def walk_case():
+    """4,096 atoms (128 atom tiles of 32) on a 17 x 33 x 33 grid: 2 stages of
+    16 abscissa rows and 1,089 columns = 5 column tiles of 256."""
+    if not _WALK_CASE:
+        cr, occ, bdw, cf, eck, vi, inc = problem(ncell=(8, 8, 8), n=(17, 33, 33))
+        F = O.model_f(cr.pos, cr.iscat, occ, bdw, cf, eck, vi, inc, cr.rten())
+        gi = np.random.default_rng(21).standard_normal(F.size)
+        ref = O.model_grad(cr.pos, cr.iscat, occ, bdw, cf, eck, vi, inc, cr.rten(), F, gi)
+        _WALK_CASE.update(prob=(cr, occ, bdw, cf, eck, vi, inc), gi=gi, ref=ref)
+    return _WALK_CASE
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "fp16x2"])
+@pytest.mark.parametrize("walk,expect", [(1, 1), (2, 2), (3, 3), (5, 5), (32, 5), (0, None)])
+def test_model_vjp_split_column_walk(ctx, prec, walk, expect):
+    """Split reverse pass on a short grid: one work-group per column tile
+    (walk 1), column ranges with A resident in LDS (2 and 3 leave a ragged
+    last range, 5 and 32 walk all five column tiles in one range), and the
+    planned walk.  Each matches the FP64 gradient to the scheme's bar (FP16x2
+    is the lower-precision mode whose A-only split leaves ~1e-4 on random
+    structures), and the walks agree with one another to FP32 summation-order
+    rounding."""
+    case = walk_case()
+    cr, occ, bdw, cf, eck, vi, inc = case["prob"]
+    d, keep = ctx.model_desc(cr.pos, cr.iscat, occ, bdw, cf, eck, vi, inc, cr.rten())
+    ctx.set_precision(prec)
+    ctx.set_adjoint_walk(walk)
+    try:
+        icalc, fcalc = ctx.model_intensity(d)
+        g = ctx.model_vjp(d, fcalc, case["gi"])
+        used = ctx.stats()["adjoint_walk"]
+        ctx.set_adjoint_walk(1)
+        g1 = ctx.model_vjp(d, fcalc, case["gi"])
+    finally:
+        ctx.set_adjoint_walk(0)
+        ctx.set_precision("fp32")
+    e = grad_err(g, case["ref"])
+    e1 = grad_err(g, g1)
+    print(f"  {prec} walk {walk} (used {used}): vs fp64", " ".join(f"{x:.2e}" for x in e),
+          "| vs walk 1", " ".join(f"{x:.1e}" for x in e1))
+    if expect is not None:
+        assert used == expect
+    assert max(e) <= {"bf16x3": GRAD_BAR_PREC["bf16x3"], "fp16x2": 1e-3}[prec]
+    assert max(e1) <= 1e-5
+
+
+def test_adjoint_walk_knob_range(ctx):
+    with pytest.raises(RuntimeError):
+        ctx.set_adjoint_walk(33)
+    with pytest.raises(RuntimeError):
+        ctx.set_adjoint_walk(-1)
+    ctx.set_adjoint_walk(0)
+
+
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
 def test_model_chi2_gradient_vs_central_differences(ctx, prec):
     """200 random parameters: GPU chi^2 gradient against FP64 central differences (h = 1e-6)."""
