@@ -86,6 +86,44 @@ def build_fortran() -> str | None:
     return FORTRAN_EXE
 
 
+YELL_SRC = os.path.join(CSRC, "dc_yell.c")
+YELL_LIB = os.path.join(HERE, "libdiffuse_yell.so")
+FORTRAN_EXAMPLE_YELL = os.path.join(HERE, "fortran", "example_yell.f90")
+FORTRAN_EXE_YELL = os.path.join(OBJ, "fortran", "example_yell")
+
+
+def hdf5_root() -> str | None:
+    """Prefix holding include/hdf5.h and lib/libhdf5.so: $HDF5_ROOT, else the
+    image's conda prefix, else /usr."""
+    for r in (os.environ.get("HDF5_ROOT"), "/opt/conda", "/usr"):
+        if r and os.path.exists(os.path.join(r, "include", "hdf5.h")) and \
+                os.path.exists(os.path.join(r, "lib", "libhdf5.so")):
+            return r
+    return None
+
+
+def build_yell() -> str | None:
+    """libdiffuse_yell.so, the Yell 1.0 HDF5 writer (gcc, linked to HDF5 with
+    an rpath).  Without an HDF5 installation there is no writer, as DISCUS
+    links hdf_write_no.f90 then; returns None."""
+    root = hdf5_root()
+    if root is None:
+        return None
+    inc, lib = os.path.join(root, "include"), os.path.join(root, "lib")
+    _run(["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", inc,
+          YELL_SRC, "-o", YELL_LIB, "-L", lib, "-lhdf5", "-Wl,-rpath," + lib])
+    # the Fortran host example, when the interface module has been built (the
+    # module's own procedures reference libdiffuse_turbo, so both are linked)
+    fc = shutil.which("flang") or "/opt/rocm/llvm/bin/flang"
+    out = os.path.join(OBJ, "fortran")
+    mod = os.path.join(out, "diffuse_turbo_mod.o")
+    if os.path.exists(fc) and os.path.exists(mod):
+        _run([fc, "-O2", "-module-dir", out, "-I", out, FORTRAN_EXAMPLE_YELL, mod, "-o", FORTRAN_EXE_YELL,
+              "-L", HERE, "-ldiffuse_yell", "-ldiffuse_turbo", "-Wl,-rpath," + HERE])
+    return YELL_LIB
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
     print(build_fortran())
+    print(build_yell())

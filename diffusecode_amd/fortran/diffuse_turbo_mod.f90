@@ -100,6 +100,19 @@ module diffuse_turbo_mod
       real(c_double)     :: faver_sign
    end type dc_qval_desc
 
+   ! Yell 1.0 HDF5 output (include/diffuse_yell.h, libdiffuse_yell.so):
+   ! gen_hdf5_write's arguments (lib_f90/hdf_write.f90:12-16)
+   type, bind(C), public :: dc_yell_desc
+      integer(c_int)     :: inc(3)
+      real(c_double)     :: eck(12)
+      real(c_double)     :: vi(9)
+      integer(c_int)     :: extr(3)
+      real(c_double)     :: a0(3)
+      real(c_double)     :: win(3)
+      integer(c_int)     :: value
+      real(c_double)     :: valmax
+   end type dc_yell_desc
+
    interface
       integer(c_int) function dc_create(ctx, device) bind(C, name='dc_create')
          import :: c_ptr, c_int
@@ -199,11 +212,19 @@ module diffuse_turbo_mod
          type(dc_run_desc), intent(in) :: run
          type(c_ptr), value :: type_occ, csf, dsi
       end function dc_four_run_nufft
+
+      ! in libdiffuse_yell.so (link -ldiffuse_yell where it is called)
+      integer(c_int) function dc_write_yell(path, desc, qvalues, sigma) bind(C, name='dc_write_yell')
+         import :: c_ptr, c_int, c_char, dc_yell_desc
+         character(kind=c_char), dimension(*), intent(in) :: path
+         type(dc_yell_desc), intent(in) :: desc
+         type(c_ptr), value :: qvalues, sigma
+      end function dc_write_yell
    end interface
 
    public :: dc_create, dc_destroy, dc_set_precision, dc_last_error_type, dc_strucf
    public :: dc_crystal_set, dc_four_run, dc_four_post, dc_four_fill, dc_qval
-   public :: dc_nufft1, dc_four_strucf_nufft, dc_four_run_nufft
+   public :: dc_nufft1, dc_four_strucf_nufft, dc_four_run_nufft, dc_write_yell
    public :: dc_four_strucf, dc_shutdown
 
    type(c_ptr), save :: the_ctx = c_null_ptr

@@ -102,6 +102,7 @@ class Fourier:
         self.res_tr = np.zeros((3, 3))     # diff_tr
         self.type_occ = None               # cr_occ per type, the NUFFT weight
         self.want_dsi3d = False            # four_weight's dsi3d (for 3D-PDF output)
+        self.extr = [1, 2, 3]              # extr_abs/_ord/_top: h, k, l along the axes (graphic.f90:2695-2697)
         self.ctx = ctx
         self.precision = precision
         self.crystal = None
@@ -256,6 +257,42 @@ class Fourier:
             if self.inc[k] > 1 and self.res_sigma[k] > 0.0:
                 v = self.vi[:, k]
                 self.res_tr[k, k] = np.sqrt(v @ g @ v) / self.res_sigma[k]
+
+    def _extr(self, axis, comp):
+        """'abscissa | ordinate | top  h|k|l' (fourier.f90:223-240, 622-639,
+        889-906): which reciprocal component runs along the axis."""
+        m = {"h": 1, "k": 2, "l": 3}
+        if comp not in m:
+            raise FourierError(-6, ER_COMM, "parameter must be h, k or l")
+        self.extr[axis] = m[comp]
+
+    def abscissa(self, comp):
+        self._extr(0, comp)
+
+    def ordinate(self, comp):
+        self._extr(1, comp)
+
+    def top(self, comp):
+        self._extr(2, comp)
+
+    def output_hdf5(self, path, value="inten", laver=False, scale=1.0, valmax=0.0, **qval_kw):
+        """DISCUS output with 'outfile type hdf5' (graphic.f90:606-636): the
+        qval of every pixel of the last run (dc_qval on the GPU) times the output
+        scale, written as a Yell 1.0 file (dc_write_yell, hdf_write.f90:12-326).
+        Returns the values written (before valmax)."""
+        from . import yell
+        if self.dsi is None or self.crystal is None:
+            raise FourierError(-118, ER_APPL, "You need to calculate a Fourier / Patterson first")
+        c = self.crystal
+        qd, keep = _lib.Context.qval_desc(self.inc, self.eck, self.vi, c.rten(), value, laver=laver, **qval_kw)
+        try:
+            q = self.ctx.qval(qd, csf=self.csf, acsf=self.acsf, dsi=self.dsi)
+        except _lib.DCError as e:
+            raise FourierError(e.ier_num, e.ier_typ, str(e)) from None
+        q = q * scale
+        yell.write_yell(path, q, self.eck, self.vi, c.lattice[:3], c.lattice[3:], value=qd.value, valmax=valmax,
+                        extr=self.extr)
+        return q
 
     def set_precision(self, precision):
         self.precision = precision

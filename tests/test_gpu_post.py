@@ -5,6 +5,8 @@ Bars: FFT-based steps (four_conv, Lanczos) max|d|/max|ref| <= 1e-12 (FP64
 transforms of different libraries differ in rounding only); four_weight,
 the point-group average and the Bragg fill are bit-exact (same operations in
 the same order); qval <= 1e-13 relative (exp/atan2/sqrt rounding)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -316,3 +318,41 @@ def test_fourier_menu_filter_resolution_and_nufft(ctx):
     out = g.run()
     ref = O.four_run(cr, g.eck, g.vi, g.inc, cf, four_friedel=False)
     assert rel(out["dsi"], ref["dsi"]) <= 1e-6
+
+
+def test_fourier_output_hdf5(ctx, tmp_path):
+    """'output' with 'outfile type hdf5' (graphic.f90:606-636): dc_qval of the
+    run's pixels on the GPU times the output scale, written as a Yell 1.0 file;
+    the file's data is the FP64 oracle's intensity (x scale) and its step
+    sizes follow 'abscissa k / ordinate h'."""
+    from diffusecode_amd import build as B
+    if B.hdf5_root() is None or not os.path.exists(B.YELL_LIB):
+        pytest.skip("no HDF5 installation / libdiffuse_yell.so not built")
+    from diffusecode_amd.fourier import Fourier
+    from h5read import H5
+    cr = X.rocksalt(n=3, seed=5)
+    f = Fourier(cr, ctx=ctx)
+    f.ll(-1.5, -1.5, 0.0)
+    f.lr(1.5, -1.5, 0.0)
+    f.ul(-1.5, 1.5, 0.0)
+    f.nabs(13)
+    f.nord(11)
+    f.abscissa("k")
+    f.ordinate("h")
+    f.run()
+    path = tmp_path / "run.h5"
+    q = f.output_hdf5(path, value="inten", scale=2.0)
+    ref = O.four_run(cr, f.eck, f.vi, f.inc, FR.formtab(cr))
+    assert rel(q, 2.0 * ref["dsi"]) <= 1e-6
+    h5 = H5()
+    fid = h5.open(path)
+    try:
+        data, r = h5.doubles(fid, "data")
+        steps, _ = h5.doubles(fid, "step_sizes")
+        cell, _ = h5.doubles(fid, "unit_cell")
+    finally:
+        h5.L.H5Fclose(fid)
+    assert r["dims"] == (13, 11, 1)
+    np.testing.assert_array_equal(data, q)
+    np.testing.assert_array_equal(steps, [f.vi[1, 0], f.vi[0, 1], f.vi[2, 2]])
+    np.testing.assert_array_equal(cell, list(cr.lattice))
