@@ -33,6 +33,7 @@ library's HIP events on its stream):
 """
 import argparse
 import ctypes as C
+import dataclasses
 import json
 import os
 import re
@@ -46,9 +47,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "atom·hkl pairs/s, split-BF16 MFMA vs plain FP32 HIP, at ≤1e-5 rel err vs FP64"
-PEAK = {"bf16x3": 2500.0, "fp16x2": 2500.0, "fp32": 157.3}   # TFLOP/s dense
-KSPLIT = {"bf16x3": 3, "fp16x2": 2, "fp32": 1}
+PEAK = {"bf16x3": 2500.0, "fp16x3": 2500.0, "fp16x2": 2500.0, "fp32": 157.3}   # TFLOP/s dense
+KSPLIT = {"bf16x3": 3, "fp16x3": 3, "fp16x2": 2, "fp32": 1}
 HBM_PEAK = 8000.0   # GB/s
+SHIFT = np.array([0.37, -0.21, 0.113])   # general origin shift of the coherent-sum check
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "phase_gemm_fcc40_161cube_traffic.md")
 
 
@@ -330,6 +332,26 @@ def main():
     v16 = pairs / t16
     dsi_16 = d_dsi.view(inc[2], inc[1], inc[0]).cpu().numpy()
     log(f"fp16x2: {t16 * 1e3:.2f} ms/step, {v16 / 1e12:.2f} Tpairs/s, gemm {g16:.2f} ms/launch")
+    # FP16x3 (both sides split, 512-atom slices): the split scheme that holds
+    # 1e-5 on coherent sums as well
+    t163, g163, l163 = run_steps("fp16x3", fp32_steps, 1)
+    v163 = pairs / t163
+    dsi_163 = d_dsi.view(inc[2], inc[1], inc[0]).cpu().numpy()
+    log(f"fp16x3: {t163 * 1e3:.2f} ms/step, {v163 / 1e12:.2f} Tpairs/s, gemm {g163:.2f} ms/launch")
+
+    # coherent sums: the same crystal shifted by a general vector.  Its sites
+    # are no longer at 0 and 1/2, so at the Bragg peaks every cell adds the
+    # same inexact phase and the per-product error of a scheme adds up over
+    # all atoms; one run per scheme, the same two sampled layers
+    moved = dataclasses.replace(c, pos=c.pos + SHIFT)
+    ctx.crystal_set(moved)
+    coherent_samples = {}
+    for prec in ("bf16x3", "fp16x3", "fp16x2", "fp32"):
+        ctx.set_precision(prec)
+        ctx.four_run_dev(desc, csf=d_csf, dsi=d_dsi)
+        torch.cuda.synchronize()
+        dm = d_dsi.view(inc[2], inc[1], inc[0]).cpu().numpy()
+        coherent_samples[prec] = {0.0: dm[80, ::4, ::4].T.copy(), 1.0: dm[100, ::4, ::4].T.copy()}
 
     # dominant kernel: the phase GEMM; algorithmic MFMA flops = 8 x k_split per
     # computed atom.hkl pair (the GEMM computes the Friedel-reduced region)
@@ -345,14 +367,21 @@ def main():
                                 "peak": PEAK["fp32"]},
                 "fp16x2_kernel": {"avg_launch_ms": round(g16, 3),
                                   "achieved": round(16.0 * pairs_computed_per_launch / (g16 * 1e-3) / 1e12, 2),
-                                  "peak": PEAK["fp16x2"]}}
+                                  "peak": PEAK["fp16x2"]},
+                "fp16x3_kernel": {"avg_launch_ms": round(g163, 3), "launches_per_step": l163,
+                                  "achieved": round(24.0 * natoms * fnum0 * inc[1] * inc[2] / l163 / (g163 * 1e-3)
+                                                    / 1e12, 2),
+                                  "peak": PEAK["fp16x3"]}}
 
     # bounded sample of the GPU output for the check in the CPU leg: two
     # l-layers (l = 0 and l = 1) subsampled every 4th point (h, k step 0.2)
     head_samples = {0.0: dsi_split[80, ::4, ::4].T.copy(), 1.0: dsi_split[100, ::4, ::4].T.copy()}
     fp16_samples = {0.0: dsi_16[80, ::4, ::4].T.copy(), 1.0: dsi_16[100, ::4, ::4].T.copy()}
+    f163_samples = {0.0: dsi_163[80, ::4, ::4].T.copy(), 1.0: dsi_163[100, ::4, ::4].T.copy()}
     max_err = None
     err16 = None
+    err163 = None
+    coherent = None
     ctx.close()
 
     extras = {}
@@ -393,6 +422,19 @@ def main():
             log(f"sampled max|dI|/max(I) vs FP64 oracle: {max_err:.3e}")
             err16 = sampled_err(fp16_samples)
             log(f"fp16x2 sampled max|dI|/max(I): {err16:.3e}")
+            err163 = sampled_err(f163_samples)
+            log(f"fp16x3 sampled max|dI|/max(I): {err163:.3e}")
+            moved = dataclasses.replace(c, pos=c.pos + SHIFT)
+            mrefs = {}
+            for lval in (0.0, 1.0):
+                e2, v2, i2 = FR.grid_block((-4, -4, lval), (4, 4, lval), (41, 41, 1))
+                mrefs[lval] = O.four_run(moved, e2, v2, i2, cf)["dsi"][:, :, 0]
+            coherent = {"workload": "the headline crystal shifted by t = (0.37, -0.21, 0.113) (all 256k atoms in "
+                                    "phase at the Bragg peaks), layers l = 0 and 1 every 4th point vs FP64"}
+            for prec, smp in coherent_samples.items():
+                errs = [(np.abs(smp[l] - mrefs[l]).max(), mrefs[l].max()) for l in mrefs]
+                coherent[prec] = float(max(e for e, _ in errs) / max(m for _, m in errs))
+            log(f"coherent sums, max|dI|/max(I): {json.dumps(coherent)}")
             if LOTS_CHECK and "lots" in extras:
                 cl = LOTS_CHECK["crystal"]
                 e3, v3, i3 = FR.grid_block((-10.0, -10.0, 0.0), (10.0, 10.0, 0.0), (41, 41, 1))
@@ -431,7 +473,8 @@ def main():
                    "atoms": int(c.natoms), "grid": inc, "pairs_per_step": pairs, "precision": args.prec},
         "fp32_value": v32, "fp32_ms_per_step": t32 * 1e3, "speedup_vs_fp32": value / v32,
         "fp16x2_value": v16, "fp16x2_ms_per_step": t16 * 1e3, "fp16x2_max_rel_err_sampled": err16,
-        "max_rel_err_sampled": max_err,
+        "fp16x3_value": v163, "fp16x3_ms_per_step": t163 * 1e3, "fp16x3_max_rel_err_sampled": err163,
+        "max_rel_err_sampled": max_err, "coherent_sum_max_rel_err": coherent,
         "roofline": roofline, "cpu_baseline": cpu,
     }
     line.update(extras)

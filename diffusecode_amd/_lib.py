@@ -27,7 +27,7 @@ DC_ERR_NOMEM = -1002
 DC_ERR_NODEV = -1003
 DC_ERR_STATE = -1004
 
-PREC = {"fp32": 0, "bf16x3": 1, "fp16x2": 2}
+PREC = {"fp32": 0, "bf16x3": 1, "fp16x2": 2, "fp16x3": 3}
 LOT = {"off": 1, "box": 2, "eli": 3}
 CFPKT = 9999
 

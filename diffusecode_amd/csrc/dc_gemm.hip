@@ -32,6 +32,14 @@
 //              products xh*yh + xh*yl + xl*yh, FP32 accumulation.
 //   SCH_FP16X2 v_mfma_f32_16x16x32_f16, A split (ah + al), B single half,
 //              products ah*b + al*b.
+//   SCH_FP16X3 v_mfma_f32_16x16x32_f16, both sides split into FP16 hi + lo,
+//              products xh*yh + xh*yl + xl*yh: the same MFMA count as
+//              BF16X3, but 11-bit halves leave 2^-22 instead of 2^-16 per
+//              product, which coherent sums (Bragg peaks of crystals whose
+//              sites are not at special positions) add up over every atom.
+//              The FP16 schemes scale the B amplitudes of each slice by a
+//              power of two (max |w| into [0.5, 1)) and undo it exactly in
+//              the epilogue, so any weight range keeps FP16's precision.
 //
 // Pipeline: LDS holds two element buffers and two table buffers.  In
 // iteration s every wave builds its share of the elements of stage s+1 and
@@ -47,6 +55,11 @@ namespace dc {
 constexpr int SCH_FP32 = 0;
 constexpr int SCH_BF16X3 = 1;
 constexpr int SCH_FP16X2 = 2;
+constexpr int SCH_FP16X3 = 3;
+// three products (hi.hi, hi.lo, lo.hi) with both sides split
+__host__ __device__ constexpr bool split3(int s) { return s == SCH_BF16X3 || s == SCH_FP16X3; }
+// FP16 operands (v_mfma_f32_16x16x32_f16)
+__host__ __device__ constexpr bool f16ops(int s) { return s == SCH_FP16X2 || s == SCH_FP16X3; }
 
 constexpr int KS = 16;            // atoms per LDS stage
 
@@ -58,17 +71,19 @@ typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ uint32_t pack2(float a, float b, std::integral_constant<int, SCH_BF16X3>) {
-    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));
+template <int SCHEME>
+__device__ __forceinline__ uint32_t pack2(float a, float b) {
+    if constexpr (f16ops(SCHEME))
+        return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, f16x2));
+    else
+        return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));
 }
-__device__ __forceinline__ uint32_t pack2(float a, float b, std::integral_constant<int, SCH_FP16X2>) {
-    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, f16x2));
-}
-__device__ __forceinline__ f32x2 unpack2(uint32_t w, std::integral_constant<int, SCH_BF16X3>) {
-    return (f32x2){__builtin_bit_cast(float, w << 16), __builtin_bit_cast(float, w & 0xFFFF0000u)};
-}
-__device__ __forceinline__ f32x2 unpack2(uint32_t w, std::integral_constant<int, SCH_FP16X2>) {
-    return __builtin_convertvector(__builtin_bit_cast(f16x2, w), f32x2);
+template <int SCHEME>
+__device__ __forceinline__ f32x2 unpack2(uint32_t w) {
+    if constexpr (f16ops(SCHEME))
+        return __builtin_convertvector(__builtin_bit_cast(f16x2, w), f32x2);
+    else
+        return (f32x2){__builtin_bit_cast(float, w << 16), __builtin_bit_cast(float, w & 0xFFFF0000u)};
 }
 
 // Round the 64-bit phase to the 32-bit turn fraction used by sincos_turn.
@@ -98,7 +113,7 @@ constexpr int FST = 36;
 template <int SCHEME, class T>
 struct Elems {
     uint32_t a_hi[T::RT * 16], a_lo[T::RT * 16];
-    uint32_t b_hi[T::CT * 16], b_lo[(SCHEME == SCH_BF16X3 ? T::CT : 1) * 16];
+    uint32_t b_hi[T::CT * 16], b_lo[(split3(SCHEME) ? T::CT : 1) * 16];
 };
 template <class T>
 struct Elems<SCH_FP32, T> {
@@ -122,6 +137,7 @@ struct Smem {
     int a_k16[T::RH], a_wpos[T::RH];   // wraps before group h, position of the wrap inside it
     int b_k16[T::CH], b_wpos[T::CH];
     float s_a[T::RT], s_b[T::CT];      // s^2 row / column parts of the tile (damped variant)
+    float wmax[T::NW];                 // per-wave max |w| of the slice (FP16 schemes)
 };
 
 // Side geometry of the tile: start index, offset along the fast axis, period.
@@ -148,7 +164,7 @@ __device__ __forceinline__ float2 cis(uint64_t ph) {
 template <class T>
 __device__ __forceinline__ void build_tables(Tables<T> &tb, const int *__restrict__ a_k16,
                                              const int *__restrict__ b_k16, const AtomRec &ar, bool live,
-                                             const Side &sa, const Side &sb, int pt) {
+                                             const Side &sa, const Side &sb, int pt, float wscale) {
     const int a = pt & (KS - 1), r = pt >> 4;
     constexpr int NR = T::NPT / KS;           // helper threads per atom
     if (r == 0) tb.dmp[a] = live ? ar.pad * 1.4426950408889634f : 0.0f;
@@ -170,7 +186,7 @@ __device__ __forceinline__ void build_tables(Tables<T> &tb, const int *__restric
             v = cmul(v, e1);
         }
     } else {
-        const float w = live ? ar.w : 0.0f;
+        const float w = live ? ar.w * wscale : 0.0f;   // wscale: power of two, exact
         const uint64_t wa = ar.ra1 - (uint64_t)sa.period * ar.ra;
         const uint64_t wb = ar.ca1 - (uint64_t)sb.period * ar.ca;
         const uint64_t pa = (uint64_t)sa.a0 * ar.ra + (uint64_t)sa.hi0 * ar.ra1;
@@ -226,13 +242,12 @@ __device__ __forceinline__ void elems_group(Elems<SCHEME, T> &el, const float2 *
         *reinterpret_cast<f32x4 *>(dst) = f32x4{v[0].x, v[0].y, v[1].x, v[1].y};
         *reinterpret_cast<f32x4 *>(dst + 4) = f32x4{v[2].x, v[2].y, v[3].x, v[3].y};
     } else {
-        constexpr auto S = std::integral_constant<int, SCHEME>{};
         u32x4 hi, lo;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            hi[q] = pack2(v[q].x, v[q].y, S);
-            const f32x2 hvv = unpack2(hi[q], S);
-            lo[q] = pack2(v[q].x - hvv.x, v[q].y - hvv.y, S);
+            hi[q] = pack2<SCHEME>(v[q].x, v[q].y);
+            const f32x2 hvv = unpack2<SCHEME>(hi[q]);
+            lo[q] = pack2<SCHEME>(v[q].x - hvv.x, v[q].y - hvv.y);
         }
         const int o = pk_off(row, g);
         if constexpr (isA) {
@@ -240,7 +255,7 @@ __device__ __forceinline__ void elems_group(Elems<SCHEME, T> &el, const float2 *
             *reinterpret_cast<u32x4 *>(&el.a_lo[o]) = lo;
         } else {
             *reinterpret_cast<u32x4 *>(&el.b_hi[o]) = hi;
-            if constexpr (SCHEME == SCH_BF16X3) *reinterpret_cast<u32x4 *>(&el.b_lo[o]) = lo;
+            if constexpr (split3(SCHEME)) *reinterpret_cast<u32x4 *>(&el.b_lo[o]) = lo;
         }
     }
 }
@@ -317,14 +332,14 @@ __device__ __forceinline__ void mfma_stage(const Elems<SCHEME, T> &el, f32x4 (&a
             arl[i] = l ^ 0x80000000u;
             ail[i] = (l >> 16) | (l << 16);
         }
-        constexpr int NTERM = SCHEME == SCH_BF16X3 ? 3 : 2;
+        constexpr int NTERM = split3(SCHEME) ? 3 : 2;
         if constexpr (JOUTER) {
 #pragma unroll
             for (int j = 0; j < T::WC; ++j) {
                 const int o = pk_off(wc0 + 16 * j + li, lg);
                 const u32x4 yh = *reinterpret_cast<const u32x4 *>(&el.b_hi[o]);
                 u32x4 yl = yh;
-                if constexpr (SCHEME == SCH_BF16X3) yl = *reinterpret_cast<const u32x4 *>(&el.b_lo[o]);
+                if constexpr (split3(SCHEME)) yl = *reinterpret_cast<const u32x4 *>(&el.b_lo[o]);
 #pragma unroll
                 for (int t = 0; t < NTERM; ++t)
 #pragma unroll
@@ -332,8 +347,8 @@ __device__ __forceinline__ void mfma_stage(const Elems<SCHEME, T> &el, f32x4 (&a
                         const bool alo = (t == NTERM - 1);
                         const u32x4 xr = alo ? arl[i] : arh[i];
                         const u32x4 xi = alo ? ail[i] : aih[i];
-                        const u32x4 y = (SCHEME == SCH_BF16X3 && t == 1) ? yl : yh;
-                        if constexpr (SCHEME == SCH_BF16X3) {
+                        const u32x4 y = (split3(SCHEME) && t == 1) ? yl : yh;
+                        if constexpr (!f16ops(SCHEME)) {
                             accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, xr), __builtin_bit_cast(bf16x8, y), accR[i][j], 0, 0, 0);
                             accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, xi), __builtin_bit_cast(bf16x8, y), accI[i][j], 0, 0, 0);
                         } else {
@@ -348,7 +363,7 @@ __device__ __forceinline__ void mfma_stage(const Elems<SCHEME, T> &el, f32x4 (&a
         for (int j = 0; j < (JOUTER ? 0 : T::WC); ++j) {
             const int o = pk_off(wc0 + 16 * j + li, lg);
             bh[j] = *reinterpret_cast<const u32x4 *>(&el.b_hi[o]);
-            if constexpr (SCHEME == SCH_BF16X3) bl[j] = *reinterpret_cast<const u32x4 *>(&el.b_lo[o]);
+            if constexpr (split3(SCHEME)) bl[j] = *reinterpret_cast<const u32x4 *>(&el.b_lo[o]);
         }
 #pragma unroll
         for (int t = 0; t < NTERM; ++t) {
@@ -356,12 +371,12 @@ __device__ __forceinline__ void mfma_stage(const Elems<SCHEME, T> &el, f32x4 (&a
             for (int j = 0; j < (JOUTER ? 0 : T::WC); ++j)
 #pragma unroll
                 for (int i = 0; i < T::WR; ++i) {
-                    // terms: (a_hi, b_hi), (a_hi, b_lo) [BF16X3 only], (a_lo, b_hi)
+                    // terms: (a_hi, b_hi), (a_hi, b_lo) [three-product schemes], (a_lo, b_hi)
                     const bool alo = (t == NTERM - 1);
                     const u32x4 xr = alo ? arl[i] : arh[i];
                     const u32x4 xi = alo ? ail[i] : aih[i];
-                    const u32x4 y = (SCHEME == SCH_BF16X3 && t == 1) ? bl[j] : bh[j];
-                    if constexpr (SCHEME == SCH_BF16X3) {
+                    const u32x4 y = (split3(SCHEME) && t == 1) ? bl[j] : bh[j];
+                    if constexpr (!f16ops(SCHEME)) {
                         accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, xr), __builtin_bit_cast(bf16x8, y), accR[i][j], 0, 0, 0);
                         accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, xi), __builtin_bit_cast(bf16x8, y), accI[i][j], 0, 0, 0);
                     } else {
@@ -453,11 +468,32 @@ phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice
         const AtomRec r1 = fetch(nst > 1 ? 1 : 0);
         AtomRec rn = r1;
         if constexpr (DEEP) rn = fetch(nst > 2 ? 2 : 0);
+        // FP16 schemes: B amplitudes of the slice scaled by 2^-e, max |w| = f 2^e
+        // with f in [0.5, 1); the epilogue multiplies by 2^e (both exact)
+        float wscale = 1.0f, wunscale = 1.0f;
+        if constexpr (f16ops(SCHEME)) {
+            float m = 0.0f;
+            for (int k = kbeg + pt; k < kend; k += T::NT) m = fmaxf(m, fabsf(rec[k].w));
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+            if ((pt & 63) == 0) sm.wmax[wave] = m;
+        }
         __syncthreads();
-        build_tables<T>(sm.t[0], sm.a_k16, sm.b_k16, r0, kbeg + at < kend, sa, sb, pt);
+        if constexpr (f16ops(SCHEME)) {
+            float m = 0.0f;
+#pragma unroll
+            for (int w = 0; w < T::NW; ++w) m = fmaxf(m, sm.wmax[w]);
+            if (m > 0.0f && m <= 3.0e38f) {
+                int e = 0;
+                (void)frexpf(m, &e);
+                wscale = ldexpf(1.0f, -e);
+                wunscale = ldexpf(1.0f, e);
+            }
+        }
+        build_tables<T>(sm.t[0], sm.a_k16, sm.b_k16, r0, kbeg + at < kend, sa, sb, pt, wscale);
         __syncthreads();
         build_elems<SCHEME, T, DAMP>(sm.e[0], sm.t[0], sm.a_wpos, sm.b_wpos, sm.s_a, sm.s_b, wave);
-        if (nst > 1) build_tables<T>(sm.t[1], sm.a_k16, sm.b_k16, r1, kbeg + KS + at < kend, sa, sb, pt);
+        if (nst > 1) build_tables<T>(sm.t[1], sm.a_k16, sm.b_k16, r1, kbeg + KS + at < kend, sa, sb, pt, wscale);
         __syncthreads();
         for (int s = 0; s < nst; ++s) {
             const int cur = s & 1, nxt = cur ^ 1;
@@ -469,7 +505,7 @@ phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice
                 rt = fetch(s + 2 < nst ? s + 2 : 0);
             }
             if (s + 1 < nst) build_elems<SCHEME, T, DAMP>(sm.e[nxt], sm.t[nxt], sm.a_wpos, sm.b_wpos, sm.s_a, sm.s_b, wave);
-            if (s + 2 < nst) build_tables<T>(sm.t[cur], sm.a_k16, sm.b_k16, rt, kbeg + (s + 2) * KS + at < kend, sa, sb, pt);
+            if (s + 2 < nst) build_tables<T>(sm.t[cur], sm.a_k16, sm.b_k16, rt, kbeg + (s + 2) * KS + at < kend, sa, sb, pt, wscale);
             if (live) mfma_stage<SCHEME, T>(sm.e[cur], accR, accI, wr0, wc0);
             if constexpr (TWO) {
                 if ((s & 3) == 3 || s + 1 == nst) {
@@ -508,7 +544,12 @@ phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int r = row0 + wr0 + 16 * i + 4 * lg + v;
-                    if (r < g.R) out[r + (long long)g.R * c] = make_float2(accR[i][j][v], accI[i][j][v]);
+                    if (r < g.R) {
+                        if constexpr (f16ops(SCHEME))
+                            out[r + (long long)g.R * c] = make_float2(accR[i][j][v] * wunscale, accI[i][j][v] * wunscale);
+                        else
+                            out[r + (long long)g.R * c] = make_float2(accR[i][j][v], accI[i][j][v]);
+                    }
                 }
             }
     }
@@ -578,6 +619,7 @@ hipError_t launch_phase_gemm(int scheme, int tile, const AtomRec *rec, const int
     case SCH_FP32: return launch_scheme<SCH_FP32>(tile, rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st);
     case SCH_BF16X3: return launch_scheme<SCH_BF16X3>(tile, rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st);
     case SCH_FP16X2: return launch_scheme<SCH_FP16X2>(tile, rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st);
+    case SCH_FP16X3: return launch_scheme<SCH_FP16X3>(tile, rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st);
     default: return hipErrorInvalidValue;
     }
 }

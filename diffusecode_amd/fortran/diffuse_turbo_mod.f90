@@ -18,7 +18,7 @@ module diffuse_turbo_mod
    private
 
    integer(c_int), parameter, public :: DC_OK = 0
-   integer(c_int), parameter, public :: DC_PREC_FP32 = 0, DC_PREC_BF16X3 = 1, DC_PREC_FP16X2 = 2
+   integer(c_int), parameter, public :: DC_PREC_FP32 = 0, DC_PREC_BF16X3 = 1, DC_PREC_FP16X2 = 2, DC_PREC_FP16X3 = 3
    integer(c_int), parameter, public :: DC_LOT_OFF = 1, DC_LOT_BOX = 2, DC_LOT_ELI = 3
 
    ! dc_run_desc of include/diffuse_turbo.h

@@ -62,7 +62,7 @@ def ctx():
     c.close()
 
 
-@pytest.mark.parametrize("prec,bar", [("fp32", 1e-6), ("bf16x3", 1e-5)])
+@pytest.mark.parametrize("prec,bar", [("fp32", 1e-6), ("bf16x3", 1e-5), ("fp16x3", 1e-5)])
 def test_batch_matches_oracle(ctx, prec, bar):
     ctx.set_precision(prec)
     try:

@@ -13,7 +13,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-BAR = {"fp32": 1e-6, "bf16x3": 1e-5}
+BAR = {"fp32": 1e-6, "bf16x3": 1e-5, "fp16x3": 1e-5}
 
 
 @pytest.fixture(scope="module")
@@ -40,7 +40,7 @@ def test_layer_call_without_atoms_is_zero(ctx):
     assert F.shape == tuple(num) and np.all(F == 0)
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "fp16x3"])
 def test_single_atom_and_slice_boundary(ctx, prec):
     """1 atom, and 16,385 atoms of one type (one full 16,384-atom slice plus a
     1-atom slice), on a grid through integer hkl.  A single atom has no
@@ -63,7 +63,7 @@ def test_single_atom_and_slice_boundary(ctx, prec):
         assert err <= (4e-5 if (prec == "bf16x3" and pos.shape[0] == 1) else BAR[prec])
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "fp16x3"])
 def test_type_without_atoms_and_long_line(ctx, prec):
     """A scattering type with no atoms in the crystal, on a centred 1D line of
     2001 points through the Bragg peaks h00 (Friedel-halved to 1001)."""

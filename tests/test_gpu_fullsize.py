@@ -1,0 +1,134 @@
+"""GPU parity at the benchmark sizes, through properties that do not depend on
+the size (the exact FP64 oracle cannot cover a whole 161^3 block in test
+time):
+
+* the structure-factor run on fcc 40^3 cells (256,000 atoms, 3 types) over
+  the 161^3 block h, k, l in [-4, 4] (the bench workload):
+  - two subsampled l-layers (41 x 41, through the Bragg peaks) against the
+    exact FP64 oracle, for the crystal and for the crystal shifted by a
+    general vector t;
+  - translation invariance: shifting every atom by the same vector t changes
+    F by the phase exp(2 pi i h.t) only, so I is unchanged.  The sites of the
+    fcc crystal are at 0 and 1/2, where the phases of the Bragg peaks are
+    exact; after the shift every cell adds the same inexact phase, a
+    coherent sum over 256,000 atoms;
+  - Friedel symmetry I(h) = I(-h) of a run without Friedel halving, where
+    every pixel of the block is computed;
+* the refinement model on 32,768 atoms and a 65^3 block (the adjoint bench
+  workload):
+  - duality of the derivative maps, <J v, u> = <v, J^T u>, between the JVP
+    (forward-mode columns) and the VJP (adjoint) on the same F;
+  - the split BF16x3 adjoint (column walk over all 17 column tiles) against
+    the FP32 adjoint.
+
+Bars: the forward's max|dI|/max(I) bars, 1e-6 (FP32) and 1e-5 (FP16x3), for
+each run against the oracle, and twice that between two runs.  BF16x3 is held
+to its per-product bound on coherent sums, 4e-5 (the rounding of the lo
+halves and the dropped lo*lo term leave ~2^-17 per real product, which a
+coherent sum does not average out): it meets 1e-5 only where the sum is not
+coherent.  Duality to 1e-5 of |Jv| |u|; the split adjoint to 1e-4 of max|g|
+per parameter kind."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from diffusecode_amd import _lib
+from diffusecode_amd import crystal as X
+from diffusecode_amd import fourier as FR
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+BAR = {"fp32": 1e-6, "fp16x3": 1e-5, "bf16x3": 4e-5}
+SHIFT = np.array([0.37, -0.21, 0.113])
+_CASE = {}
+
+
+def headline():
+    if not _CASE:
+        c = X.fcc_ordered(n=40, a=4.0, ntypes=3, mc_steps=100_000, seed=2)
+        eck, vi, inc = FR.grid_block((-4, -4, -4), (4, 4, 4), (161, 161, 161))
+        cf = FR.formtab(c)
+        moved = dataclasses.replace(c, pos=c.pos + SHIFT)
+        refs = {}
+        for name, cc in (("c", c), ("moved", moved)):
+            for k, lval in ((80, 0.0), (100, 1.0)):
+                e2, v2, i2 = FR.grid_block((-4, -4, lval), (4, 4, lval), (41, 41, 1))
+                refs[name, k] = O.four_run(cc, e2, v2, i2, cf)["dsi"][:, :, 0]
+        _CASE.update(c=c, moved=moved, eck=eck, vi=vi, inc=inc, cf=cf, refs=refs)
+    return _CASE
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = _lib.Context()
+    yield c
+    c.close()
+
+
+def run(ctx, c, h, prec, friedel=True):
+    ctx.set_precision(prec)
+    try:
+        ctx.crystal_set(c)
+        desc, keep = ctx.run_desc(h["eck"], h["vi"], h["inc"], c.rten(), h["cf"], four_friedel=friedel)
+        return ctx.four_run(desc)["dsi"]
+    finally:
+        ctx.set_precision("fp32")
+
+
+def sampled_err(dsi, h, name):
+    refs = {k: r for (n, k), r in h["refs"].items() if n == name}
+    scale = max(r.max() for r in refs.values())
+    return max(np.abs(dsi[::4, ::4, k] - r).max() for k, r in refs.items()) / scale
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16x3", "bf16x3"])
+def test_headline_layers_and_translation(ctx, prec):
+    h = headline()
+    dsi = run(ctx, h["c"], h, prec)
+    err = sampled_err(dsi, h, "c")
+    dsi_t = run(ctx, h["moved"], h, prec)
+    err_t = sampled_err(dsi_t, h, "moved")
+    terr = np.abs(dsi_t - dsi).max() / dsi.max()
+    print(f"  {prec}: sampled layers vs FP64 {err:.2e}, shifted crystal vs FP64 {err_t:.2e}, "
+          f"I(shifted) vs I {terr:.2e}")
+    assert err <= BAR[prec]
+    assert err_t <= BAR[prec]
+    assert terr <= 2 * BAR[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16x3", "bf16x3"])
+def test_headline_friedel_symmetry_without_halving(ctx, prec):
+    h = headline()
+    dsi = run(ctx, h["c"], h, prec, friedel=False)
+    ferr = np.abs(dsi - dsi[::-1, ::-1, ::-1]).max() / dsi.max()
+    print(f"  {prec}: max|I(h) - I(-h)|/max(I) = {ferr:.2e}")
+    assert ferr <= 2 * BAR[prec]
+
+
+def test_refinement_duality_and_split_adjoint(ctx):
+    from diffusecode_amd.refine import scattering_table
+    cr = X.orthorhombic()
+    occ, bdw = X.refinement_parameters(cr)
+    cf = scattering_table(cr.scat)
+    eck, vi, inc = FR.grid_block((-2.0, -2.0, -2.0), (2.0, 2.0, 2.0), (65, 65, 65))
+    d, keep = ctx.model_desc(cr.pos, cr.iscat, occ, bdw, cf, eck, vi, inc, cr.rten())
+    icalc, fcalc = ctx.model_intensity(d)
+    rng = np.random.default_rng(17)
+    u = rng.standard_normal(icalc.size)
+    v = rng.standard_normal((1, len(cr.iscat), 5)) * np.array([1e-3, 1e-3, 1e-3, 1e-2, 1e-2])
+    g32 = ctx.model_vjp(d, fcalc, u)
+    jv = ctx.model_jvp(d, fcalc, v)[:, 0]
+    lhs, rhs = float(jv @ u), float((v[0] * g32).sum())
+    dual = abs(lhs - rhs) / (np.linalg.norm(jv) * np.linalg.norm(u))
+    ctx.set_precision("bf16x3")
+    try:
+        gs = ctx.model_vjp(d, fcalc, u)
+        walk = ctx.stats()["adjoint_walk"]
+    finally:
+        ctx.set_precision("fp32")
+    serr = max(np.abs(gs[:, k] - g32[:, k]).max() / np.abs(g32[:, k]).max() for k in range(5))
+    print(f"  duality |<Jv,u> - <v,J^T u>|/(|Jv||u|) = {dual:.2e}; split adjoint (walk {walk}) vs FP32 {serr:.2e}")
+    assert dual <= 1e-5
+    assert serr <= 1e-4

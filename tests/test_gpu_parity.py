@@ -1,7 +1,7 @@
 """GPU parity: HIP path (through the C ABI) against the CPU oracle.
 
 Bars: max|dI|/max(I) <= 1e-6 for plain FP32,
-<= 1e-5 for the split BF16x3 / FP16x2 paths, against the FP64 exact oracle.
+<= 1e-5 for the split BF16x3 / FP16x3 / FP16x2 paths, against the FP64 exact oracle.
 """
 import numpy as np
 import pytest
@@ -12,7 +12,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-BAR = {"fp32": 1e-6, "bf16x3": 1e-5, "fp16x2": 1e-5}
+BAR = {"fp32": 1e-6, "bf16x3": 1e-5, "fp16x3": 1e-5, "fp16x2": 1e-5}
 
 
 @pytest.fixture(scope="module")
@@ -46,7 +46,7 @@ def lmn_corner(lmn, off, vi):
     return sum(int(m) * np.asarray(v) for m, v in zip(lmn, vecs))
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "fp16x2"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "fp16x3", "fp16x2"])
 @pytest.mark.parametrize("shape", [(29, 23, 1), (17, 13, 11), (5, 1, 1), (1, 1, 1), (200, 3, 1)])
 def test_strucf_layer_call(ctx, prec, shape):
     """dc_strucf follows the lmn/off_shift phase construction of four_strucf
@@ -80,7 +80,7 @@ def _ntiles():
     return len(_lib.Context.gemm_tiles())
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "fp16x3"])
 @pytest.mark.parametrize("shape", [(41, 23, 19), (19, 37, 1), (17, 9, 5)])
 @pytest.mark.parametrize("tile", range(8))
 def test_every_tile_configuration(ctx, prec, shape, tile):
@@ -126,7 +126,7 @@ def test_fp16x2_bragg_dominated(ctx):
     assert err <= 1e-4
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "fp16x3"])
 def test_strucf_layer_form_factor(ctx, prec):
     """lform path with host istl and with device-computed istl give the oracle's F*cfact."""
     print(f"case strucf-form prec={prec}", flush=True)
@@ -162,7 +162,7 @@ def run_pair(ctx, c, prec, eck, vi, inc, **kw):
     return gpu, ref
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "fp16x3"])
 def test_four_run_rocksalt_hk0(ctx, prec):
     """The full rocksalt workload: 10x10x10 cells (8000 sites, 5% vacancies,
     sigma 0.05 A), hk0 layer h,k in [-5,5] at 101x101 (Friedel-halved), FP64
@@ -177,7 +177,7 @@ def test_four_run_rocksalt_hk0(ctx, prec):
     np.testing.assert_allclose(gpu["acsf"], 0.0)
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "fp16x3"])
 def test_four_run_lots_aver(ctx, prec):
     """Box lots with periodic wrap, 'set aver' <F> subtraction, cross-lot variance."""
     print(f"case four_run lots prec={prec}", flush=True)
@@ -209,7 +209,7 @@ def test_four_run_ellipsoid_3d(ctx):
     assert err <= 1e-6
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "fp16x3"])
 def test_fortran_host_dropin(prec):
     """The Fortran host program (diffuse_turbo_mod, ISO_C_BINDING) calls the
     library like DISCUS's four_strucf and checks a closed form."""
