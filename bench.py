@@ -168,7 +168,7 @@ def bench_nufft(_lib, torch, c, steps=3):
     """The NUFFT technique (four_run_nufft, fourier_sup.f90:327-690) on the
     headline workload: fcc 40^3 cells, 161^3 block h,k,l in [-4,4] (step 0.05
     -> scales 2, positions 2 pi * 2 * pos / 40), per type a 3D type-1 NUFFT at
-    tol 1e-7 as DISCUS sets it, form factors, sum, |F|^2 (dc_four_run_nufft_dev,
+    tol 1e-7 as DISCUS sets it (two types per transform), form factors, sum, |F|^2 (dc_four_run_nufft_dev,
     outputs in HBM).  Accuracy of I against the exact FP64 oracle on two
     subsampled l-layers, as for the headline (checked in the CPU leg)."""
     from diffusecode_amd import fourier as FR
@@ -199,8 +199,9 @@ def bench_nufft(_lib, torch, c, steps=3):
     return {"ms": round(ms, 3), "value": natoms * n / (ms * 1e-3), "unit": "atom·hkl pairs/s (full grid)",
             "tol": 1e-7, "max_rel_err_sampled": None,
             "_samples": {0.0: dsi[::4, ::4, 80].copy(), 1.0: dsi[::4, ::4, 100].copy()},
-            "note": "four_run_nufft on the headline workload: per type spread (exponential-of-semicircle, w=8), "
-                    "324^3 hipFFT Z2Z, deconvolution, form factor; sum over types and |F|^2; FP64"}
+            "note": "four_run_nufft on the headline workload: types in pairs, real spread of each type "
+                    "(exponential-of-semicircle, w=8) into one component of a 324^3 complex grid, hipFFT Z2Z, "
+                    "Hermitian separation and deconvolution, form factor; sum over types and |F|^2; FP64"}
 
 
 def bench_post(_lib, FR, torch, steps=5):

@@ -192,6 +192,74 @@ __global__ __launch_bounds__(256) void spread3_team_kernel(SpreadArgs a, long lo
     }
 }
 
+// 3D, one real strength sv for all points (the occupancy of one atom type):
+// a team of W lanes spreads one point into component `comp` of the complex
+// grid (0 real, 1 imaginary), lane t taking abscissa offset t.  Two types
+// share one grid, one in each component, so one transform serves both and
+// each point costs W^3 atomics instead of the 2 W^3 of a complex strength.
+template <int W>
+__global__ __launch_bounds__(256) void spread3_real_kernel(SpreadArgs a, long long M, const double *__restrict__ x,
+                                                           const double *__restrict__ y, const double *__restrict__ z,
+                                                           double sv, int comp, double *__restrict__ grid) {
+    constexpr int PER = 64 / W;
+    const int lane = threadIdx.x & 63;
+    const int team = lane / W;
+    const int i1 = lane - team * W;
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+    for (long long base = wave * PER; base < M; base += nwaves * PER) {
+        const long long j = base + team;
+        if (team >= PER || j >= M) continue;
+        double u[3], lo[3];
+        const double *xs[3] = {x, y, z};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            double v = xs[d][j] * a.scale[d];
+            v = v - floor(v / (double)a.n[d]) * (double)a.n[d];
+            u[d] = v;
+            lo[d] = ceil(v - a.half_w);
+        }
+        double k2[W], k3[W];
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+            const double z2 = (lo[1] + (double)i - u[1]) * a.two_over_w;
+            const double z3 = (lo[2] + (double)i - u[2]) * a.two_over_w;
+            const double s2 = 1.0 - z2 * z2, s3 = 1.0 - z3 * z3;
+            k2[i] = s2 > 0.0 ? exp(a.beta * (sqrt(s2) - 1.0)) : 0.0;
+            k3[i] = s3 > 0.0 ? exp(a.beta * (sqrt(s3) - 1.0)) : 0.0;
+        }
+        const double z1 = (lo[0] + (double)i1 - u[0]) * a.two_over_w;
+        const double s1 = 1.0 - z1 * z1;
+        const double cv = sv * (s1 > 0.0 ? exp(a.beta * (sqrt(s1) - 1.0)) : 0.0);
+        const long long g1 = wrapi((long long)lo[0] + i1, a.n[0]);
+        const long long l1 = (long long)lo[1], l2 = (long long)lo[2];
+        for (int i3 = 0; i3 < W; ++i3) {
+            const long long g3 = wrapi(l2 + i3, a.n[2]) * a.n[1];
+            const double v3 = cv * k3[i3];
+#pragma unroll
+            for (int i2 = 0; i2 < W; ++i2) {
+                const long long g = (g3 + wrapi(l1 + i2, a.n[1])) * a.n[0] + g1;
+                unsafeAtomicAdd(grid + 2 * g + comp, v3 * k2[i2]);
+            }
+        }
+    }
+}
+
+void launch_spread_real(int w, hipStream_t st, const SpreadArgs &a, long long M, const double *x, const double *y,
+                        const double *z, double sv, int comp, double *g) {
+    const int tg = grid_for(M * w);   // w threads per point
+    switch (w) {
+    case 2: spread3_real_kernel<2><<<tg, 256, 0, st>>>(a, M, x, y, z, sv, comp, g); return;
+    case 4: spread3_real_kernel<4><<<tg, 256, 0, st>>>(a, M, x, y, z, sv, comp, g); return;
+    case 6: spread3_real_kernel<6><<<tg, 256, 0, st>>>(a, M, x, y, z, sv, comp, g); return;
+    case 8: spread3_real_kernel<8><<<tg, 256, 0, st>>>(a, M, x, y, z, sv, comp, g); return;
+    case 10: spread3_real_kernel<10><<<tg, 256, 0, st>>>(a, M, x, y, z, sv, comp, g); return;
+    case 12: spread3_real_kernel<12><<<tg, 256, 0, st>>>(a, M, x, y, z, sv, comp, g); return;
+    case 14: spread3_real_kernel<14><<<tg, 256, 0, st>>>(a, M, x, y, z, sv, comp, g); return;
+    default: spread3_real_kernel<16><<<tg, 256, 0, st>>>(a, M, x, y, z, sv, comp, g); return;
+    }
+}
+
 template <int DIM>
 void launch_spread(int w, int grid, hipStream_t st, const SpreadArgs &a, long long M, const double *x, const double *y,
                    const double *z, const double2 *c, double *g) {
@@ -240,6 +308,29 @@ __global__ void deconv_kernel(DeconvArgs a, const double2 *__restrict__ B, const
         const double f = c1[m1] * c2[m2] * c3[m3];
         const double2 b = B[g];
         fk[e] = make_double2(b.x * f, b.y * f);
+    }
+}
+
+// Two real grids transformed as one complex grid Z = FFT(gA + i gB): the
+// Hermitian symmetry of each part separates them,
+//   GA(k) = (Z(k) + conj Z(-k)) / 2,   GB(k) = (Z(k) - conj Z(-k)) / (2 i),
+// then the same correction as deconv_kernel (phihat is even in k).
+__global__ void deconv_pair_kernel(DeconvArgs a, const double2 *__restrict__ Z, const double *__restrict__ c1,
+                                   const double *__restrict__ c2, const double *__restrict__ c3, double2 *__restrict__ fa,
+                                   double2 *__restrict__ fb) {
+    const long long total = a.out[0] * a.out[1] * a.out[2];
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long long)gridDim.x * blockDim.x) {
+        const long long o1 = e % a.out[0];
+        const long long o2 = (e / a.out[0]) % a.out[1];
+        const long long o3 = e / (a.out[0] * a.out[1]);
+        const long long m1 = o1 * a.sub[0], m2 = o2 * a.sub[1], m3 = o3 * a.sub[2];
+        const long long k1 = m1 - a.N[0] / 2, k2 = m2 - a.N[1] / 2, k3 = m3 - a.N[2] / 2;
+        const long long gp = wrapi(k1, a.n[0]) + a.n[0] * (wrapi(k2, a.n[1]) + a.n[1] * wrapi(k3, a.n[2]));
+        const long long gm = wrapi(-k1, a.n[0]) + a.n[0] * (wrapi(-k2, a.n[1]) + a.n[1] * wrapi(-k3, a.n[2]));
+        const double f = 0.5 * (c1[m1] * c2[m2] * c3[m3]);
+        const double2 zp = Z[gp], zm = Z[gm];
+        fa[e] = make_double2((zp.x + zm.x) * f, (zp.y - zm.y) * f);
+        fb[e] = make_double2((zp.y + zm.y) * f, (zm.x - zp.x) * f);
     }
 }
 
@@ -328,6 +419,7 @@ struct NufftState {
     std::map<std::tuple<long long, long long, int>, DevBuf> corr;   // (N, n, w) -> 1 / phihat on the device
     std::vector<std::vector<double>> corr_host;                      // sources of the uploads, kept alive
     DevBuf grid, pts[3], c, fk, one;
+    DevBuf pts2[3], fk2;                                             // the second type of a pair (run level)
 };
 
 namespace dc {
@@ -335,7 +427,8 @@ void nufft_release(dc_ctx *c) {
     if (!c->nufft) return;
     for (auto &kv : c->nufft->plans) (void)hipfftDestroy(kv.second);
     NufftState *s = c->nufft;
-    for (DevBuf *b : {&s->grid, &s->pts[0], &s->pts[1], &s->pts[2], &s->c, &s->fk, &s->one})
+    for (DevBuf *b : {&s->grid, &s->pts[0], &s->pts[1], &s->pts[2], &s->c, &s->fk, &s->one, &s->pts2[0], &s->pts2[1],
+                      &s->pts2[2], &s->fk2})
         if (b->p) (void)hipFree(b->p);
     for (auto &kv : s->corr)
         if (kv.second.p) (void)hipFree(kv.second.p);
@@ -368,14 +461,11 @@ int nu_params(dc_ctx *c, int dim, double tol, const long long N[3], NuParams &p)
     return 0;
 }
 
-// Spread, transform, deconvolve; points/weights already on the device.
-int nufft_run(dc_ctx *c, const NuParams &p, long long M, const double *dx, const double *dy, const double *dz,
-              const double2 *dc, int iflag, const long long sub[3], const long long out[3], double2 *dfk) {
+// Correction tables 1/phihat per axis (cached per (N, n, w)) and the FFT plan
+// of the fine grid (cached per grid), on the context's stream.
+int nu_setup(dc_ctx *c, const NuParams &p, const double *corr[3], hipfftHandle &h) {
     NufftState &s = nu_state(c);
-    const long long ng = p.n[0] * p.n[1] * p.n[2];
-    int st = ensure(c, s.grid, sizeof(double2) * (size_t)ng);
-    if (st) return st;
-    const double *corr[3];
+    int st;
     for (int d = 0; d < 3; ++d) {
         auto key = std::make_tuple(d < p.dim ? p.N[d] : 1LL, d < p.dim ? p.n[d] : 1LL, d < p.dim ? p.w : 0);
         auto it = s.corr.find(key);
@@ -387,30 +477,8 @@ int nufft_run(dc_ctx *c, const NuParams &p, long long M, const double *dx, const
         }
         corr[d] = reinterpret_cast<const double *>(it->second.p);
     }
-    double *g = ptr<double>(s.grid);
-    zero_kernel<<<grid_for(2 * ng), 256, 0, c->st>>>(2 * ng, g);
-    SpreadArgs a{};
-    a.dim = p.dim;
-    a.w = p.w;
-    a.beta = p.beta;
-    a.half_w = 0.5 * p.w;
-    a.two_over_w = 2.0 / p.w;
-    for (int d = 0; d < 3; ++d) {
-        a.n[d] = p.n[d];
-        a.scale[d] = (double)p.n[d] / (2.0 * 3.14159265358979323846);
-    }
-    if (M > 0) {
-        if (p.dim == 1)
-            launch_spread<1>(p.w, grid_for(M), c->st, a, M, dx, dy, dz, dc, g);
-        else if (p.dim == 2)
-            launch_spread<2>(p.w, grid_for(M), c->st, a, M, dx, dy, dz, dc, g);
-        else
-            launch_spread<3>(p.w, grid_for(M), c->st, a, M, dx, dy, dz, dc, g);
-    }
-    HIPCHK(hipGetLastError());
     // fine-grid transform (row-major dims: slowest first)
     auto key = std::make_tuple(p.n[0], p.n[1], p.n[2]);
-    hipfftHandle h;
     auto it = s.plans.find(key);
     if (it == s.plans.end()) {
         int dims[3];
@@ -422,9 +490,24 @@ int nufft_run(dc_ctx *c, const NuParams &p, long long M, const double *dx, const
         h = it->second;
     }
     if (hipfftSetStream(h, c->st) != HIPFFT_SUCCESS) return set_err(c, DC_ERR_DEVICE, DC_ER_LIB, "hipfftSetStream failed");
-    hipfftResult r = hipfftExecZ2Z(h, reinterpret_cast<hipfftDoubleComplex *>(g), reinterpret_cast<hipfftDoubleComplex *>(g),
-                                   iflag > 0 ? HIPFFT_BACKWARD : HIPFFT_FORWARD);
-    if (r != HIPFFT_SUCCESS) return set_err(c, DC_ERR_DEVICE, DC_ER_LIB, "hipfftExecZ2Z failed (" + std::to_string((int)r) + ")");
+    return 0;
+}
+
+SpreadArgs spread_args(const NuParams &p) {
+    SpreadArgs a{};
+    a.dim = p.dim;
+    a.w = p.w;
+    a.beta = p.beta;
+    a.half_w = 0.5 * p.w;
+    a.two_over_w = 2.0 / p.w;
+    for (int d = 0; d < 3; ++d) {
+        a.n[d] = p.n[d];
+        a.scale[d] = (double)p.n[d] / (2.0 * 3.14159265358979323846);
+    }
+    return a;
+}
+
+DeconvArgs deconv_args(const NuParams &p, const long long sub[3], const long long out[3]) {
     DeconvArgs da{};
     for (int d = 0; d < 3; ++d) {
         da.N[d] = p.N[d];
@@ -432,8 +515,73 @@ int nufft_run(dc_ctx *c, const NuParams &p, long long M, const double *dx, const
         da.sub[d] = sub[d];
         da.out[d] = out[d];
     }
+    return da;
+}
+
+int nu_exec(dc_ctx *c, hipfftHandle h, double *g, int iflag) {
+    hipfftResult r = hipfftExecZ2Z(h, reinterpret_cast<hipfftDoubleComplex *>(g), reinterpret_cast<hipfftDoubleComplex *>(g),
+                                   iflag > 0 ? HIPFFT_BACKWARD : HIPFFT_FORWARD);
+    if (r != HIPFFT_SUCCESS) return set_err(c, DC_ERR_DEVICE, DC_ER_LIB, "hipfftExecZ2Z failed (" + std::to_string((int)r) + ")");
+    return 0;
+}
+
+// Spread, transform, deconvolve; points/weights already on the device.
+int nufft_run(dc_ctx *c, const NuParams &p, long long M, const double *dx, const double *dy, const double *dz,
+              const double2 *dc, int iflag, const long long sub[3], const long long out[3], double2 *dfk) {
+    NufftState &s = nu_state(c);
+    const long long ng = p.n[0] * p.n[1] * p.n[2];
+    int st = ensure(c, s.grid, sizeof(double2) * (size_t)ng);
+    if (st) return st;
+    const double *corr[3];
+    hipfftHandle h;
+    if ((st = nu_setup(c, p, corr, h))) return st;
+    double *g = ptr<double>(s.grid);
+    zero_kernel<<<grid_for(2 * ng), 256, 0, c->st>>>(2 * ng, g);
+    const SpreadArgs a = spread_args(p);
+    if (M > 0) {
+        if (p.dim == 1)
+            launch_spread<1>(p.w, grid_for(M), c->st, a, M, dx, dy, dz, dc, g);
+        else if (p.dim == 2)
+            launch_spread<2>(p.w, grid_for(M), c->st, a, M, dx, dy, dz, dc, g);
+        else
+            launch_spread<3>(p.w, grid_for(M), c->st, a, M, dx, dy, dz, dc, g);
+    }
+    HIPCHK(hipGetLastError());
+    if ((st = nu_exec(c, h, g, iflag))) return st;
+    const DeconvArgs da = deconv_args(p, sub, out);
     deconv_kernel<<<grid_for(out[0] * out[1] * out[2]), 256, 0, c->st>>>(da, reinterpret_cast<const double2 *>(g), corr[0], corr[1],
                                                                           corr[2], dfk);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// Two 3D transforms of real constant strengths (sa on the points xa, sb on
+// xb) as one complex transform: A spread into the real part, B into the
+// imaginary part, separated in the deconvolution.  mb = 0: A alone (plain
+// deconvolution).
+int nufft_run_real_pair(dc_ctx *c, const NuParams &p, long long ma, const double *const xa[3], double sa, long long mb,
+                        const double *const xb[3], double sb, int iflag, const long long sub[3], const long long out[3],
+                        double2 *fa, double2 *fb) {
+    NufftState &s = nu_state(c);
+    const long long ng = p.n[0] * p.n[1] * p.n[2];
+    int st = ensure(c, s.grid, sizeof(double2) * (size_t)ng);
+    if (st) return st;
+    const double *corr[3];
+    hipfftHandle h;
+    if ((st = nu_setup(c, p, corr, h))) return st;
+    double *g = ptr<double>(s.grid);
+    zero_kernel<<<grid_for(2 * ng), 256, 0, c->st>>>(2 * ng, g);
+    const SpreadArgs a = spread_args(p);
+    if (ma > 0) launch_spread_real(p.w, c->st, a, ma, xa[0], xa[1], xa[2], sa, 0, g);
+    if (mb > 0) launch_spread_real(p.w, c->st, a, mb, xb[0], xb[1], xb[2], sb, 1, g);
+    HIPCHK(hipGetLastError());
+    if ((st = nu_exec(c, h, g, iflag))) return st;
+    const DeconvArgs da = deconv_args(p, sub, out);
+    const int nb = grid_for(out[0] * out[1] * out[2]);
+    if (mb > 0)
+        deconv_pair_kernel<<<nb, 256, 0, c->st>>>(da, reinterpret_cast<const double2 *>(g), corr[0], corr[1], corr[2], fa, fb);
+    else
+        deconv_kernel<<<nb, 256, 0, c->st>>>(da, reinterpret_cast<const double2 *>(g), corr[0], corr[1], corr[2], fa);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -714,26 +862,48 @@ int dc_four_run_nufft_dev(dc_ctx *c, const dc_run_desc *r, const double *type_oc
     const long long out[3] = {r->inc[0], r->inc[1], r->inc[2]};
     bool first = true;
     double pairs = 0.0;
-    for (int t = 0; t < c->nscat; ++t) {
-        const long long nat = c->type_beg[t + 1] - c->type_beg[t];
-        if (nat == 0) continue;
+    // the types with atoms, two per transform (real strengths cr_occ: one in
+    // each component of the complex grid)
+    std::vector<int> types;
+    for (int t = 0; t < c->nscat; ++t)
+        if (c->type_beg[t + 1] > c->type_beg[t]) types.push_back(t);
+    if ((st = ensure(c, s.fk2, sizeof(double2) * (size_t)ntot))) return st;
+    for (size_t q = 0; q < types.size(); q += 2) {
         if (cancelled(c)) return set_err(c, DC_ERR_CANCEL, DC_ER_COMM, "cancelled");
-        for (int d = 0; d < 3; ++d)
-            if ((st = ensure(c, s.pts[d], sizeof(double) * (size_t)nat))) return st;
-        if ((st = ensure(c, s.c, sizeof(double2) * (size_t)nat))) return st;
-        gather_scaled_kernel<<<grid_for(nat), 256, 0, c->st>>>(nat, ptr<int>(c->type_idx) + c->type_beg[t], ptr<double>(c->pos),
-                                                             fac[0], fac[1], fac[2], ptr<double>(s.pts[0]), ptr<double>(s.pts[1]),
-                                                             ptr<double>(s.pts[2]));
-        occ_weights_kernel<<<grid_for(nat), 256, 0, c->st>>>(nat, type_occ ? type_occ[t] : 1.0, ptr<double2>(s.c));
+        const int ta = types[q], tb = q + 1 < types.size() ? types[q + 1] : -1;
+        const long long na = c->type_beg[ta + 1] - c->type_beg[ta];
+        const long long nb = tb >= 0 ? c->type_beg[tb + 1] - c->type_beg[tb] : 0;
+        const double *xa[3], *xb[3] = {nullptr, nullptr, nullptr};
+        for (int d = 0; d < 3; ++d) {
+            if ((st = ensure(c, s.pts[d], sizeof(double) * (size_t)na))) return st;
+            xa[d] = ptr<double>(s.pts[d]);
+        }
+        gather_scaled_kernel<<<grid_for(na), 256, 0, c->st>>>(na, ptr<int>(c->type_idx) + c->type_beg[ta], ptr<double>(c->pos),
+                                                            fac[0], fac[1], fac[2], ptr<double>(s.pts[0]), ptr<double>(s.pts[1]),
+                                                            ptr<double>(s.pts[2]));
+        if (nb > 0) {
+            for (int d = 0; d < 3; ++d) {
+                if ((st = ensure(c, s.pts2[d], sizeof(double) * (size_t)nb))) return st;
+                xb[d] = ptr<double>(s.pts2[d]);
+            }
+            gather_scaled_kernel<<<grid_for(nb), 256, 0, c->st>>>(nb, ptr<int>(c->type_idx) + c->type_beg[tb], ptr<double>(c->pos),
+                                                                fac[0], fac[1], fac[2], ptr<double>(s.pts2[0]),
+                                                                ptr<double>(s.pts2[1]), ptr<double>(s.pts2[2]));
+        }
         HIPCHK(hipGetLastError());
-        if ((st = nufft_run(c, p, nat, ptr<double>(s.pts[0]), ptr<double>(s.pts[1]), ptr<double>(s.pts[2]), ptr<double2>(s.c), 1,
-                            sub, out, ptr<double2>(s.fk))))
+        const double sa = type_occ ? type_occ[ta] : 1.0, sb = (tb >= 0 && type_occ) ? type_occ[tb] : 1.0;
+        if ((st = nufft_run_real_pair(c, p, na, xa, sa, nb, xb, sb, 1, sub, out, ptr<double2>(s.fk), ptr<double2>(s.fk2))))
             return st;
-        form_acc_kernel<<<grid_for(ntot), 256, 0, c->st>>>(g, t, r->ncfact, first ? 1 : 0, ptr<double2>(c->cfact),
+        form_acc_kernel<<<grid_for(ntot), 256, 0, c->st>>>(g, ta, r->ncfact, first ? 1 : 0, ptr<double2>(c->cfact),
                                                           ptr<double2>(s.fk), acc);
-        HIPCHK(hipGetLastError());
         first = false;
-        pairs += (double)nat * (double)ntot;
+        pairs += (double)na * (double)ntot;
+        if (nb > 0) {
+            form_acc_kernel<<<grid_for(ntot), 256, 0, c->st>>>(g, tb, r->ncfact, 0, ptr<double2>(c->cfact), ptr<double2>(s.fk2),
+                                                              acc);
+            pairs += (double)nb * (double)ntot;
+        }
+        HIPCHK(hipGetLastError());
     }
     if (first) zero_kernel<<<grid_for(2 * ntot), 256, 0, c->st>>>(2 * ntot, reinterpret_cast<double *>(acc));
     if (r->fave > 0.0) {

@@ -157,3 +157,27 @@ def test_run_nufft_errors(ctx):
     with pytest.raises(_lib.DCError) as e:
         ctx.four_run_nufft(desc)
     assert e.value.ier_num == _lib.DC_ERR_ARG
+
+
+@pytest.mark.parametrize("case", ["fcc 3 types", "ortho 4 types", "ortho, type 2 empty"])
+def test_run_nufft_type_pairs(ctx, case):
+    """The run level transforms two types at a time (real strengths, one in
+    each component of the complex grid, separated by Hermitian symmetry):
+    3 types (a pair and a single), 4 types (two pairs), and 4 types whose
+    type 2 has no atoms.  Distinct type_occ per type catch a swapped
+    component: the reference is the exact run with each type's form-factor
+    column scaled by its type_occ."""
+    import dataclasses
+    print(f"case run_nufft pairs {case}", flush=True)
+    if case.startswith("fcc"):
+        cr = X.fcc_ordered(n=6, ntypes=3, mc_steps=3000, seed=4)
+    else:
+        cr = X.orthorhombic(ncell=(6, 6, 6), seed=6)
+        if "empty" in case:
+            cr = dataclasses.replace(cr, iscat=np.where(cr.iscat == 2, 0, cr.iscat).astype(np.int32))
+    occ = np.array([0.3, 0.7, 0.9, 0.55][:cr.nscat])
+    got, (eck, vi, inc, cf) = _run(ctx, cr, (-2.0,) * 3, (2.0,) * 3, (13, 13, 13), type_occ=occ)
+    ref = O.four_run(cr, eck, vi, inc, cf * occ[None, :], four_friedel=False)
+    err = float(np.abs(got["dsi"] - ref["dsi"]).max() / ref["dsi"].max())
+    print(f"  {case}: max|dI|/max(I) {err:.2e}")
+    assert err <= 1e-6
