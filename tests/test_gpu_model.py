@@ -271,3 +271,27 @@ def test_model_jvp_vs_central_differences_and_vjp(ctx, prec, bar):
         print(f"  {prec} direction {k}: max|dI - fd|/max|fd| = {e:.2e}, duality {dual:.2e}")
         assert e <= bar
         assert dual <= bar
+
+
+@pytest.mark.parametrize("prec,bar", [("fp16x3", 1e-5), ("bf16x3", 1e-5)])
+@pytest.mark.parametrize("scale", [1e-6, 1.0, 1e4, "mixed"])
+def test_model_forward_weight_range(ctx, prec, bar, scale):
+    """The FP16 schemes scale each slice's B amplitudes (the per-atom weights
+    occ) by a power of two into [0.5, 1) and undo it in the epilogue: weights
+    of 1e-6 or 1e4, beyond FP16's normal range, and weights spanning four
+    decades inside one slice keep the split bar against the FP64 oracle."""
+    cr, occ, bdw, cf, eck, vi, inc = problem(n=(17, 9, 5))
+    if scale == "mixed":
+        occ = occ * np.logspace(-4, 0, occ.size)[np.random.default_rng(3).permutation(occ.size)]
+    else:
+        occ = occ * scale
+    ctx.set_precision(prec)
+    try:
+        d, keep = ctx.model_desc(cr.pos, cr.iscat, occ, bdw, cf, eck, vi, inc, cr.rten())
+        icalc, fcalc = ctx.model_intensity(d)
+    finally:
+        ctx.set_precision("fp32")
+    F = O.model_f(cr.pos, cr.iscat, occ, bdw, cf, eck, vi, inc, cr.rten())
+    e = rel_i(icalc, np.abs(F) ** 2)
+    print(f"  {prec} occ x {scale}: max|dI|/max(I) = {e:.3e}")
+    assert e <= bar
