@@ -200,8 +200,9 @@ def bench_nufft(_lib, torch, c, steps=3):
             "tol": 1e-7, "max_rel_err_sampled": None,
             "_samples": {0.0: dsi[::4, ::4, 80].copy(), 1.0: dsi[::4, ::4, 100].copy()},
             "note": "four_run_nufft on the headline workload: types in pairs, real spread of each type "
-                    "(exponential-of-semicircle, w=8) into one component of a 324^3 complex grid, hipFFT Z2Z, "
-                    "Hermitian separation and deconvolution, form factor; sum over types and |F|^2; FP64"}
+                    "(exponential-of-semicircle, w=8) into one component of a 324^3 complex grid, pruned 3D transform "
+                    "(hipFFT 1D passes, gather-transposes of the needed modes), Hermitian separation and deconvolution, "
+                    "form factor; sum over types and |F|^2; FP64"}
 
 
 def bench_post(_lib, FR, torch, steps=5):

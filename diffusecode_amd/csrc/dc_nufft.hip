@@ -18,6 +18,7 @@
 #include <hipfft/hipfft.h>
 
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <cmath>
 #include <map>
@@ -311,26 +312,92 @@ __global__ void deconv_kernel(DeconvArgs a, const double2 *__restrict__ B, const
     }
 }
 
-// Two real grids transformed as one complex grid Z = FFT(gA + i gB): the
-// Hermitian symmetry of each part separates them,
-//   GA(k) = (Z(k) + conj Z(-k)) / 2,   GB(k) = (Z(k) - conj Z(-k)) / (2 i),
-// then the same correction as deconv_kernel (phihat is even in k).
-__global__ void deconv_pair_kernel(DeconvArgs a, const double2 *__restrict__ Z, const double *__restrict__ c1,
-                                   const double *__restrict__ c2, const double *__restrict__ c3, double2 *__restrict__ fa,
-                                   double2 *__restrict__ fb) {
+// Pruned 3D transform.  After the full FFT along x, only the x columns the
+// output needs (list L0, c0 of them) go on: tr_xy gathers them and
+// transposes so that y is contiguous, (y, i, z); after the y pass tr_yz
+// gathers the needed y rows (L1, c1) with z contiguous, (z, j, i), for the z
+// pass.  Tiles of 32 x 32 through LDS keep both sides coalesced.
+constexpr int TR = 32, TRY = 8;
+
+// out[y + n1 (i + c0 z)] = g[L0[i] + n0 (y + n1 z)]
+__global__ __launch_bounds__(TR *TRY) void tr_xy_kernel(const double2 *__restrict__ g, const int *__restrict__ L0,
+                                                         long long n0, long long n1, long long n2, int c0,
+                                                         double2 *__restrict__ out) {
+    __shared__ double2 tile[TR][TR + 1];
+    const int i0 = blockIdx.x * TR, y0 = blockIdx.y * TR;
+    const long long z = blockIdx.z;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i = i0 + tx;
+    const int xi = i < c0 ? L0[i] : 0;
+#pragma unroll
+    for (int r = 0; r < TR; r += TRY) {
+        const long long y = y0 + ty + r;
+        if (i < c0 && y < n1) tile[ty + r][tx] = g[xi + n0 * (y + n1 * z)];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < TR; r += TRY) {
+        const int ii = i0 + ty + r;
+        const long long y = y0 + tx;
+        if (ii < c0 && y < n1) out[y + n1 * (ii + (long long)c0 * z)] = tile[tx][ty + r];
+    }
+}
+
+// out[z + n2 (j + c1 i)] = in[L1[j] + n1 (i + c0 z)]
+__global__ __launch_bounds__(TR *TRY) void tr_yz_kernel(const double2 *__restrict__ in, const int *__restrict__ L1,
+                                                         long long n1, long long n2, int c0, int c1,
+                                                         double2 *__restrict__ out) {
+    __shared__ double2 tile[TR][TR + 1];
+    const int j0 = blockIdx.x * TR;
+    const long long z0 = (long long)blockIdx.y * TR;
+    const long long i = blockIdx.z;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int j = j0 + tx;
+    const int yj = j < c1 ? L1[j] : 0;
+#pragma unroll
+    for (int r = 0; r < TR; r += TRY) {
+        const long long z = z0 + ty + r;
+        if (j < c1 && z < n2) tile[ty + r][tx] = in[yj + n1 * (i + (long long)c0 * z)];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < TR; r += TRY) {
+        const int jj = j0 + ty + r;
+        const long long z = z0 + tx;
+        if (jj < c1 && z < n2) out[z + n2 * (jj + (long long)c1 * i)] = tile[tx][ty + r];
+    }
+}
+
+// Output modes of a 3D transform: the value of mode (o1, o2, o3) sits at
+// Z[P0[o1] s0 + P1[o2] s1 + P2[o3] s2] and its mirror -k at the M tables
+// (pruned layout (z, j, i): strides n2 c1, n2, 1 on compact positions; full
+// grid: 1, n0, n0 n1 on fine indices).
+struct PruneArgs {
+    long long out[3];
+    long long s[3];
+    const int *P[3], *M[3];
+    const double *corr[3];
+    long long sub[3];
+};
+
+__global__ void deconv3_kernel(PruneArgs a, const double2 *__restrict__ Z, double2 *__restrict__ fa,
+                               double2 *__restrict__ fb) {
     const long long total = a.out[0] * a.out[1] * a.out[2];
     for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long long)gridDim.x * blockDim.x) {
         const long long o1 = e % a.out[0];
         const long long o2 = (e / a.out[0]) % a.out[1];
         const long long o3 = e / (a.out[0] * a.out[1]);
-        const long long m1 = o1 * a.sub[0], m2 = o2 * a.sub[1], m3 = o3 * a.sub[2];
-        const long long k1 = m1 - a.N[0] / 2, k2 = m2 - a.N[1] / 2, k3 = m3 - a.N[2] / 2;
-        const long long gp = wrapi(k1, a.n[0]) + a.n[0] * (wrapi(k2, a.n[1]) + a.n[1] * wrapi(k3, a.n[2]));
-        const long long gm = wrapi(-k1, a.n[0]) + a.n[0] * (wrapi(-k2, a.n[1]) + a.n[1] * wrapi(-k3, a.n[2]));
-        const double f = 0.5 * (c1[m1] * c2[m2] * c3[m3]);
-        const double2 zp = Z[gp], zm = Z[gm];
-        fa[e] = make_double2((zp.x + zm.x) * f, (zp.y - zm.y) * f);
-        fb[e] = make_double2((zp.y + zm.y) * f, (zm.x - zp.x) * f);
+        const double f = a.corr[0][o1 * a.sub[0]] * a.corr[1][o2 * a.sub[1]] * a.corr[2][o3 * a.sub[2]];
+        const double2 zp = Z[a.P[0][o1] * a.s[0] + a.P[1][o2] * a.s[1] + a.P[2][o3] * a.s[2]];
+        if (!fb) {
+            fa[e] = make_double2(zp.x * f, zp.y * f);
+        } else {
+            // two real grids in one complex transform: Hermitian separation
+            const double2 zm = Z[a.M[0][o1] * a.s[0] + a.M[1][o2] * a.s[1] + a.M[2][o3] * a.s[2]];
+            const double h = 0.5 * f;
+            fa[e] = make_double2((zp.x + zm.x) * h, (zp.y - zm.y) * h);
+            fb[e] = make_double2((zp.y + zm.y) * h, (zm.x - zp.x) * h);
+        }
     }
 }
 
@@ -414,8 +481,20 @@ std::vector<double> correction(long long N, long long n, int w, double beta) {
 
 }  // namespace
 
+// Index tables of the pruned 3D transform for one (grid, modes, subsampling)
+// shape: kept fine indices along x and y (the output modes k and their
+// mirrors -k) and, per output mode, the positions of k and -k.
+struct PruneTables {
+    int c0 = 0, c1 = 0;
+    DevBuf L0, L1, P[3], M[3];
+    std::vector<std::vector<int>> host;   // sources of the uploads, kept alive
+};
+
 struct NufftState {
     std::map<std::tuple<long long, long long, long long>, hipfftHandle> plans;
+    std::map<std::pair<long long, long long>, hipfftHandle> plans1d;   // (length, batch): contiguous lines
+    std::map<std::array<long long, 13>, PruneTables> prune;
+    DevBuf work;                                                     // the y pass of the pruned transform
     std::map<std::tuple<long long, long long, int>, DevBuf> corr;   // (N, n, w) -> 1 / phihat on the device
     std::vector<std::vector<double>> corr_host;                      // sources of the uploads, kept alive
     DevBuf grid, pts[3], c, fk, one;
@@ -426,7 +505,14 @@ namespace dc {
 void nufft_release(dc_ctx *c) {
     if (!c->nufft) return;
     for (auto &kv : c->nufft->plans) (void)hipfftDestroy(kv.second);
+    for (auto &kv : c->nufft->plans1d) (void)hipfftDestroy(kv.second);
     NufftState *s = c->nufft;
+    for (auto &kv : s->prune) {
+        PruneTables &t = kv.second;
+        for (DevBuf *b : {&t.L0, &t.L1, &t.P[0], &t.P[1], &t.P[2], &t.M[0], &t.M[1], &t.M[2]})
+            if (b->p) (void)hipFree(b->p);
+    }
+    if (s->work.p) (void)hipFree(s->work.p);
     for (DevBuf *b : {&s->grid, &s->pts[0], &s->pts[1], &s->pts[2], &s->c, &s->fk, &s->one, &s->pts2[0], &s->pts2[1],
                       &s->pts2[2], &s->fk2})
         if (b->p) (void)hipFree(b->p);
@@ -463,7 +549,7 @@ int nu_params(dc_ctx *c, int dim, double tol, const long long N[3], NuParams &p)
 
 // Correction tables 1/phihat per axis (cached per (N, n, w)) and the FFT plan
 // of the fine grid (cached per grid), on the context's stream.
-int nu_setup(dc_ctx *c, const NuParams &p, const double *corr[3], hipfftHandle &h) {
+int nu_setup(dc_ctx *c, const NuParams &p, const double *corr[3], hipfftHandle &h, bool need_plan) {
     NufftState &s = nu_state(c);
     int st;
     for (int d = 0; d < 3; ++d) {
@@ -477,6 +563,7 @@ int nu_setup(dc_ctx *c, const NuParams &p, const double *corr[3], hipfftHandle &
         }
         corr[d] = reinterpret_cast<const double *>(it->second.p);
     }
+    if (!need_plan) return 0;
     // fine-grid transform (row-major dims: slowest first)
     auto key = std::make_tuple(p.n[0], p.n[1], p.n[2]);
     auto it = s.plans.find(key);
@@ -525,6 +612,123 @@ int nu_exec(dc_ctx *c, hipfftHandle h, double *g, int iflag) {
     return 0;
 }
 
+int plan1d(dc_ctx *c, long long n, long long batch, hipfftHandle &h) {
+    NufftState &s = nu_state(c);
+    auto key = std::make_pair(n, batch);
+    auto it = s.plans1d.find(key);
+    if (it == s.plans1d.end()) {
+        int nn = (int)n;
+        hipfftResult r = hipfftPlanMany(&h, 1, &nn, nullptr, 1, nn, nullptr, 1, nn, HIPFFT_Z2Z, (int)batch);
+        if (r != HIPFFT_SUCCESS) return set_err(c, DC_ERR_DEVICE, DC_ER_LIB, "hipfftPlanMany failed (" + std::to_string((int)r) + ")");
+        s.plans1d[key] = h;
+    } else {
+        h = it->second;
+    }
+    if (hipfftSetStream(h, c->st) != HIPFFT_SUCCESS) return set_err(c, DC_ERR_DEVICE, DC_ER_LIB, "hipfftSetStream failed");
+    return 0;
+}
+
+int prune_tables(dc_ctx *c, const NuParams &p, const long long sub[3], const long long out[3], bool compact,
+                 PruneTables *&tp) {
+    NufftState &s = nu_state(c);
+    const std::array<long long, 13> key = {p.n[0], p.n[1], p.n[2], p.N[0], p.N[1], p.N[2],
+                                           sub[0], sub[1], sub[2], out[0], out[1], out[2], compact ? 1 : 0};
+    auto it = s.prune.find(key);
+    if (it != s.prune.end()) {
+        tp = &it->second;
+        return 0;
+    }
+    PruneTables t;
+    std::vector<int> L[2];
+    int st;
+    for (int d = 0; d < 3; ++d) {
+        std::vector<int> kp(out[d]), km(out[d]);
+        for (long long o = 0; o < out[d]; ++o) {
+            const long long k = o * sub[d] - p.N[d] / 2;
+            kp[o] = (int)(((k % p.n[d]) + p.n[d]) % p.n[d]);
+            km[o] = (int)(((-k % p.n[d]) + p.n[d]) % p.n[d]);
+        }
+        if (compact && d < 2) {
+            std::vector<int> all(kp);
+            all.insert(all.end(), km.begin(), km.end());
+            std::sort(all.begin(), all.end());
+            all.erase(std::unique(all.begin(), all.end()), all.end());
+            for (long long o = 0; o < out[d]; ++o) {
+                kp[o] = (int)(std::lower_bound(all.begin(), all.end(), kp[o]) - all.begin());
+                km[o] = (int)(std::lower_bound(all.begin(), all.end(), km[o]) - all.begin());
+            }
+            L[d] = all;
+        }
+        t.host.push_back(kp);
+        t.host.push_back(km);
+        if ((st = upload(c, t.P[d], t.host[t.host.size() - 2].data(), out[d]))) return st;
+        if ((st = upload(c, t.M[d], t.host.back().data(), out[d]))) return st;
+    }
+    t.c0 = (int)L[0].size();
+    t.c1 = (int)L[1].size();
+    t.host.push_back(L[0]);
+    if ((st = upload(c, t.L0, t.host.back().data(), L[0].size()))) return st;
+    t.host.push_back(L[1]);
+    if ((st = upload(c, t.L1, t.host.back().data(), L[1].size()))) return st;
+    tp = &s.prune.emplace(key, std::move(t)).first->second;
+    return 0;
+}
+
+// 3D fine grid g -> output modes, pruned: full FFT along x, the kept x
+// columns along y, the kept (x, y) lines along z, then the correction (and,
+// with fb, the separation of two real grids).  g is overwritten.
+int nu_transform3(dc_ctx *c, const NuParams &p, const double *const corr[3], double *g, int iflag, const long long sub[3],
+                  const long long out[3], double2 *fa, double2 *fb) {
+    NufftState &s = nu_state(c);
+    const long long n0 = p.n[0], n1 = p.n[1], n2 = p.n[2];
+    // the transposes put n2 and the kept x count (<= n0) on grid z
+    const bool prunable = n2 <= 65535 && n0 <= 65535;
+    PruneTables *t = nullptr;
+    int st = prune_tables(c, p, sub, out, prunable, t);
+    if (st) return st;
+    PruneArgs a{};
+    for (int d = 0; d < 3; ++d) {
+        a.out[d] = out[d];
+        a.sub[d] = sub[d];
+        a.P[d] = ptr<int>(t->P[d]);
+        a.M[d] = ptr<int>(t->M[d]);
+        a.corr[d] = corr[d];
+    }
+    if (!prunable) {
+        const double *cr[3];
+        hipfftHandle h;
+        if ((st = nu_setup(c, p, cr, h, true)) || (st = nu_exec(c, h, g, iflag))) return st;
+        a.s[0] = 1;
+        a.s[1] = n0;
+        a.s[2] = n0 * n1;
+        deconv3_kernel<<<grid_for(out[0] * out[1] * out[2]), 256, 0, c->st>>>(a, reinterpret_cast<const double2 *>(g), fa, fb);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+    const int c0 = t->c0, c1 = t->c1;
+    if ((st = ensure(c, s.work, sizeof(double2) * (size_t)(n1 * c0 * n2)))) return st;
+    hipfftHandle hx, hy, hz;
+    if ((st = plan1d(c, n0, n1 * n2, hx)) || (st = plan1d(c, n1, (long long)c0 * n2, hy)) ||
+        (st = plan1d(c, n2, (long long)c1 * c0, hz)))
+        return st;
+    double2 *w = ptr<double2>(s.work);
+    if ((st = nu_exec(c, hx, g, iflag))) return st;
+    tr_xy_kernel<<<dim3((c0 + TR - 1) / TR, (unsigned)((n1 + TR - 1) / TR), (unsigned)n2), dim3(TR, TRY), 0, c->st>>>(
+        reinterpret_cast<const double2 *>(g), ptr<int>(t->L0), n0, n1, n2, c0, w);
+    HIPCHK(hipGetLastError());
+    if ((st = nu_exec(c, hy, reinterpret_cast<double *>(w), iflag))) return st;
+    tr_yz_kernel<<<dim3((c1 + TR - 1) / TR, (unsigned)((n2 + TR - 1) / TR), (unsigned)c0), dim3(TR, TRY), 0, c->st>>>(
+        w, ptr<int>(t->L1), n1, n2, c0, c1, reinterpret_cast<double2 *>(g));
+    HIPCHK(hipGetLastError());
+    if ((st = nu_exec(c, hz, g, iflag))) return st;
+    a.s[0] = n2 * c1;
+    a.s[1] = n2;
+    a.s[2] = 1;
+    deconv3_kernel<<<grid_for(out[0] * out[1] * out[2]), 256, 0, c->st>>>(a, reinterpret_cast<const double2 *>(g), fa, fb);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 // Spread, transform, deconvolve; points/weights already on the device.
 int nufft_run(dc_ctx *c, const NuParams &p, long long M, const double *dx, const double *dy, const double *dz,
               const double2 *dc, int iflag, const long long sub[3], const long long out[3], double2 *dfk) {
@@ -534,7 +738,7 @@ int nufft_run(dc_ctx *c, const NuParams &p, long long M, const double *dx, const
     if (st) return st;
     const double *corr[3];
     hipfftHandle h;
-    if ((st = nu_setup(c, p, corr, h))) return st;
+    if ((st = nu_setup(c, p, corr, h, p.dim < 3))) return st;
     double *g = ptr<double>(s.grid);
     zero_kernel<<<grid_for(2 * ng), 256, 0, c->st>>>(2 * ng, g);
     const SpreadArgs a = spread_args(p);
@@ -547,6 +751,7 @@ int nufft_run(dc_ctx *c, const NuParams &p, long long M, const double *dx, const
             launch_spread<3>(p.w, grid_for(M), c->st, a, M, dx, dy, dz, dc, g);
     }
     HIPCHK(hipGetLastError());
+    if (p.dim == 3) return nu_transform3(c, p, corr, g, iflag, sub, out, dfk, nullptr);
     if ((st = nu_exec(c, h, g, iflag))) return st;
     const DeconvArgs da = deconv_args(p, sub, out);
     deconv_kernel<<<grid_for(out[0] * out[1] * out[2]), 256, 0, c->st>>>(da, reinterpret_cast<const double2 *>(g), corr[0], corr[1],
@@ -568,22 +773,14 @@ int nufft_run_real_pair(dc_ctx *c, const NuParams &p, long long ma, const double
     if (st) return st;
     const double *corr[3];
     hipfftHandle h;
-    if ((st = nu_setup(c, p, corr, h))) return st;
+    if ((st = nu_setup(c, p, corr, h, false))) return st;
     double *g = ptr<double>(s.grid);
     zero_kernel<<<grid_for(2 * ng), 256, 0, c->st>>>(2 * ng, g);
     const SpreadArgs a = spread_args(p);
     if (ma > 0) launch_spread_real(p.w, c->st, a, ma, xa[0], xa[1], xa[2], sa, 0, g);
     if (mb > 0) launch_spread_real(p.w, c->st, a, mb, xb[0], xb[1], xb[2], sb, 1, g);
     HIPCHK(hipGetLastError());
-    if ((st = nu_exec(c, h, g, iflag))) return st;
-    const DeconvArgs da = deconv_args(p, sub, out);
-    const int nb = grid_for(out[0] * out[1] * out[2]);
-    if (mb > 0)
-        deconv_pair_kernel<<<nb, 256, 0, c->st>>>(da, reinterpret_cast<const double2 *>(g), corr[0], corr[1], corr[2], fa, fb);
-    else
-        deconv_kernel<<<nb, 256, 0, c->st>>>(da, reinterpret_cast<const double2 *>(g), corr[0], corr[1], corr[2], fa);
-    HIPCHK(hipGetLastError());
-    return 0;
+    return nu_transform3(c, p, corr, g, iflag, sub, out, fa, mb > 0 ? fb : nullptr);
 }
 
 int check_points(dc_ctx *c, int dim, long long M, const double *x, const double *y, const double *z) {
