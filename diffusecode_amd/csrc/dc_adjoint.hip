@@ -529,6 +529,44 @@ __global__ void __launch_bounds__(NT) adjoint_split_kernel(AdjArgs a) {
 constexpr int RES_ST = 5;
 constexpr int RES_MAXCT = 32;   // column tiles per work-group at most (bounds the FP32 column sums)
 
+// Epilogue of one column tile of a walk: column tables, contraction, and the
+// row totals added into the lane's running sums tot (entries 3 li .. 3 li + 2).
+__device__ __forceinline__ void walk_tile_epilogue(const AdjArgs &a, const float *__restrict__ sq, const AtomRec *at,
+                                                   ColTables &ct, const f32x4 (&accR)[3][WA][WC],
+                                                   const f32x4 (&accI)[3][WA][WC], int col0, float (&tot)[3]) {
+    const int li = threadIdx.x & 15;
+    __syncthreads();   // the previous tile's contraction has read the column tables
+    epi_tables(a, at, ct, col0);
+    __syncthreads();
+    float acc[WA][4][6];
+#pragma unroll
+    for (int i = 0; i < WA; ++i)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) acc[i][v][k] = 0.0f;
+    epi_contract(a, sq, at, ct, accR, accI, col0, acc);
+    float part[3];
+    row_reduce_scatter(acc, part, li);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) tot[q] += part[q];
+}
+
+// End of a walk: the lanes' totals into the per-wave slots, summed over the
+// waves in order and written as partial set blockIdx.x.
+__device__ __forceinline__ void walk_finish(const AdjArgs &a, float (&red)[NWC][NA][6], const float (&tot)[3], int atom0,
+                                            int na) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
+    const int wa0 = (wave % NWA) * 16 * WA;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const int f = 3 * li + q, i = f / 24, v = (f / 6) % 4, k = f % 6;
+        red[wave / NWA][wa0 + 16 * i + 4 * lg + v][k] = tot[q];
+    }
+    __syncthreads();
+    epi_write(a, red, blockIdx.x, atom0, na);
+}
+
 struct AdSmemR {
     StageA a[RES_ST];
     AtomRec at[NA];
@@ -564,7 +602,6 @@ __global__ void __launch_bounds__(NT) adjoint_split_res_kernel(AdjArgs a) {
     if (ct0 < ct1) load_bwords(qh, ql, a, ct0 * NC + wc0, 0, bh, bl);
     __syncthreads();
 
-    const int li = threadIdx.x & 15, lg = (threadIdx.x & 63) >> 4;
     float tot[3] = {0.0f, 0.0f, 0.0f};   // row totals of flat entries 3 li .. 3 li + 2, over the column range
     for (int ct = ct0; ct < ct1; ++ct) {
         f32x4 accR[3][WA][WC], accI[3][WA][WC];
@@ -591,29 +628,118 @@ __global__ void __launch_bounds__(NT) adjoint_split_res_kernel(AdjArgs a) {
                 }
             }
         }
-        __syncthreads();   // the previous tile's contraction has read the column tables
-        epi_tables(a, sm.at, sm.ct, ct * NC);
-        __syncthreads();
-        float acc[WA][4][6];
-#pragma unroll
-        for (int i = 0; i < WA; ++i)
-#pragma unroll
-            for (int v = 0; v < 4; ++v)
-#pragma unroll
-                for (int k = 0; k < 6; ++k) acc[i][v][k] = 0.0f;
-        epi_contract(a, sq, sm.at, sm.ct, accR, accI, ct * NC, acc);
-        float part[3];
-        row_reduce_scatter(acc, part, li);
-#pragma unroll
-        for (int q = 0; q < 3; ++q) tot[q] += part[q];
+        walk_tile_epilogue(a, sq, sm.at, sm.ct, accR, accI, ct * NC, tot);
     }
+    walk_finish(a, sm.red, tot, atom0, na);
+}
+
+// FP32 adjoint for short grids, the same walk: the FP32 A elements of every
+// stage (20 KB for 5 stages) stay in LDS, and each lane loads its Q operands
+// (row 2 ks + lg / 2 of the stage, column li of each 16-column tile) from L2
+// into registers: each k-step's slot is refilled with the next (column tile,
+// stage) as soon as its MFMAs have read it, a stage ahead of use.
+struct AdSmemRF {
+    float a[RES_ST][KP][NA];   // A(k', atom), k' = 2 rr + (0 re, 1 im)
+    float sr[RES_ST][KR];
+    AtomRec at[NA];
+    float red[NWC][NA][6];
+    ColTables ct;
+    RowTablesT<RES_ST> rt;
+};
+
+__device__ __forceinline__ void load_qslot(const AdjArgs &a, long long qbase, int cw, int st, int ks, float2 (&q)[WC]) {
+    const int lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        const int f = 3 * li + q, i = f / 24, v = (f / 6) % 4, k = f % 6;
-        sm.red[wave / NWA][wa0 + 16 * i + 4 * lg + v][k] = tot[q];
+    for (int j = 0; j < WC; ++j) {
+        const int r = st * KR + 2 * ks + (lg >> 1), col = cw + 16 * j + li;
+        q[j] = (r < a.R && col < a.C) ? a.q[qbase + col + (long long)a.C * r] : make_float2(0.f, 0.f);
+    }
+}
+
+__global__ void __launch_bounds__(NT) adjoint_res_kernel(AdjArgs a) {
+    __shared__ AdSmemRF sm;
+    const int4 tile = a.tiles[blockIdx.y];
+    const int atom0 = tile.x, na = tile.y - tile.x;
+    const long long qbase = (long long)tile.z * a.plane;
+    const float *sq = a.sq + (long long)tile.w * a.sq_stride;
+    const int ntc = (a.C + NC - 1) / NC;
+    const int ct0 = blockIdx.x * a.ctpg, ct1 = min(ntc, ct0 + a.ctpg);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
+    const int wa0 = (wave % NWA) * 16 * WA, wc0 = (wave / NWA) * 16 * WC;
+    const bool im = (lg & 1) != 0;
+
+    for (int x = threadIdx.x; x < NA; x += NT) {
+        if (x < na)
+            sm.at[x] = a.rec[atom0 + x];
+        else
+            sm.at[x] = AtomRec{};
     }
     __syncthreads();
-    epi_write(a, sm.red, blockIdx.x, atom0, na);
+    const int nst = (a.R + KR - 1) / KR;   // <= RES_ST, checked by the launcher
+    build_row_tables(sm.rt, sm.at, nst);
+    __syncthreads();
+    for (int s = 0; s < nst; ++s) {
+        const int t = threadIdx.x, at = t & (NA - 1), rr = t / NA;   // NT == NA * KR
+        const int r = s * KR + rr;
+        float2 v = make_float2(0.f, 0.f);
+        if (r < a.R && at < na) v = row_element(sm.rt, sm.at[at], at, s, rr, r, true, sq[r]);
+        sm.a[s][2 * rr][at] = v.x;
+        sm.a[s][2 * rr + 1][at] = v.y;
+        if (t < KR) sm.sr[s][t] = s * KR + t < a.R ? sq[s * KR + t] : 0.0f;
+    }
+    float2 qr[KR / 2][WC];
+    if (ct0 < ct1)
+#pragma unroll
+        for (int ks = 0; ks < KR / 2; ++ks) load_qslot(a, qbase, ct0 * NC + wc0, 0, ks, qr[ks]);
+    __syncthreads();
+
+    float tot[3] = {0.0f, 0.0f, 0.0f};
+    for (int ct = ct0; ct < ct1; ++ct) {
+        f32x4 accR[3][WA][WC], accI[3][WA][WC];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int i = 0; i < WA; ++i)
+#pragma unroll
+                for (int j = 0; j < WC; ++j) {
+                    accR[k][i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+                    accI[k][i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+                }
+        for (int s = 0; s < nst; ++s) {
+            const int nct = s + 1 < nst ? ct : ct + 1, ns = s + 1 < nst ? s + 1 : 0;
+            const bool more = nct < ct1;
+#pragma unroll
+            for (int ks = 0; ks < KR / 2; ++ks) {
+                const int rr = 2 * ks + (lg >> 1);
+                const float wr = (float)(s * KR + rr) - a.rc;
+                const float ws = sm.sr[s][rr];
+                float av[WA][3];
+#pragma unroll
+                for (int i = 0; i < WA; ++i) {
+                    const float x = sm.a[s][4 * ks + lg][wa0 + 16 * i + li];
+                    av[i][0] = x;
+                    av[i][1] = x * wr;
+                    av[i][2] = x * ws;
+                }
+#pragma unroll
+                for (int j = 0; j < WC; ++j) {
+                    const float2 b = qr[ks][j];
+                    const float bR = im ? -b.y : b.x;
+                    const float bI = im ? b.x : b.y;
+#pragma unroll
+                    for (int i = 0; i < WA; ++i)
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            accR[k][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i][k], bR, accR[k][i][j], 0, 0, 0);
+                            accI[k][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i][k], bI, accI[k][i][j], 0, 0, 0);
+                        }
+                }
+                if (more) load_qslot(a, qbase, nct * NC + wc0, ns, ks, qr[ks]);
+            }
+        }
+        walk_tile_epilogue(a, sq, sm.at, sm.ct, accR, accI, ct * NC, tot);
+    }
+    walk_finish(a, sm.red, tot, atom0, na);
 }
 
 // FP32 complex Q planes -> split words: for plane t, part (hi, lo), row
@@ -820,7 +946,7 @@ hipError_t launch_fill_zero(double *p, long long n, hipStream_t st) {
 int adjoint_tile_atoms() { return NA; }
 int adjoint_tile_cols() { return NC; }
 
-// Short grids (split scheme, R <= 16 RES_ST) walk column ranges with A
+// Short grids (R <= 16 RES_ST, every scheme) walk column ranges with A
 // resident in LDS; one such work-group fills a CU, so the walk length is
 // chosen to balance rounds of work-groups over the CUs against the per-group
 // prologue (about half a column tile of work).
@@ -829,7 +955,8 @@ int adjoint_max_walk() { return RES_MAXCT; }
 int adjoint_plan(AdjArgs &a, int ntiles, bool split, int force) {
     const int ntc = std::max(1, (a.C + NC - 1) / NC);
     a.ctpg = 1;
-    if (!split || ntiles <= 0 || a.R <= 0 || (a.R + SKR - 1) / SKR > RES_ST) return ntc;
+    (void)split;
+    if (ntiles <= 0 || a.R <= 0 || (a.R + SKR - 1) / SKR > RES_ST) return ntc;
     if (force > 0) {
         a.ctpg = std::min({force, ntc, RES_MAXCT});
         return (ntc + a.ctpg - 1) / a.ctpg;
@@ -863,19 +990,17 @@ int adjoint_plan(AdjArgs &a, int ntiles, bool split, int force) {
 hipError_t launch_adjoint(const AdjArgs &a, int ntiles, hipStream_t st, bool split) {
     if (ntiles <= 0 || a.R <= 0 || a.C <= 0) return hipSuccess;
     const int ntc = (a.C + NC - 1) / NC;
-    if (a.ctpg < 1 || (a.ctpg > 1 && !split)) return hipErrorInvalidValue;
+    if (a.ctpg < 1 || a.ctpg > RES_MAXCT || (a.ctpg > 1 && (a.R + SKR - 1) / SKR > RES_ST)) return hipErrorInvalidValue;
     dim3 grid(ntc, ntiles);
-    if (split) {
-        if (!a.qs || a.Rg * 4 < a.R || a.Rg % 4) return hipErrorInvalidValue;
-        if ((a.R + SKR - 1) / SKR <= RES_ST && a.ctpg > 1) {
-            if (a.ctpg > RES_MAXCT) return hipErrorInvalidValue;
-            grid.x = (ntc + a.ctpg - 1) / a.ctpg;
+    if (split && (!a.qs || a.Rg * 4 < a.R || a.Rg % 4)) return hipErrorInvalidValue;
+    if (a.ctpg > 1) {
+        grid.x = (ntc + a.ctpg - 1) / a.ctpg;
+        if (split)
             adjoint_split_res_kernel<<<grid, dim3(NT), 0, st>>>(a);
-        } else if (a.ctpg == 1) {
-            adjoint_split_kernel<<<grid, dim3(NT), 0, st>>>(a);
-        } else {
-            return hipErrorInvalidValue;
-        }
+        else
+            adjoint_res_kernel<<<grid, dim3(NT), 0, st>>>(a);
+    } else if (split) {
+        adjoint_split_kernel<<<grid, dim3(NT), 0, st>>>(a);
     } else {
         adjoint_kernel<<<grid, dim3(NT), 0, st>>>(a);
     }

@@ -277,7 +277,7 @@ int dc_batch_chi2(dc_ctx *ctx, const dc_batch_desc *batch, const double *target,
 /* Autotuner and hipGraph capture of batch calls (both on by default). */
 int dc_set_autotune(dc_ctx *ctx, int on);
 int dc_set_graphs(dc_ctx *ctx, int on);
-/* Column walk of the BF16x3 / FP16x2 reverse pass on grids of at most 80
+/* Column walk of the reverse pass (every precision) on grids of at most 80
  * abscissa points (dc_model_vjp, dc_model_chi2, dc_batch_chi2): 0 (default)
  * plans it from the grid and the CU count; 1..32 forces that many column
  * tiles of 256 per work-group (1 = one work-group per column tile).  A tuning

@@ -104,10 +104,10 @@ def walk_case():
     return _WALK_CASE
 
 
-@pytest.mark.parametrize("prec", ["bf16x3", "fp16x2"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "fp16x2"])
 @pytest.mark.parametrize("walk,expect", [(1, 1), (2, 2), (3, 3), (5, 5), (32, 5), (0, None)])
-def test_model_vjp_split_column_walk(ctx, prec, walk, expect):
-    """Split reverse pass on a short grid: one work-group per column tile
+def test_model_vjp_column_walk(ctx, prec, walk, expect):
+    """Reverse pass on a short grid (FP32 and split): one work-group per column tile
     (walk 1), column ranges with A resident in LDS (2 and 3 leave a ragged
     last range, 5 and 32 walk all five column tiles in one range), and the
     planned walk.  Each matches the FP64 gradient to the scheme's bar (FP16x2
@@ -134,7 +134,7 @@ def test_model_vjp_split_column_walk(ctx, prec, walk, expect):
           "| vs walk 1", " ".join(f"{x:.1e}" for x in e1))
     if expect is not None:
         assert used == expect
-    assert max(e) <= {"bf16x3": GRAD_BAR_PREC["bf16x3"], "fp16x2": 1e-3}[prec]
+    assert max(e) <= {"fp32": GRAD_BAR_PREC["fp32"], "bf16x3": GRAD_BAR_PREC["bf16x3"], "fp16x2": 1e-3}[prec]
     assert max(e1) <= 1e-5
 
 
