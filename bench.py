@@ -271,7 +271,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--extras", default="lots,adjoint,batch,post,nufft",
+    ap.add_argument("--extras", default="lots,adjoint,batch,post,nufft,peaks",
                     help="comma list of the extra measurements to run")
     ap.add_argument("--cpu-layers", type=int, default=40)
     ap.add_argument("--calibrate-copy", type=int, default=0)
@@ -406,6 +406,21 @@ def main():
     if "post" in want:
         extras["post"] = bench_post(_lib, FR, torch)
         log(f"post: {json.dumps(extras['post'])}")
+    if "peaks" in want:
+        # the device's ceilings measured with plain MFMA chains and HBM streams
+        # (dc_measure_peaks); the fractions above stay priced against the
+        # published peaks, these say what a plain stream of the same
+        # instruction reaches on this part
+        pctx = _lib.Context(device=0)
+        try:
+            pk = pctx.measure_peaks(1 << 32)
+        finally:
+            pctx.close()
+        mkey = {"bf16x3": "mfma_bf16_tflops", "fp16x3": "mfma_f16_tflops", "fp16x2": "mfma_f16_tflops",
+                "fp32": "mfma_f32_tflops"}[args.prec]
+        roofline["measured_ceilings"] = {k: round(v, 2) if isinstance(v, float) else v for k, v in pk.items()}
+        roofline["frac_of_measured_ceiling"] = round(roofline["achieved"] / pk[mkey], 4)
+        log(f"peaks: {json.dumps(roofline['measured_ceilings'])}")
 
     cpu = None
     if not args.no_cpu:

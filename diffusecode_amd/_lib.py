@@ -33,7 +33,7 @@ CFPKT = 9999
 
 EXPORTS = [
     "dc_create", "dc_destroy", "dc_last_error", "dc_last_error_type", "dc_set_precision",
-    "dc_get_precision", "dc_set_gemm_tile", "dc_set_adjoint_walk", "dc_gemm_num_tiles", "dc_gemm_tile_shape", "dc_set_cancel_flag", "dc_version", "dc_device_info", "dc_strucf",
+    "dc_get_precision", "dc_set_gemm_tile", "dc_set_adjoint_walk", "dc_gemm_num_tiles", "dc_gemm_tile_shape", "dc_set_cancel_flag", "dc_version", "dc_device_info", "dc_measure_peaks", "dc_strucf",
     "dc_crystal_set", "dc_four_run", "dc_four_run_dev", "dc_last_stats", "dc_synchronize",
     "dc_model_intensity", "dc_model_vjp", "dc_model_chi2", "dc_model_jvp",
     "dc_batch_chi2", "dc_set_autotune", "dc_set_graphs", "dc_batch_info",
@@ -67,6 +67,11 @@ class Stats(C.Structure):
     _fields_ = [("pairs", C.c_double), ("gemm_ms", C.c_double), ("total_ms", C.c_double),
                 ("reduce_ms", C.c_double), ("gemm_launches", C.c_int64), ("slices", C.c_int64),
                 ("forward_ms", C.c_double), ("adjoint_ms", C.c_double), ("adjoint_walk", C.c_int64)]
+
+
+class Peaks(C.Structure):
+    _fields_ = [("mfma_bf16_tflops", C.c_double), ("mfma_f16_tflops", C.c_double), ("mfma_f32_tflops", C.c_double),
+                ("hbm_read_gbs", C.c_double), ("hbm_copy_gbs", C.c_double), ("hbm_bytes", C.c_int64), ("num_cu", C.c_int)]
 
 
 class ModelDesc(C.Structure):
@@ -126,6 +131,7 @@ def lib():
     L.dc_set_cancel_flag.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
     L.dc_version.restype = C.c_char_p
     L.dc_device_info.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.POINTER(C.c_int)]
+    L.dc_measure_peaks.argtypes = [C.c_void_p, C.c_int64, C.POINTER(Peaks)]
     L.dc_strucf.argtypes = [C.c_void_p, _dp, C.c_int64, C.c_int64, _dp, _dp, _dp, _dp, _ip, _ip, _ip, _dp,
                             C.c_int64, _ip, _dp, _dp, C.c_int, _dp]
     L.dc_crystal_set.argtypes = [C.c_void_p, C.POINTER(CrystalDesc)]
@@ -269,6 +275,13 @@ class Context:
         n = C.c_int()
         self.check(lib().dc_device_info(self.h, buf, 64, C.byref(n)))
         return buf.value.decode(), n.value
+
+    def measure_peaks(self, hbm_bytes=0):
+        """Ceilings measured with plain instruction streams (dc_measure_peaks):
+        MFMA TFLOP/s for BF16, FP16 and FP32 chains, HBM read and copy GB/s."""
+        p = Peaks()
+        self.check(lib().dc_measure_peaks(self.h, int(hbm_bytes), C.byref(p)))
+        return {k: getattr(p, k) for k, _ in Peaks._fields_}
 
     def synchronize(self):
         self.check(lib().dc_synchronize(self.h))

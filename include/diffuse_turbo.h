@@ -85,6 +85,25 @@ int dc_set_cancel_flag(dc_ctx *ctx, const volatile int *flag);
 const char *dc_version(void);
 int dc_device_info(const dc_ctx *ctx, char *gfx_name, int len, int *num_cu);
 
+/* Ceilings of the device for the instructions the path relies on, measured
+ * with plain streams of them (no counterpart in DISCUS; a measurement aid for
+ * roofline figures, which are priced against the published peaks):
+ * independent chains of v_mfma_f32_16x16x32_bf16 / _f16 and
+ * v_mfma_f32_16x16x4_f32 with operands in registers, and a streaming read
+ * and copy of hbm_bytes (0: 1 GiB; a multiple of 16 in [16 MiB, 64 GiB]).
+ * Allocates and frees two hbm_bytes device buffers; best of 3 launches each.
+ * hbm_copy_gbs counts the bytes read plus the bytes written. */
+typedef struct dc_peaks {
+    double mfma_bf16_tflops;
+    double mfma_f16_tflops;
+    double mfma_f32_tflops;
+    double hbm_read_gbs;
+    double hbm_copy_gbs;
+    int64_t hbm_bytes;
+    int num_cu;
+} dc_peaks;
+int dc_measure_peaks(dc_ctx *ctx, int64_t hbm_bytes, dc_peaks *out);
+
 /*
  * dc_strucf -- drop-in for DISCUS's per-type kernel
  *   four_strucf(iscat, lform, ldiscamb, is_anis, isym, ientry, fnum)
