@@ -952,16 +952,7 @@ int adjoint_tile_cols() { return NC; }
 // prologue (about half a column tile of work).
 int adjoint_max_walk() { return RES_MAXCT; }
 
-int adjoint_plan(AdjArgs &a, int ntiles, bool split, int force) {
-    const int ntc = std::max(1, (a.C + NC - 1) / NC);
-    a.ctpg = 1;
-    (void)split;
-    if (ntiles <= 0 || a.R <= 0 || (a.R + SKR - 1) / SKR > RES_ST) return ntc;
-    if (force > 0) {
-        a.ctpg = std::min({force, ntc, RES_MAXCT});
-        return (ntc + a.ctpg - 1) / a.ctpg;
-    }
-    // CU count of the current device, queried once per device
+int device_cu_count() {
     static std::atomic<int> cus[64];
     int dev = 0, ncu = 256;
     if (hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < 64) {
@@ -972,6 +963,19 @@ int adjoint_plan(AdjArgs &a, int ntiles, bool split, int force) {
             cus[dev].store(ncu, std::memory_order_relaxed);
         }
     }
+    return ncu;
+}
+
+int adjoint_plan(AdjArgs &a, int ntiles, bool split, int force) {
+    const int ntc = std::max(1, (a.C + NC - 1) / NC);
+    a.ctpg = 1;
+    (void)split;
+    if (ntiles <= 0 || a.R <= 0 || (a.R + SKR - 1) / SKR > RES_ST) return ntc;
+    if (force > 0) {
+        a.ctpg = std::min({force, ntc, RES_MAXCT});
+        return (ntc + a.ctpg - 1) / a.ctpg;
+    }
+    const int ncu = device_cu_count();
     double best = 0.0;
     int bestc = 0;
     for (int ctpg = std::min(ntc, RES_MAXCT); ctpg >= 1; --ctpg) {

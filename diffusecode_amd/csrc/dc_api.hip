@@ -106,20 +106,40 @@ int max_corner_bin(const Grid &g) {
 
 long long padded(long long n, long long t) { return (n + t - 1) / t * t; }
 
+// Atoms per slice of the split schemes: enough slices that a launch has
+// about 1024 work items (wgs = work-groups per slice), within
+// [KCHUNK_MIN, KCHUNK_MAX]; FP32 and FP16X3 use fixed slice lengths that
+// bound their accumulators' rounding on coherent sums.
+static int kchunk_atoms(long long wgs, long long natoms, int prec) {
+    if (prec == DC_PREC_FP32) return KCHUNK_FP32;
+    if (prec == DC_PREC_FP16X3) return KCHUNK_FP16X3;
+    long long want = std::max<long long>(1, (1024 + wgs - 1) / std::max<long long>(wgs, 1));
+    long long k = (natoms + want - 1) / want;
+    k = std::min<long long>(std::max<long long>(k, KCHUNK_MIN), KCHUNK_MAX);
+    return (int)padded(k, GEMM_KS);
+}
+
 // GEMM plan for the computed region f of grid g.
 //   mapping 0: rows = abscissa (no wrap), cols = (ordinate, top), fast period C0
 //   mapping 1: rows = (abscissa, ordinate), fast period R0, cols = top (no wrap)
 // Fast periods are padded to >= 16 when the side wraps (the two-level operand
 // tables of dc_gemm.hip need at most one wrap per 16-group); padded pixels
-// are computed and never read.  The tile configuration minimises the padded
-// work of whole work-group tiles (ties: larger tiles); forced >= 0 overrides,
-// only_mapping >= 0 restricts the mapping, and the split-only tiles are
-// skipped for the FP32 scheme.
-
-Plan plan_gemm(Grid &g, int forced, int only_mapping, int prec) {
+// are computed and never read.  forced >= 0 fixes the tile (the mapping is
+// then the one with the least padded work), only_mapping >= 0 restricts the
+// mapping, and the split-only tiles are skipped for the FP32 scheme.
+// Cost of a (mapping, tile) candidate:
+//   FP32: padded work x (1 + (16/RT + 16/CT) / 2);
+//   split schemes: the launch's rounds of work-groups over the CUs x the
+//   work of one work-group (RT x CT x slice atoms) x the tile's measured
+//   cost factor (TileInfo::eff), with the slices cut as kchunk_atoms cuts
+//   them for natoms atoms.  Small problems fill the device with few work
+//   items, so the rounds term favours smaller tiles there; large problems
+//   reduce to padded work x eff.  natoms <= 0: padded work x eff.
+Plan plan_gemm(Grid &g, int forced, int only_mapping, int prec, long long natoms) {
     Plan best;
     double best_cost = 1e300;
     const int nt = gemm_num_tiles();
+    const int ncu = device_cu_count();
     for (int mapping = 0; mapping < 2; ++mapping) {
         if (mapping == 1 && g.f[2] <= 1) continue;
         if (only_mapping >= 0 && mapping != only_mapping) continue;
@@ -143,8 +163,19 @@ Plan plan_gemm(Grid &g, int forced, int only_mapping, int prec) {
             if (forced >= 0 && t != forced) continue;
             TileInfo ti = gemm_tile_info(t);
             if (forced < 0 && ti.split_only && prec == DC_PREC_FP32) continue;
-            double work = (double)padded(gg.R, ti.rt) * (double)padded(gg.C, ti.ct);
-            double cost = work * (1.0 + 0.5 * (16.0 / ti.rt + 16.0 / ti.ct));
+            const double work = (double)padded(gg.R, ti.rt) * (double)padded(gg.C, ti.ct);
+            double cost;
+            if (prec == DC_PREC_FP32 || forced >= 0) {
+                cost = work * (1.0 + 0.5 * (16.0 / ti.rt + 16.0 / ti.ct));
+            } else if (natoms <= 0) {
+                cost = work * ti.eff;
+            } else {
+                const long long wgs = (long long)((gg.R + ti.rt - 1) / ti.rt) * ((gg.C + ti.ct - 1) / ti.ct);
+                const long long k = kchunk_atoms(wgs, natoms, prec);
+                const long long nsl = (natoms + k - 1) / k;
+                const long long rounds = (wgs * nsl + ncu - 1) / ncu;
+                cost = (double)rounds * ti.rt * ti.ct * (double)std::min(k, natoms) * ti.eff;
+            }
             if (cost < best_cost) {
                 best_cost = cost;
                 best.gg = gg;
@@ -159,14 +190,9 @@ Plan plan_gemm(Grid &g, int forced, int only_mapping, int prec) {
 }
 
 int kchunk_for(const Plan &pl, long long natoms_batch, int prec) {
-    if (prec == DC_PREC_FP32) return KCHUNK_FP32;
-    if (prec == DC_PREC_FP16X3) return KCHUNK_FP16X3;
     TileInfo ti = gemm_tile_info(pl.tile);
-    long long wgs = ((pl.gg.R + ti.rt - 1) / ti.rt) * ((pl.gg.C + ti.ct - 1) / ti.ct);
-    long long want = std::max<long long>(1, (1024 + wgs - 1) / wgs);
-    long long k = (natoms_batch + want - 1) / want;
-    k = std::min<long long>(std::max<long long>(k, KCHUNK_MIN), KCHUNK_MAX);
-    return (int)padded(k, GEMM_KS);
+    long long wgs = (long long)((pl.gg.R + ti.rt - 1) / ti.rt) * ((pl.gg.C + ti.ct - 1) / ti.ct);
+    return kchunk_atoms(wgs, natoms_batch, prec);
 }
 
 
@@ -392,7 +418,7 @@ int dc_strucf(dc_ctx *c, const double *xat, int64_t nxat, int64_t ld_xat, const 
         if (max_corner_bin(g) >= ncfact) return set_err(c, DC_ERR_ISTL, DC_ER_FOUR, "sin(theta)/lambda beyond table");
     }
 
-    const Plan pl = plan_gemm(g, c->tile, -1, c->prec);
+    const Plan pl = plan_gemm(g, c->tile, -1, c->prec, nxat);
     const GemmGeom &gg = pl.gg;
     const int mapping = pl.mapping;
 
@@ -647,7 +673,12 @@ int dc_four_run_dev(dc_ctx *c, const dc_run_desc *r, double *d_csf, double *d_ac
     if (max_corner_bin(g) > std::min(CFPKT, r->ncfact - 1))
         return set_err(c, DC_ERR_ISTL, DC_ER_FOUR, "sin(theta)/lambda beyond the form factor table");
 
-    const Plan pl = plan_gemm(g, c->tile, -1, c->prec);
+    // atoms of one GEMM launch: the crystal, or the lots of a batch
+    long long natoms_launch = c->natoms;
+    if (lots)
+        natoms_launch = std::min<long long>(natoms_launch, (long long)std::max(r->nlots, 1) * r->ls[0] * r->ls[1] *
+                                                               r->ls[2] * c->ncatoms);
+    const Plan pl = plan_gemm(g, c->tile, -1, c->prec, natoms_launch);
     int st = 0;
     if ((st = upload(c, c->cfact, r->cfact, 2 * (size_t)r->ncfact * c->nscat))) return st;
 
@@ -745,7 +776,7 @@ int dc_four_run_dev(dc_ctx *c, const dc_run_desc *r, double *d_csf, double *d_ac
         for (int k = 0; k < 3; ++k) gl_full.f[k] = g.n[k];
         gl_full.P = ntot;
         gl_full.friedel = 0;
-        const Plan pll = plan_gemm(gl_full, c->tile, -1, c->prec);
+        const Plan pll = plan_gemm(gl_full, c->tile, -1, c->prec, (long long)(lp.size() / 3));
         GatherList gll;
         gll.segs.push_back({0, (int)(lp.size() / 3)});
         if ((st = gemm_segments(c, gl_full, pll, gll, lp.data(), ss, nslice, false))) return st;

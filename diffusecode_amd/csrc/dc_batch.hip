@@ -167,7 +167,7 @@ int build_state(dc_ctx *c, const dc_batch_desc *d, BatchState &b) {
         g.win[k] = d->vi[6 + k];
     }
     g.P = (long long)g.f[0] * g.f[1] * g.f[2];
-    b.pl = plan_gemm(g, c->tile, 0, c->prec);
+    b.pl = plan_gemm(g, c->tile, 0, c->prec, d->atom_beg[d->nmembers]);
     const GemmGeom &gg = b.pl.gg;
 
     // member-major, type-sorted atom order; slices; adjoint tiles

@@ -597,10 +597,18 @@ using CfgH = TileCfg<1, 11, 8, 1>;  // 128 x 176
 // split_only: tiles whose FP32 variant (two accumulator sets) spills so much
 // that it is slower than the 96x256 tile (which spills a little and is still
 // the fastest FP32 tile, measured); the planner does not offer them to FP32.
+// eff: split-scheme time per padded (output x atom) relative to 256 x 176,
+// measured with every tile forced (BF16X3 and FP16X3, MI355X) on 3D blocks of
+// 101^3 and 121^3 points (256k atoms), 8 layers of 401^2 with 64 lots and the
+// 161^3 block: 128x256 1.08-1.27, 96x256 1.05-1.32, 64x256 1.57-1.70,
+// 128x128/4w 2.14-2.43, 128x128/8w 1.54-1.76, 16x256 6.2-6.6, 128x176
+// 1.24-1.39.  The smaller tiles generate more operand elements per product
+// and have more work items per launch, each with its own set-up.
 static const TileInfo kTiles[] = {
-    {CfgA::RT, CfgA::CT, CfgA::NT, 1}, {CfgB::RT, CfgB::CT, CfgB::NT, 0}, {CfgC::RT, CfgC::CT, CfgC::NT, 0},
-    {CfgD::RT, CfgD::CT, CfgD::NT, 1}, {CfgE::RT, CfgE::CT, CfgE::NT, 0}, {CfgF::RT, CfgF::CT, CfgF::NT, 0},
-    {CfgG::RT, CfgG::CT, CfgG::NT, 1}, {CfgH::RT, CfgH::CT, CfgH::NT, 1},
+    {CfgA::RT, CfgA::CT, CfgA::NT, 1, 1.17f}, {CfgB::RT, CfgB::CT, CfgB::NT, 0, 1.28f},
+    {CfgC::RT, CfgC::CT, CfgC::NT, 0, 1.62f}, {CfgD::RT, CfgD::CT, CfgD::NT, 1, 2.30f},
+    {CfgE::RT, CfgE::CT, CfgE::NT, 0, 1.65f}, {CfgF::RT, CfgF::CT, CfgF::NT, 0, 6.30f},
+    {CfgG::RT, CfgG::CT, CfgG::NT, 1, 1.00f}, {CfgH::RT, CfgH::CT, CfgH::NT, 1, 1.33f},
 };
 
 int gemm_num_tiles() { return (int)(sizeof(kTiles) / sizeof(kTiles[0])); }

@@ -57,6 +57,7 @@ __device__ inline int istl_point(const Grid &g, int i, int j, int l) {
 struct TileInfo {
     int rt, ct, nt;
     int split_only;         // tile offered to the split schemes only (its FP32 variant spills)
+    float eff;              // split schemes: time per padded output x atom, relative to 256 x 176
 };
 
 struct ReduceArgs {
@@ -189,6 +190,8 @@ int adjoint_tile_cols();
 // (<= ceil(C / adjoint_tile_cols())), which adjoint_finish sums.  force > 0
 // asks for that many column tiles per work-group where the walk applies.
 int adjoint_plan(AdjArgs &a, int ntiles, bool split, int force = 0);
+// Compute units of the current device (queried once per device; 256 if unknown).
+int device_cu_count();
 int adjoint_max_walk();
 // split = true: BF16x3 MFMA on qs (made by launch_q_split), else FP32 on q.
 hipError_t launch_adjoint(const AdjArgs &a, int ntiles, hipStream_t st, bool split = false);

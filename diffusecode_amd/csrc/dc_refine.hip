@@ -60,7 +60,7 @@ int plan_model(dc_ctx *c, const dc_model_desc *m, ModelPlan &mp) {
     }
     for (int k = 0; k < 9; ++k) g.rten[k] = m->rten[k];
     g.P = (long long)g.f[0] * g.f[1] * g.f[2];
-    mp.pl = plan_gemm(g, c->tile, 0, c->prec);
+    mp.pl = plan_gemm(g, c->tile, 0, c->prec, m->natoms);
     const GemmGeom &gg = mp.pl.gg;
 
     // metric orthogonality of the abscissa against the other axes
