@@ -523,14 +523,16 @@ phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice
             }
             // The three steps of a stage touch disjoint buffers (elements of
             // s+1 from tables of s+1, tables of s+2, MFMAs on elements of s),
-            // so their order is free.  BF16X3: waves 4..7 issue the MFMAs
-            // first, so the two waves sharing a SIMD (w and w+4) overlap one's
-            // operand generation with the other's matrix work instead of both
-            // generating, then both multiplying.  Measured on the headline's
-            // 256 x 176 tile: BF16X3 2% faster, FP16X3 unchanged, FP32 2%
-            // slower; on the narrower tiles the second copy of the MFMA block
-            // costs registers.  So only BF16X3 on the wide-column tiles.
-            const bool mfma_first = SCHEME == SCH_BF16X3 && T::WC > 4 && ((wave >> 2) & 1) != 0;
+            // so their order is free.  Three-product schemes: waves 4..7 issue
+            // the MFMAs first, so the two waves sharing a SIMD (w and w+4)
+            // overlap one's operand generation with the other's matrix work
+            // instead of both generating, then both multiplying.  Measured on
+            // the headline's 256 x 176 tile: BF16X3 2% faster, FP16X3 1-1.6%
+            // faster (with B read one column tile ahead), FP16X2 unchanged,
+            // FP32 2% slower; on the narrower tiles the second copy of the
+            // MFMA block costs registers.  So BF16X3 and FP16X3 on the
+            // wide-column tiles.
+            const bool mfma_first = split3(SCHEME) && T::WC > 4 && ((wave >> 2) & 1) != 0;
             if (mfma_first && live) mfma_stage<SCHEME, T>(sm.e[cur], accR, accI, wr0, wc0);
             if (s + 1 < nst) build_elems<SCHEME, T, DAMP>(sm.e[nxt], sm.t[nxt], sm.a_wpos, sm.b_wpos, sm.s_a, sm.s_b, wave);
             if (s + 2 < nst) build_tables<T>(sm.t[cur], sm.a_k16, sm.b_k16, rt, kbeg + (s + 2) * KS + at < kend, sa, sb, pt, wscale);
