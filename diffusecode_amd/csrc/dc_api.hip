@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include "../../include/diffuse_turbo.h"
@@ -195,15 +196,92 @@ int kchunk_for(const Plan &pl, long long natoms_batch, int prec) {
     return kchunk_atoms(wgs, natoms_batch, prec);
 }
 
+// Slice length of the split schemes for the (type, count) segments of one
+// launch.  The launch has one work-group per (slice, tile), and every tile's
+// LDS footprint allows one work-group per CU, so it runs in rounds of ncu
+// work items; a last round that is mostly empty costs as much as a full one
+// (the headline 161^3 block had 21 slices x 51 tiles = 1,071 items, 4.2
+// rounds run as 5).  A candidate length k splits every segment into
+// ceil(n / k) near-equal slices (add_segments); its predicted time is
+// ceil(W / ncu) x (L + 64), with L the longest slice, 64 atoms standing for
+// an item's set-up, and W the items weighted by (length + 64) / (L + 64):
+// exact when the slices are equal, close to the fluid limit when short
+// slices fill the gaps.  The candidates are the lengths in [KCHUNK_MIN,
+// KCHUNK_MAX] at which some segment's slice count changes; the shortest
+// predicted time wins, ties going to the longer slices (fewer planes to
+// reduce).  Launches that already have 16 rounds of items or more, and
+// populations (many segments), keep the rule of kchunk_atoms.
+int kchunk_for(const Plan &pl, const std::vector<std::pair<int, int>> &segs, int prec) {
+    long long natoms = 0;
+    for (auto &sg : segs) natoms += std::max(sg.second, 0);
+    const int k0 = kchunk_for(pl, natoms, prec);
+    if (prec == DC_PREC_FP32 || prec == DC_PREC_FP16X3 || natoms <= 0) return k0;
+    const TileInfo ti = gemm_tile_info(pl.tile);
+    const long long wgs = (long long)((pl.gg.R + ti.rt - 1) / ti.rt) * ((pl.gg.C + ti.ct - 1) / ti.ct);
+    const int ncu = device_cu_count();
+    long long items0 = 0;
+    for (auto &sg : segs)
+        if (sg.second > 0) items0 += (sg.second + k0 - 1) / k0;
+    items0 *= wgs;
+    // 16 rounds or more: the last one costs at most 1/16 -- keep the rule
+    if (items0 >= 16LL * ncu || segs.size() > 512) return k0;
+    // candidates: the lengths at which a segment's slice count changes
+    std::vector<int> cand;
+    for (auto &sg : segs) {
+        if (sg.second <= 0) continue;
+        const int n = sg.second;
+        for (int ns = (n + KCHUNK_MAX - 1) / KCHUNK_MAX; ns <= (n + KCHUNK_MIN - 1) / KCHUNK_MIN; ++ns) {
+            const int k = (n + ns - 1) / ns;
+            if (k >= KCHUNK_MIN && k <= KCHUNK_MAX) cand.push_back(k);
+        }
+        if ((long long)cand.size() > 4096) break;
+    }
+    cand.push_back(k0);
+    std::sort(cand.begin(), cand.end(), std::greater<int>());
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    int best = k0;
+    double best_t = 1e300;
+    for (int k : cand) {
+        // add_segments splits a segment of n atoms into ns = ceil(n / k)
+        // slices of ceil(n / ns) atoms at most
+        double L = 0.0;
+        for (auto &sg : segs) {
+            if (sg.second <= 0) continue;
+            const long long ns = (sg.second + k - 1) / k;
+            L = std::max(L, (double)((sg.second + ns - 1) / ns));
+        }
+        L += 64.0;
+        double w = 0.0;
+        for (auto &sg : segs) {
+            if (sg.second <= 0) continue;
+            const long long ns = (sg.second + k - 1) / k;
+            w += (double)ns * ((double)((sg.second + ns - 1) / ns) + 64.0) / L;
+        }
+        const double t = std::ceil(w * (double)wgs / ncu - 1e-9) * L;
+        if (t < best_t * (1.0 - 1e-9)) {
+            best_t = t;
+            best = k;
+        }
+    }
+    return best;
+}
 
+
+// Each segment of n atoms becomes ceil(n / kchunk) slices of near-equal
+// length (they differ by at most one atom, none longer than kchunk), so the
+// work items of a launch are uniform.
 void add_segments(SliceSet &s, const std::vector<std::pair<int, int>> &segs, int kchunk, int &atom_off) {
     for (auto &sg : segs) {
-        int n = sg.second;
+        const int n = sg.second;
         s.seg_first.push_back((int)s.type.size());
-        for (int b = 0; b < n; b += kchunk) {
+        const int ns = (n + kchunk - 1) / kchunk;
+        int b = 0;
+        for (int q = 0; q < ns; ++q) {
+            const int len = n / ns + (q < n % ns ? 1 : 0);
             if (s.beg.empty()) s.beg.push_back(atom_off);
             s.type.push_back(sg.first);
-            s.beg.push_back(atom_off + std::min(n, b + kchunk));
+            b += len;
+            s.beg.push_back(atom_off + b);
         }
         atom_off += n;
     }
@@ -610,7 +688,7 @@ int gemm_segments(dc_ctx *c, const Grid &g, const Plan &pl, GatherList &gl,
     HIPCHK(launch_prep(pa, ptr<AtomRec>(c->rec), c->st));
     int off = 0;
     ss = SliceSet();
-    add_segments(ss, gl.segs, kchunk_for(pl, n, c->prec), off);
+    add_segments(ss, gl.segs, kchunk_for(pl, gl.segs, c->prec), off);
     if (ss.beg.empty()) ss.beg.push_back(0);
     nslice = (int)ss.type.size();
     if ((st = upload(c, c->slice_beg, ss.beg.data(), ss.beg.size()))) return st;

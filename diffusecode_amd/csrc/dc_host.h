@@ -134,6 +134,7 @@ long long padded(long long n, long long t);
 int max_corner_bin(const Grid &g);   // largest sin(theta)/lambda bin over the grid corners
 Plan plan_gemm(Grid &g, int forced, int only_mapping = -1, int prec = 1, long long natoms = 0);
 int kchunk_for(const Plan &pl, long long natoms_batch, int prec);
+int kchunk_for(const Plan &pl, const std::vector<std::pair<int, int>> &segs, int prec);
 void add_segments(SliceSet &s, const std::vector<std::pair<int, int>> &segs, int kchunk, int &atom_off);
 int run_gemm(dc_ctx *c, const Plan &pl, int nslice, float2 *partial, const float *sq = nullptr);
 void batch_release(dc_ctx *c);   // frees the cached batch plan, buffers and graph

@@ -141,7 +141,7 @@ int model_forward(dc_ctx *c, const dc_model_desc *m, ModelPlan &mp) {
     const GemmGeom &gg = mp.pl.gg;
     SliceSet ss;
     int off = 0;
-    add_segments(ss, mp.segs, kchunk_for(mp.pl, mp.nsorted, c->prec), off);
+    add_segments(ss, mp.segs, kchunk_for(mp.pl, mp.segs, c->prec), off);
     if (ss.beg.empty()) ss.beg.push_back(0);
     const int nslice = (int)ss.type.size();
     ss.lot = {0, nslice};
