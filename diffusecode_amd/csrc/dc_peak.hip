@@ -2,7 +2,7 @@
 // on, measured with plain streams of them:
 //   * dense MFMA chains: v_mfma_f32_16x16x32_bf16 (BF16x3), _f16 (FP16x2,
 //     FP16x3) and v_mfma_f32_16x16x4_f32 (FP32), 8 independent accumulators
-//     per wave, operands in registers, 8 waves per SIMD;
+//     per wave, operands in registers, 8 waves per SIMD, about 1 ms per launch;
 //   * HBM streams: a 16-byte-per-lane read of a buffer much larger than the
 //     L2 and MALL caches, and a copy of it.
 // The roofline fractions of the kernels are priced against the published
@@ -166,8 +166,10 @@ extern "C" int dc_measure_peaks(dc_ctx *c, int64_t hbm_bytes, dc_peaks *out) {
     float *d_out = static_cast<float *>(sink.p);
     const long long waves = (long long)grid * (PK_THREADS / 64);
     double ms = 0;
-    // iteration counts: roughly 0.1 s per launch at the published peaks
-    const int it16 = 256, it32 = 128;
+    // iteration counts: about 1 ms per launch at the published peaks (short
+    // launches under-read the ceiling: 0.13 ms launches reached 2.06-2.10
+    // PFLOP/s BF16 where 1 ms launches reach 2.4)
+    const int it16 = 2048, it32 = 1024;
     int st = best_ms(c, t, [&] { mfma_peak_kernel<PK_BF16><<<grid, PK_THREADS, 0, c->st>>>(d_out, it16, 1.0f / 4096); }, ms);
     if (st) return st;
     out->mfma_bf16_tflops = waves * (double)it16 * PK_CHAINS * pk_flops(PK_BF16) / (ms * 1e-3) / 1e12;
