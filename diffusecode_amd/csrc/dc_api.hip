@@ -196,7 +196,7 @@ int kchunk_for(const Plan &pl, long long natoms_batch, int prec) {
     return kchunk_atoms(wgs, natoms_batch, prec);
 }
 
-// Slice length of the split schemes for the (type, count) segments of one
+// Slice length for the (type, count) segments of one
 // launch.  The launch has one work-group per (slice, tile), and every tile's
 // LDS footprint allows one work-group per CU, so it runs in rounds of ncu
 // work items; a last round that is mostly empty costs as much as a full one
@@ -209,13 +209,19 @@ int kchunk_for(const Plan &pl, long long natoms_batch, int prec) {
 // slices fill the gaps.  The candidates are the lengths in [KCHUNK_MIN,
 // KCHUNK_MAX] at which some segment's slice count changes; the shortest
 // predicted time wins, ties going to the longer slices (fewer planes to
-// reduce).  Launches that already have 16 rounds of items or more, and
+// reduce).  FP32 and FP16X3 search between half their fixed slice cap and
+// the cap.  Launches that already have 16 rounds of items or more, and
 // populations (many segments), keep the rule of kchunk_atoms.
 int kchunk_for(const Plan &pl, const std::vector<std::pair<int, int>> &segs, int prec) {
     long long natoms = 0;
     for (auto &sg : segs) natoms += std::max(sg.second, 0);
     const int k0 = kchunk_for(pl, natoms, prec);
-    if (prec == DC_PREC_FP32 || prec == DC_PREC_FP16X3 || natoms <= 0) return k0;
+    if (natoms <= 0) return k0;
+    // FP32 and FP16X3 cap their slices (accumulator rounding on coherent
+    // sums); shorter slices are allowed, down to half the cap
+    const bool capped = prec == DC_PREC_FP32 || prec == DC_PREC_FP16X3;
+    const int kmax = capped ? k0 : KCHUNK_MAX;
+    const int kmin = capped ? k0 / 2 : KCHUNK_MIN;
     const TileInfo ti = gemm_tile_info(pl.tile);
     const long long wgs = (long long)((pl.gg.R + ti.rt - 1) / ti.rt) * ((pl.gg.C + ti.ct - 1) / ti.ct);
     const int ncu = device_cu_count();
@@ -230,9 +236,9 @@ int kchunk_for(const Plan &pl, const std::vector<std::pair<int, int>> &segs, int
     for (auto &sg : segs) {
         if (sg.second <= 0) continue;
         const int n = sg.second;
-        for (int ns = (n + KCHUNK_MAX - 1) / KCHUNK_MAX; ns <= (n + KCHUNK_MIN - 1) / KCHUNK_MIN; ++ns) {
+        for (int ns = (n + kmax - 1) / kmax; ns <= (n + kmin - 1) / kmin; ++ns) {
             const int k = (n + ns - 1) / ns;
-            if (k >= KCHUNK_MIN && k <= KCHUNK_MAX) cand.push_back(k);
+            if (k >= kmin && k <= kmax) cand.push_back(k);
         }
         if ((long long)cand.size() > 4096) break;
     }
