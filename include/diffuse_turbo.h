@@ -54,7 +54,7 @@ extern "C" {
 #define DC_PREC_FP32 0          /* plain FP32 MFMA (the drop-in)            */
 #define DC_PREC_BF16X3 1        /* split BF16 hi/lo, 3 MFMA products        */
 #define DC_PREC_FP16X2 2        /* split FP16 hi/lo on one side, 2 products */
-#define DC_PREC_FP16X3 3        /* split FP16 hi/lo on both sides, 3 products, 512-atom slices */
+#define DC_PREC_FP16X3 3        /* split FP16 hi/lo on both sides, 3 products, slices of <= 512 atoms */
 
 /* ---- lot shapes (discus/prog/diffuse_mod.f90:10-12) --------------------- */
 #define DC_LOT_OFF 1
