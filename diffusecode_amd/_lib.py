@@ -33,7 +33,7 @@ CFPKT = 9999
 
 EXPORTS = [
     "dc_create", "dc_destroy", "dc_last_error", "dc_last_error_type", "dc_set_precision",
-    "dc_get_precision", "dc_set_gemm_tile", "dc_set_adjoint_walk", "dc_gemm_num_tiles", "dc_gemm_tile_shape", "dc_set_cancel_flag", "dc_version", "dc_device_info", "dc_measure_peaks", "dc_strucf",
+    "dc_get_precision", "dc_set_gemm_tile", "dc_set_slice_atoms", "dc_set_phase_dither", "dc_set_adjoint_walk", "dc_gemm_num_tiles", "dc_gemm_tile_shape", "dc_set_cancel_flag", "dc_version", "dc_device_info", "dc_measure_peaks", "dc_strucf", "dc_four_stltab",
     "dc_crystal_set", "dc_four_run", "dc_four_run_dev", "dc_last_stats", "dc_synchronize",
     "dc_model_intensity", "dc_model_vjp", "dc_model_chi2", "dc_model_jvp",
     "dc_batch_chi2", "dc_set_autotune", "dc_set_graphs", "dc_batch_info",
@@ -126,6 +126,8 @@ def lib():
     L.dc_set_precision.argtypes = [C.c_void_p, C.c_int]
     L.dc_get_precision.argtypes = [C.c_void_p]
     L.dc_set_gemm_tile.argtypes = [C.c_void_p, C.c_int]
+    L.dc_set_slice_atoms.argtypes = [C.c_void_p, C.c_int]
+    L.dc_set_phase_dither.argtypes = [C.c_void_p, C.c_int]
     L.dc_set_adjoint_walk.argtypes = [C.c_void_p, C.c_int]
     L.dc_gemm_tile_shape.argtypes = [C.c_int, _ip, _ip, _ip]
     L.dc_set_cancel_flag.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
@@ -134,6 +136,7 @@ def lib():
     L.dc_measure_peaks.argtypes = [C.c_void_p, C.c_int64, C.POINTER(Peaks)]
     L.dc_strucf.argtypes = [C.c_void_p, _dp, C.c_int64, C.c_int64, _dp, _dp, _dp, _dp, _ip, _ip, _ip, _dp,
                             C.c_int64, _ip, _dp, _dp, C.c_int, _dp]
+    L.dc_four_stltab.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _ip, _dp, _ip]
     L.dc_crystal_set.argtypes = [C.c_void_p, C.POINTER(CrystalDesc)]
     L.dc_four_run.argtypes = [C.c_void_p, C.POINTER(RunDesc), _dp, _dp, _dp, _dp, _dp]
     L.dc_four_run_dev.argtypes = [C.c_void_p, C.POINTER(RunDesc), C.c_void_p, C.c_void_p, C.c_void_p,
@@ -244,6 +247,14 @@ class Context:
     def set_gemm_tile(self, tile):
         self.check(lib().dc_set_gemm_tile(self.h, int(tile)))
 
+    def set_slice_atoms(self, n):
+        """Atoms per GEMM slice: 0 = planned, n > 0 forced."""
+        self.check(lib().dc_set_slice_atoms(self.h, int(n)))
+
+    def set_phase_dither(self, on):
+        """Per-atom phase rotation of the phase GEMM (default on)."""
+        self.check(lib().dc_set_phase_dither(self.h, 1 if on else 0))
+
     def set_adjoint_walk(self, walk):
         """Column tiles per work-group of the split reverse pass on short
         grids: 0 = planned, 1..32 forced."""
@@ -305,6 +316,15 @@ class Context:
                              _d(None if rten is None else _f64(rten, 9)), int(lform), _d(out))
         self.check(st)
         return out.view(np.complex128).reshape(tuple(num), order="F")
+
+    def four_stltab(self, xm, uin, vin, win, num, rten):
+        """four_stltab on the device: istl (num) int32, Fortran order (raises
+        DCError -3 when a bin exceeds CFPKT)."""
+        num = _i32(num, 3)
+        out = np.zeros(int(np.prod(num)), np.int32)
+        self.check(lib().dc_four_stltab(self.h, _d(_f64(xm, 3)), _d(_f64(uin, 3)), _d(_f64(vin, 3)),
+                                        _d(_f64(win, 3)), _i(num), _d(_f64(rten, 9)), _i(out)))
+        return out.reshape(tuple(num), order="F")
 
     # -- run level -----------------------------------------------------------
     def crystal_set(self, crystal):

@@ -27,13 +27,12 @@ constexpr int KCHUNK_MAX = 16384;
 // into a second set of FP32 sums; with 1,024-atom slices summed in FP64 the
 // outer sum has 16 terms, keeping max|dI|/max(I) near 1e-7 on coherent sums
 // (Bragg peaks of displaced crystals), where one FP32 accumulator over
-// thousands of atoms reaches 1e-6.  BF16X3 and FP16X2 keep long slices.
-// FP16X3, whose products carry 2^-22, keeps 512-atom slices: on the coherent
-// sums of a shifted fcc 40^3 crystal (all 256,000 atoms in phase at the Bragg
-// peaks) its FP32 accumulator gives max|dI|/max(I) = 1.2e-5 over 1,024-atom
-// slices and 5.7e-6 over 512 (measured; 3.1e-6 over 256).
+// thousands of atoms reaches 1e-6.  The split schemes keep long slices: with
+// the per-atom phase rotation of the GEMM (dither_turn) their accumulator and
+// operand rounding stays incoherent (shifted fcc 40^3 crystal, all 256,000
+// atoms in phase at the Bragg peaks: FP16X3 4.0e-7 over 4,096-atom slices,
+// 4.9e-5 without the rotation).
 constexpr int KCHUNK_FP32 = 1024;
-constexpr int KCHUNK_FP16X3 = 512;
 constexpr double PARTIAL_BUDGET = 6.0e9;   // bytes of slice planes per batch
 
 }  // namespace dc
@@ -83,12 +82,13 @@ bool zero_increment(const double *vi, const int *inc) {
 namespace dc {
 
 // sin(theta)/lambda bin with the expression of four_stltab
-// (fourier_sup.f90:2663-2674); mirrors istl_point of dc_lot.hip.
+// (fourier_sup.f90:2663-2674, h rounded to FP32); mirrors istl_point of
+// dc_internal.h.
 #pragma clang fp contract(off)
 int host_istl(const double *xm, const double *u, const double *v, const double *w, int i, int j, int l,
               const double *rten) {
     double h[3], t[3];
-    for (int k = 0; k < 3; ++k) h[k] = xm[k] + u[k] * (double)i + v[k] * (double)j + w[k] * (double)l;
+    for (int k = 0; k < 3; ++k) h[k] = stl_h(xm[k], u[k], v[k], w[k], i, j, l);
     for (int r = 0; r < 3; ++r) t[r] = rten[r] * h[0] + rten[r + 3] * h[1] + rten[r + 6] * h[2];
     double q2 = (h[0] * t[0] + h[1] * t[1] + h[2] * t[2]) / 4.0;
     return (int)std::llround(std::sqrt(q2) * (1.0 / CFINC));
@@ -109,11 +109,10 @@ long long padded(long long n, long long t) { return (n + t - 1) / t * t; }
 
 // Atoms per slice of the split schemes: enough slices that a launch has
 // about 1024 work items (wgs = work-groups per slice), within
-// [KCHUNK_MIN, KCHUNK_MAX]; FP32 and FP16X3 use fixed slice lengths that
-// bound their accumulators' rounding on coherent sums.
+// [KCHUNK_MIN, KCHUNK_MAX]; FP32 uses a fixed slice length that bounds its
+// accumulators' rounding on coherent sums.
 static int kchunk_atoms(long long wgs, long long natoms, int prec) {
-    if (prec == DC_PREC_FP32) return KCHUNK_FP32;
-    if (prec == DC_PREC_FP16X3) return KCHUNK_FP16X3;
+    if (prec == DC_PREC_FP32) return KCHUNK_FP32;   // (FP32 with FP64 sums takes long slices)
     long long want = std::max<long long>(1, (1024 + wgs - 1) / std::max<long long>(wgs, 1));
     long long k = (natoms + want - 1) / want;
     k = std::min<long long>(std::max<long long>(k, KCHUNK_MIN), KCHUNK_MAX);
@@ -160,13 +159,18 @@ Plan plan_gemm(Grid &g, int forced, int only_mapping, int prec, long long natoms
             ps[0] = 1; ps[1] = gg.R0; ps[2] = gg.R;
         }
         gg.P = (long long)gg.R * gg.C;
+        const bool acc64 = prec == SCHEME_FP32_ACC64;
+        // FP32 with FP64 sums is built for a few tiles only; a forced tile
+        // outside them is ignored for it
+        const bool use_forced = forced >= 0 && (!acc64 || gemm_tile_info(forced).acc64);
         for (int t = 0; t < nt; ++t) {
-            if (forced >= 0 && t != forced) continue;
+            if (use_forced && t != forced) continue;
             TileInfo ti = gemm_tile_info(t);
-            if (forced < 0 && ti.split_only && prec == DC_PREC_FP32) continue;
+            if (!use_forced && ti.split_only && prec == DC_PREC_FP32) continue;
+            if (acc64 && !ti.acc64) continue;
             const double work = (double)padded(gg.R, ti.rt) * (double)padded(gg.C, ti.ct);
             double cost;
-            if (prec == DC_PREC_FP32 || forced >= 0) {
+            if (prec == DC_PREC_FP32 || acc64 || use_forced) {
                 cost = work * (1.0 + 0.5 * (16.0 / ti.rt + 16.0 / ti.ct));
             } else if (natoms <= 0) {
                 cost = work * ti.eff;
@@ -182,6 +186,7 @@ Plan plan_gemm(Grid &g, int forced, int only_mapping, int prec, long long natoms
                 best.gg = gg;
                 best.mapping = mapping;
                 best.tile = t;
+                best.scheme = prec;
                 g.ps[0] = ps[0]; g.ps[1] = ps[1]; g.ps[2] = ps[2];
                 g.plane = gg.P;
             }
@@ -209,17 +214,17 @@ int kchunk_for(const Plan &pl, long long natoms_batch, int prec) {
 // slices fill the gaps.  The candidates are the lengths in [KCHUNK_MIN,
 // KCHUNK_MAX] at which some segment's slice count changes; the shortest
 // predicted time wins, ties going to the longer slices (fewer planes to
-// reduce).  FP32 and FP16X3 search between half their fixed slice cap and
-// the cap.  Launches that already have 16 rounds of items or more, and
+// reduce).  FP32 searches between half its fixed slice cap and the cap.
+// Launches that already have 16 rounds of items or more, and
 // populations (many segments), keep the rule of kchunk_atoms.
 int kchunk_for(const Plan &pl, const std::vector<std::pair<int, int>> &segs, int prec) {
     long long natoms = 0;
     for (auto &sg : segs) natoms += std::max(sg.second, 0);
     const int k0 = kchunk_for(pl, natoms, prec);
     if (natoms <= 0) return k0;
-    // FP32 and FP16X3 cap their slices (accumulator rounding on coherent
-    // sums); shorter slices are allowed, down to half the cap
-    const bool capped = prec == DC_PREC_FP32 || prec == DC_PREC_FP16X3;
+    // FP32 caps its slices (accumulator rounding on coherent sums); shorter
+    // slices are allowed, down to half the cap
+    const bool capped = prec == DC_PREC_FP32;
     const int kmax = capped ? k0 : KCHUNK_MAX;
     const int kmin = capped ? k0 / 2 : KCHUNK_MIN;
     const TileInfo ti = gemm_tile_info(pl.tile);
@@ -294,7 +299,7 @@ void add_segments(SliceSet &s, const std::vector<std::pair<int, int>> &segs, int
 }
 
 int run_gemm(dc_ctx *c, const Plan &pl, int nslice, float2 *partial, const float *sq) {
-    const GemmGeom &gg = pl.gg;
+    const GemmGeom gg = launch_geom(c, pl.gg);
     if (nslice <= 0) return 0;
     HIPCHK(hipEventRecord(c->ev[2], c->st));
     // launch in chunks of slices so that the cancel flag is polled
@@ -302,8 +307,9 @@ int run_gemm(dc_ctx *c, const Plan &pl, int nslice, float2 *partial, const float
     for (int s0 = 0; s0 < nslice; s0 += per) {
         if (cancelled(c)) return set_err(c, DC_ERR_CANCEL, DC_ER_COMM, "cancelled");
         int ns = std::min(per, nslice - s0);
-        HIPCHK(launch_phase_gemm(c->prec, pl.tile, ptr<AtomRec>(c->rec), ptr<int>(c->slice_beg),
-                                 partial + (long long)s0 * gg.P, sq, gg, s0, ns, c->st));
+        HIPCHK(launch_phase_gemm(pl.scheme, pl.tile, ptr<AtomRec>(c->rec), ptr<int>(c->slice_beg),
+                                 partial + (long long)s0 * gg.P * planes_per_slice(pl.scheme), sq, gg, s0, ns,
+                                 c->st));
         c->stats.gemm_launches++;
     }
     HIPCHK(hipEventRecord(c->ev[3], c->st));
@@ -410,6 +416,19 @@ int dc_set_gemm_tile(dc_ctx *c, int tile) {
 }
 
 int dc_gemm_num_tiles(void) { return gemm_num_tiles(); }
+
+int dc_set_phase_dither(dc_ctx *c, int on) {
+    if (!c) return DC_ERR_ARG;
+    c->dither = on ? 1 : 0;
+    return DC_OK;
+}
+
+int dc_set_slice_atoms(dc_ctx *c, int n) {
+    if (!c) return DC_ERR_ARG;
+    if (n < 0 || n > (1 << 24)) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "slice length out of range");
+    c->slice_atoms = n;
+    return DC_OK;
+}
 
 int dc_set_adjoint_walk(dc_ctx *c, int walk) {
     if (!c) return DC_ERR_ARG;
@@ -524,7 +543,7 @@ int dc_strucf(dc_ctx *c, const double *xat, int64_t nxat, int64_t ld_xat, const 
 
     SliceSet ss;
     int off = 0;
-    add_segments(ss, {{0, (int)nxat}}, kchunk_for(pl, nxat, c->prec), off);
+    add_segments(ss, {{0, (int)nxat}}, slice_len(c, kchunk_for(pl, nxat, c->prec)), off);
     if (ss.beg.empty()) ss.beg.push_back(0);
     int nslice = (int)ss.type.size();
     ss.lot = {0, nslice};
@@ -559,6 +578,39 @@ int dc_strucf(dc_ctx *c, const double *xat, int64_t nxat, int64_t ld_xat, const 
     c->stats.total_ms = ms;
     c->stats.pairs = (double)nxat * (double)ntot;
     c->stats.slices = nslice;
+    return DC_OK;
+}
+
+// ----------------------------------------------------------------------------
+int dc_four_stltab(dc_ctx *c, const double xm[3], const double uin[3], const double vin[3], const double win[3],
+                   const int num[3], const double rten[9], int *istl) {
+    if (!c) return DC_ERR_ARG;
+    c->err.clear();
+    c->err_typ = DC_ER_NONE;
+    if (!xm || !uin || !vin || !win || !num || !rten || !istl)
+        return set_err(c, DC_ERR_ARG, DC_ER_LIB, "dc_four_stltab: null pointer");
+    for (int k = 0; k < 3; ++k)
+        if (num[k] < 1) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "dc_four_stltab: num < 1");
+    const long long ntot = (long long)num[0] * num[1] * num[2];
+    if (ntot > INT32_MAX) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "dc_four_stltab: grid too large");
+    HIPCHK(hipSetDevice(c->device));
+    Grid g{};
+    for (int k = 0; k < 3; ++k) {
+        g.n[k] = g.f[k] = num[k];
+        g.xm[k] = xm[k];
+        g.uin[k] = uin[k];
+        g.vin[k] = vin[k];
+        g.win[k] = win[k];
+    }
+    for (int k = 0; k < 9; ++k) g.rten[k] = rten[k];
+    g.P = ntot;
+    int st = 0;
+    if ((st = ensure(c, c->istl, sizeof(int) * (size_t)ntot))) return st;
+    HIPCHK(launch_istl(g, ptr<int>(c->istl), c->st));
+    HIPCHK(hipMemcpyAsync(istl, c->istl.p, sizeof(int) * (size_t)ntot, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    for (long long q = 0; q < ntot; ++q)
+        if (istl[q] > CFPKT) return set_err(c, DC_ERR_ISTL, DC_ER_FOUR, "sin(theta)/lambda beyond the form factor table");
     return DC_OK;
 }
 
@@ -694,13 +746,24 @@ int gemm_segments(dc_ctx *c, const Grid &g, const Plan &pl, GatherList &gl,
     HIPCHK(launch_prep(pa, ptr<AtomRec>(c->rec), c->st));
     int off = 0;
     ss = SliceSet();
-    add_segments(ss, gl.segs, kchunk_for(pl, gl.segs, c->prec), off);
+    add_segments(ss, gl.segs, slice_len(c, kchunk_for(pl, gl.segs, pl.scheme)), off);
     if (ss.beg.empty()) ss.beg.push_back(0);
-    nslice = (int)ss.type.size();
+    const int ngemm = (int)ss.type.size();
+    // schemes with two planes per slice (hi, lo): the reductions see each
+    // plane as a slice of the same type
+    const int pps = planes_per_slice(pl.scheme);
+    if (pps > 1) {
+        std::vector<int> t2;
+        for (int v : ss.type)
+            for (int q = 0; q < pps; ++q) t2.push_back(v);
+        ss.type.swap(t2);
+        for (int &v : ss.seg_first) v *= pps;
+    }
+    nslice = ngemm * pps;
     if ((st = upload(c, c->slice_beg, ss.beg.data(), ss.beg.size()))) return st;
     if ((st = upload(c, c->slice_type, ss.type.data(), std::max<size_t>(ss.type.size(), 1)))) return st;
     if ((st = ensure(c, c->partial, sizeof(float2) * gg.P * std::max(nslice, 1)))) return st;
-    return run_gemm(c, pl, nslice, ptr<float2>(c->partial));
+    return run_gemm(c, pl, ngemm, ptr<float2>(c->partial));
 }
 
 }  // namespace
@@ -762,7 +825,14 @@ int dc_four_run_dev(dc_ctx *c, const dc_run_desc *r, double *d_csf, double *d_ac
     if (lots)
         natoms_launch = std::min<long long>(natoms_launch, (long long)std::max(r->nlots, 1) * r->ls[0] * r->ls[1] *
                                                                r->ls[2] * c->ncatoms);
-    const Plan pl = plan_gemm(g, c->tile, -1, c->prec, natoms_launch);
+    // FP32 runs whose result is a catastrophic difference -- the <F>
+    // subtraction (fave != 0) or the cross-lot variance -- keep the GEMM's
+    // partial sums in FP64 (SCHEME_FP32_ACC64): at the Bragg pixels |F_l|
+    // exceeds |F_l - <F>| many times, and an FP32 running sum's rounding,
+    // relative to |F_l|, would stay in the difference.
+    const bool want_var = d_fmean || d_dvar;
+    const int scheme = c->prec == DC_PREC_FP32 && (r->fave != 0.0 || want_var) ? SCHEME_FP32_ACC64 : c->prec;
+    const Plan pl = plan_gemm(g, c->tile, -1, scheme, natoms_launch);
     int st = 0;
     if ((st = upload(c, c->cfact, r->cfact, 2 * (size_t)r->ncfact * c->nscat))) return st;
 
@@ -770,7 +840,6 @@ int dc_four_run_dev(dc_ctx *c, const dc_run_desc *r, double *d_csf, double *d_ac
     if ((st = ensure(c, c->acsf, sizeof(double) * 2 * g.P))) return st;
     if ((st = ensure(c, c->dsi, sizeof(double) * g.P))) return st;
     if ((st = ensure(c, c->csf, sizeof(double) * 2 * g.P))) return st;
-    const bool want_var = d_fmean || d_dvar;
     if (want_var) {
         if ((st = ensure(c, c->wm, sizeof(double) * 2 * g.P))) return st;
         if ((st = ensure(c, c->wv, sizeof(double) * g.P))) return st;
@@ -860,7 +929,7 @@ int dc_four_run_dev(dc_ctx *c, const dc_run_desc *r, double *d_csf, double *d_ac
         for (int k = 0; k < 3; ++k) gl_full.f[k] = g.n[k];
         gl_full.P = ntot;
         gl_full.friedel = 0;
-        const Plan pll = plan_gemm(gl_full, c->tile, -1, c->prec, (long long)(lp.size() / 3));
+        const Plan pll = plan_gemm(gl_full, c->tile, -1, scheme, (long long)(lp.size() / 3));
         GatherList gll;
         gll.segs.push_back({0, (int)(lp.size() / 3)});
         if ((st = gemm_segments(c, gl_full, pll, gll, lp.data(), ss, nslice, false))) return st;
@@ -884,7 +953,7 @@ int dc_four_run_dev(dc_ctx *c, const dc_run_desc *r, double *d_csf, double *d_ac
     // ---- lot loop (fourier_sup.f90:140-257) ----
     long long ncell = (long long)c->icc[0] * c->icc[1] * c->icc[2];
     const int nlots = lots ? r->nlots : 1;
-    const double plane_bytes = (double)sizeof(float2) * (double)g.plane;
+    const double plane_bytes = (double)sizeof(float2) * (double)g.plane * planes_per_slice(scheme);
     int lot = 0;
     while (lot < nlots) {
         if (cancelled(c)) return set_err(c, DC_ERR_CANCEL, DC_ER_COMM, "cancelled");

@@ -22,6 +22,7 @@ using namespace dc;
 struct BatchState {
     // key: the shape of the batch
     int nmembers = 0, nscat = 0, ncfact = 0, prec = -1, tile_forced = -2, autotune = -1, graphs = -1;
+    int slice_atoms = -1, dither = -1;
     bool want_grad = false, want_icalc = false, have_weight = false, any_b = false;
     std::vector<int64_t> atom_beg;
     std::vector<int> type;
@@ -79,7 +80,7 @@ bool same_shape(const BatchState *b, const dc_ctx *c, const dc_batch_desc *d, bo
     if (b->nmembers != d->nmembers || b->nscat != d->nscat || b->ncfact != d->ncfact || b->prec != c->prec ||
         b->tile_forced != c->tile || b->autotune != c->autotune || b->graphs != c->graphs ||
         b->want_grad != want_grad || b->want_icalc != want_icalc || b->have_weight != have_weight ||
-        b->any_b != any_b || b->walk_req != c->adj_walk)
+        b->any_b != any_b || b->walk_req != c->adj_walk || b->slice_atoms != c->slice_atoms || b->dither != c->dither)
         return false;
     if (std::memcmp(b->eck, d->eck, sizeof(b->eck)) || std::memcmp(b->vi, d->vi, sizeof(b->vi)) ||
         std::memcmp(b->inc, d->inc, sizeof(b->inc)))
@@ -150,6 +151,8 @@ int build_state(dc_ctx *c, const dc_batch_desc *d, BatchState &b) {
     b.prec = c->prec;
     b.walk_req = c->adj_walk;
     b.tile_forced = c->tile;
+    b.slice_atoms = c->slice_atoms;
+    b.dither = c->dither;
     b.autotune = c->autotune;
     b.graphs = c->graphs;
     b.natoms = d->atom_beg[d->nmembers];
@@ -188,7 +191,7 @@ int build_state(dc_ctx *c, const dc_batch_desc *d, BatchState &b) {
         }
     }
     b.nsorted = (int)b.order.size();
-    const int kchunk = kchunk_for(b.pl, b.nsorted, b.prec);
+    const int kchunk = slice_len(c, kchunk_for(b.pl, b.nsorted, b.prec));
     int off = 0;
     const int na = adjoint_tile_atoms();
     size_t sg = 0;
@@ -265,7 +268,8 @@ int enqueue(dc_ctx *c, BatchState &b, int tile, int nwg, bool prep, bool rest) {
         if (b.nsorted > 0) HIPCHK(launch_prep(pa, ptr<AtomRec>(b.rec), c->st));
     }
     HIPCHK(launch_phase_gemm(c->prec, tile, ptr<AtomRec>(b.rec), ptr<int>(b.sbeg), ptr<float2>(b.partial),
-                             b.any_b ? ptr<float>(b.sq) : nullptr, gg, 0, b.nslice, c->st, nwg, ptr<int>(b.ssq)));
+                             b.any_b ? ptr<float>(b.sq) : nullptr, launch_geom(c, gg), 0, b.nslice, c->st, nwg,
+                             ptr<int>(b.ssq)));
     if (!rest) return 0;
     BatchReduceArgs ra{};
     ra.g = b.g;

@@ -33,7 +33,23 @@ static_assert(sizeof(AtomRec) == 48, "AtomRec layout");
 struct GemmGeom {
     int R, R0, C, C0;
     long long P;
+    uint32_t dither;   // per-atom phase rotation salt (0: none), see dither_turn
 };
+
+// Per-atom phase rotation of the phase GEMM: a pseudo-random turn fraction
+// (splitmix64 finaliser of the atom index and a salt).  Row operands carry
+// +psi and column operands -psi, so every product is unchanged in exact
+// arithmetic, while the rounding of the two operands (sin/cos polynomial,
+// table products, the split into hi/lo halves) differs from atom to atom.
+// Atoms with equal phases (a crystal at its Bragg peaks) then no longer add
+// the same rounding error N times: those errors grow as sqrt(N), not N.
+__host__ __device__ inline uint64_t dither_turn(uint32_t k, uint32_t salt) {
+    if (salt == 0) return 0;
+    uint64_t z = ((uint64_t)salt << 32 | k) + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
 
 // fraction of a FP64 value as 64-bit fixed point (exact for |t| < 2^11:
 // every mantissa bit of t - floor(t) lands inside the 64 fraction bits).

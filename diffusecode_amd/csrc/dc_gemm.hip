@@ -17,10 +17,19 @@
 // tile start.  It splits exactly into
 //     [P(r0) + 16 rh s + k(16 rh) w]  +  [rl s + delta w],  delta in {0, 1},
 // so each element is the product of two table entries, TH[rh] * TL_delta[rl],
-// whose phases are exact integers turned into (cos, sin) by sincos_turn (the
-// low tables as short products of exact anchors).  Per 16-atom stage a
-// work-group evaluates 16 x (24 + RT/16 + CT/16) sincos and about
-// (RT + CT) x 16 complex products instead of (RT + CT) x 16 sincos.
+// whose phases are exact integers turned into (cos, sin) by sincos_turn.  Per
+// 16-atom stage a work-group evaluates 16 x (64 + RT/16 + CT/16) sincos and
+// about (RT + CT) x 16 complex products instead of (RT + CT) x 16 sincos.
+//
+// Per-atom phase rotation (dither_turn).  Every A element of atom n carries
+// an extra pseudo-random phase +psi_n and every B element -psi_n (the low
+// tables a second one, psi2_n, which the high tables take back), so the
+// products are unchanged in exact arithmetic.  Without it, atoms of equal
+// phase -- a crystal at its Bragg peaks -- round identically, and those
+// errors add coherently over the whole slice (shifted fcc 40^3 crystal,
+// 256,000 atoms in phase: BF16X3 2.3e-5 -> 5.0e-6, FP16X3 over 4,096-atom
+// slices 4.9e-5 -> 4.0e-7, FP16X2 2.7e-4 -> 7.5e-7, FP32 3.2e-7 -> 3.9e-8
+// in max|dI|/max(I); measured cost 0.3-3% of the launch).
 //
 // The complex product is embedded as a real GEMM with the atom dimension
 // interleaved (re, im):  Cr = [Ar -Ai].[Br Bi],  Ci = [Ai Ar].[Br Bi]; each
@@ -28,6 +37,10 @@
 //
 // Schemes (template SCHEME):
 //   SCH_FP32   v_mfma_f32_16x16x4_f32, FP32 operands: the plain drop-in.
+//   SCH_FP32D  the same products with FP64 partial sums every 4 atoms, for
+//              runs whose result is a catastrophic difference (the <F>
+//              subtraction, the cross-lot variance); each slice is written
+//              as two FP32 planes (hi, lo).
 //   SCH_BF16X3 v_mfma_f32_16x16x32_bf16, x = xh + xl split on both sides,
 //              products xh*yh + xh*yl + xl*yh, FP32 accumulation.
 //   SCH_FP16X2 v_mfma_f32_16x16x32_f16, A split (ah + al), B single half,
@@ -56,6 +69,9 @@ constexpr int SCH_FP32 = 0;
 constexpr int SCH_BF16X3 = 1;
 constexpr int SCH_FP16X2 = 2;
 constexpr int SCH_FP16X3 = 3;
+constexpr int SCH_FP32D = SCHEME_FP32_ACC64;
+// FP32 operands on v_mfma_f32_16x16x4_f32 (FP32D: partial sums kept in FP64)
+__host__ __device__ constexpr bool fp32ops(int s) { return s == SCH_FP32 || s == SCH_FP32D; }
 // three products (hi.hi, hi.lo, lo.hi) with both sides split
 __host__ __device__ constexpr bool split3(int s) { return s == SCH_BF16X3 || s == SCH_FP16X3; }
 // FP16 operands (v_mfma_f32_16x16x32_f16)
@@ -116,10 +132,14 @@ struct Elems {
     uint32_t b_hi[T::CT * 16], b_lo[(split3(SCHEME) ? T::CT : 1) * 16];
 };
 template <class T>
-struct Elems<SCH_FP32, T> {
+struct ElemsF32 {
     float a[T::RT * FST];
     float b[T::CT * FST];
 };
+template <class T>
+struct Elems<SCH_FP32, T> : ElemsF32<T> {};
+template <class T>
+struct Elems<SCH_FP32D, T> : ElemsF32<T> {};
 
 // Per-stage tables, rows of KS atoms: TL_d[l] (l < 16, wrap state d) and
 // TH[h] for both sides, laid out al[2][16] | ah[RH] | bl[2][16] | bh[CH].
@@ -156,41 +176,44 @@ __device__ __forceinline__ float2 cis(uint64_t ph) {
 
 // ---------------------------------------------------------------------------
 // Tables of one stage, built by all threads (pt = thread index).  Each atom
-// a = pt % 16 has NPT/16 helper threads r = pt / 16:
+// a = pt % 16 (index katom of the record array) has NPT/16 helper threads
+// r = pt / 16:
 //   r < 8      low tables of side r/4, entries l = 4m .. 4m+3 (m = r % 4):
-//              TL0[4m+j] = cis(4m s) cis(s)^j, TL1 = TL0 * cis(w)
-//              (3 exact anchors + 7 products instead of 8 sincos)
-//   r >= 8     high-table rows h = r-8, r-8+NPT/16-8, ... of [A (RH) | B (CH)]
+//              TL0[l] = cis(l s + psi2), TL1[l] = cis(l s + w + psi2), each
+//              evaluated from its exact phase
+//   r >= 8     high-table rows h = r-8, r-8+NPT/16-8, ... of [A (RH) | B (CH)],
+//              A rows rotated by psi - psi2 and B rows by -psi - psi2
+// so every A element carries +psi and every B element -psi (dither_turn):
+// the products are unchanged, their rounding errors differ per atom.
 template <class T>
 __device__ __forceinline__ void build_tables(Tables<T> &tb, const int *__restrict__ a_k16,
                                              const int *__restrict__ b_k16, const AtomRec &ar, bool live,
-                                             const Side &sa, const Side &sb, int pt, float wscale) {
+                                             const Side &sa, const Side &sb, int pt, float wscale, uint32_t katom,
+                                             uint32_t salt) {
     const int a = pt & (KS - 1), r = pt >> 4;
     constexpr int NR = T::NPT / KS;           // helper threads per atom
+    const uint64_t psi = dither_turn(katom, salt);
+    const uint64_t psi2 = psi * 0xD6E8FEB86659FD93ull;
     if (r == 0) tb.dmp[a] = live ? ar.pad * 1.4426950408889634f : 0.0f;
     if (r < 8) {
         const bool sideb = r >= 4;
         const int m = r & 3;
         const uint64_t step = sideb ? ar.ca : ar.ra;
         const uint64_t wrp = sideb ? ar.ca1 - (uint64_t)sb.period * ar.ca : ar.ra1 - (uint64_t)sa.period * ar.ra;
-        const float2 e4 = cis((uint64_t)(4 * m) * step);
-        const float2 e1 = cis(step);
-        const float2 ew = cis(wrp);
         float2 *t0 = sideb ? tb.bl[0] : tb.al[0];
         float2 *t1 = sideb ? tb.bl[1] : tb.al[1];
-        float2 v = e4;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            t0[(4 * m + j) * KS + a] = v;
-            t1[(4 * m + j) * KS + a] = cmul(v, ew);
-            v = cmul(v, e1);
+            const uint64_t ph = (uint64_t)(4 * m + j) * step + psi2;
+            t0[(4 * m + j) * KS + a] = cis(ph);
+            t1[(4 * m + j) * KS + a] = cis(ph + wrp);
         }
     } else {
         const float w = live ? ar.w * wscale : 0.0f;   // wscale: power of two, exact
         const uint64_t wa = ar.ra1 - (uint64_t)sa.period * ar.ra;
         const uint64_t wb = ar.ca1 - (uint64_t)sb.period * ar.ca;
-        const uint64_t pa = (uint64_t)sa.a0 * ar.ra + (uint64_t)sa.hi0 * ar.ra1;
-        const uint64_t pb = ar.p0 + (uint64_t)sb.a0 * ar.ca + (uint64_t)sb.hi0 * ar.ca1;
+        const uint64_t pa = (uint64_t)sa.a0 * ar.ra + (uint64_t)sa.hi0 * ar.ra1 + psi - psi2;
+        const uint64_t pb = ar.p0 + (uint64_t)sb.a0 * ar.ca + (uint64_t)sb.hi0 * ar.ca1 - psi - psi2;
         for (int q = r - 8; q < T::RH + T::CH; q += NR - 8) {
             const bool sideb = q >= T::RH;
             const int h = sideb ? q - T::RH : q;
@@ -237,7 +260,7 @@ __device__ __forceinline__ void elems_group(Elems<SCHEME, T> &el, const float2 *
             v[q] = make_float2(v[q].x * f, v[q].y * f);
         }
     }
-    if constexpr (SCHEME == SCH_FP32) {
+    if constexpr (fp32ops(SCHEME)) {
         float *dst = (isA ? el.a : el.b) + row * FST + 8 * g;
         *reinterpret_cast<f32x4 *>(dst) = f32x4{v[0].x, v[0].y, v[1].x, v[1].y};
         *reinterpret_cast<f32x4 *>(dst + 4) = f32x4{v[2].x, v[2].y, v[3].x, v[3].y};
@@ -294,7 +317,7 @@ template <int SCHEME, class T>
 __device__ __forceinline__ void mfma_stage(const Elems<SCHEME, T> &el, f32x4 (&accR)[T::WR][T::WC],
                                            f32x4 (&accI)[T::WR][T::WC], int wr0, int wc0) {
     const int lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
-    if constexpr (SCHEME == SCH_FP32) {
+    if constexpr (fp32ops(SCHEME)) {
 #pragma unroll
         for (int ks = 0; ks < KS / 2; ++ks) {
             float av[T::WR], bR[T::WC], bI[T::WC];
@@ -405,6 +428,55 @@ __device__ __forceinline__ void mfma_stage(const Elems<SCHEME, T> &el, f32x4 (&a
     }
 }
 
+// FP32D: the MFMAs of one stage with FP64 partial sums.  v_mfma_f32_16x16x4_f32
+// adds each of its 4 products to the accumulator with its own rounding, so
+// a chain's error grows with the size of its running sum; here a chain
+// covers two k-steps (4 atoms, 8 products per output) and is then added
+// into FP64 sums.  On coherent sums (the Bragg pixels that the <F>
+// subtraction and the cross-lot variance cancel) the error of a slice
+// stays near one rounding of a 4-atom sum, not of the whole slice.
+template <class T>
+__device__ __forceinline__ void mfma_stage_d(const Elems<SCH_FP32D, T> &el, f32x4 (&accR)[T::WR][T::WC],
+                                             f32x4 (&accI)[T::WR][T::WC], double (&sR)[T::WR][T::WC][4],
+                                             double (&sI)[T::WR][T::WC][4], int wr0, int wc0) {
+    const int lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
+#pragma unroll
+    for (int ks = 0; ks < KS / 2; ++ks) {
+        float av[T::WR], bR[T::WC], bI[T::WC];
+#pragma unroll
+        for (int i = 0; i < T::WR; ++i) av[i] = el.a[(wr0 + 16 * i + li) * FST + 4 * ks + lg];
+        const int atom = 2 * ks + (lane >> 5);
+        const bool im = (lg & 1) != 0;
+#pragma unroll
+        for (int j = 0; j < T::WC; ++j) {
+            const float2 b = *reinterpret_cast<const float2 *>(&el.b[(wc0 + 16 * j + li) * FST + 2 * atom]);
+            bR[j] = im ? -b.y : b.x;
+            bI[j] = im ? b.x : b.y;
+        }
+        const bool first = (ks & 1) == 0;
+#pragma unroll
+        for (int j = 0; j < T::WC; ++j)
+#pragma unroll
+            for (int i = 0; i < T::WR; ++i) {
+                const f32x4 zr = first ? f32x4{0.f, 0.f, 0.f, 0.f} : accR[i][j];
+                const f32x4 zi = first ? f32x4{0.f, 0.f, 0.f, 0.f} : accI[i][j];
+                accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i], bR[j], zr, 0, 0, 0);
+                accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i], bI[j], zi, 0, 0, 0);
+            }
+        if (!first) {
+#pragma unroll
+            for (int j = 0; j < T::WC; ++j)
+#pragma unroll
+                for (int i = 0; i < T::WR; ++i)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        sR[i][j][v] += (double)accR[i][j][v];
+                        sI[i][j][v] += (double)accI[i][j][v];
+                    }
+        }
+    }
+}
+
 template <int SCHEME, class T, bool DAMP>
 __global__ void __launch_bounds__(T::NT)
 phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice_beg, float2 *__restrict__ partial,
@@ -448,13 +520,30 @@ phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice
 
         f32x4 accR[T::WR][T::WC], accI[T::WR][T::WC];
         // FP32 scheme: two-level accumulation.  The 16x16x4 MFMA rounds its
-        // FP32 accumulator once per 2 atoms; on coherent sums (Bragg peaks of
-        // displaced crystals) that error grows with the number of atoms, so
-        // the MFMAs accumulate 64 atoms (4 stages) at a time and the groups
-        // are added into a second set of FP32 sums.
+        // FP32 accumulator after every product; on coherent sums (Bragg peaks
+        // of displaced crystals) that error grows with the size of the
+        // partial sums, so the MFMAs accumulate 64 atoms (4 stages) at a time
+        // and the groups are added into a second set of FP32 sums.  (A flush
+        // after every stage measured slower, 60.0 vs 54.6 ms on the 161^3
+        // headline, and less accurate on its coherent sums, 3.5e-7 vs 3.9e-8:
+        // the second level then adds 64 nearly equal stage sums per slice,
+        // whose rounding no longer averages out.)
         constexpr bool TWO = SCHEME == SCH_FP32;
         constexpr int SR = TWO ? T::WR : 1, SC = TWO ? T::WC : 1;
         f32x4 sumR[SR][SC], sumI[SR][SC];
+        // FP32D: FP64 partial sums (see mfma_stage_d)
+        constexpr bool D64 = SCHEME == SCH_FP32D;
+        constexpr int DR = D64 ? T::WR : 1, DC = D64 ? T::WC : 1;
+        double dR[DR][DC][4], dI[DR][DC][4];
+#pragma unroll
+        for (int i = 0; i < DR; ++i)
+#pragma unroll
+            for (int j = 0; j < DC; ++j)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    dR[i][j][v] = 0.0;
+                    dI[i][j][v] = 0.0;
+                }
 #pragma unroll
         for (int i = 0; i < T::WR; ++i)
 #pragma unroll
@@ -480,7 +569,7 @@ phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice
         };
         // wide column blocks are register-bound: they load each record one
         // stage ahead only, the others three stages ahead
-        constexpr bool DEEP = T::WC <= 4;
+        constexpr bool DEEP = T::WC <= 4 && !D64;
         const AtomRec r0 = fetch(0);
         const AtomRec r1 = fetch(nst > 1 ? 1 : 0);
         AtomRec rn = r1;
@@ -507,10 +596,10 @@ phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice
                 wunscale = ldexpf(1.0f, e);
             }
         }
-        build_tables<T>(sm.t[0], sm.a_k16, sm.b_k16, r0, kbeg + at < kend, sa, sb, pt, wscale);
+        build_tables<T>(sm.t[0], sm.a_k16, sm.b_k16, r0, kbeg + at < kend, sa, sb, pt, wscale, kbeg + at, g.dither);
         __syncthreads();
         build_elems<SCHEME, T, DAMP>(sm.e[0], sm.t[0], sm.a_wpos, sm.b_wpos, sm.s_a, sm.s_b, wave);
-        if (nst > 1) build_tables<T>(sm.t[1], sm.a_k16, sm.b_k16, r1, kbeg + KS + at < kend, sa, sb, pt, wscale);
+        if (nst > 1) build_tables<T>(sm.t[1], sm.a_k16, sm.b_k16, r1, kbeg + KS + at < kend, sa, sb, pt, wscale, kbeg + KS + at, g.dither);
         __syncthreads();
         for (int s = 0; s < nst; ++s) {
             const int cur = s & 1, nxt = cur ^ 1;
@@ -535,8 +624,14 @@ phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice
             const bool mfma_first = split3(SCHEME) && T::WC > 4 && ((wave >> 2) & 1) != 0;
             if (mfma_first && live) mfma_stage<SCHEME, T>(sm.e[cur], accR, accI, wr0, wc0);
             if (s + 1 < nst) build_elems<SCHEME, T, DAMP>(sm.e[nxt], sm.t[nxt], sm.a_wpos, sm.b_wpos, sm.s_a, sm.s_b, wave);
-            if (s + 2 < nst) build_tables<T>(sm.t[cur], sm.a_k16, sm.b_k16, rt, kbeg + (s + 2) * KS + at < kend, sa, sb, pt, wscale);
-            if (!mfma_first && live) mfma_stage<SCHEME, T>(sm.e[cur], accR, accI, wr0, wc0);
+            if (s + 2 < nst) build_tables<T>(sm.t[cur], sm.a_k16, sm.b_k16, rt, kbeg + (s + 2) * KS + at < kend, sa, sb, pt, wscale,
+                                              kbeg + (s + 2) * KS + at, g.dither);
+            if (!mfma_first && live) {
+                if constexpr (D64)
+                    mfma_stage_d<T>(sm.e[cur], accR, accI, dR, dI, wr0, wc0);
+                else
+                    mfma_stage<SCHEME, T>(sm.e[cur], accR, accI, wr0, wc0);
+            }
             if constexpr (TWO) {
                 if ((s & 3) == 3 || s + 1 == nst) {
 #pragma unroll
@@ -564,7 +659,9 @@ phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice
 
         // epilogue: lane holds rows 4*lg + v (v = 0..3) of column li of each 16x16 tile
         const int lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
-        float2 *out = partial + (long long)(slice - slice0) * g.P;
+        // FP32D writes each slice as two planes, hi = fl(s) and lo = fl(s - hi),
+        // which the FP64 reductions add like two slices of the same type
+        float2 *out = partial + (long long)(slice - slice0) * g.P * (D64 ? 2 : 1);
 #pragma unroll
         for (int i = 0; i < T::WR; ++i)
 #pragma unroll
@@ -575,10 +672,17 @@ phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice
                 for (int v = 0; v < 4; ++v) {
                     const int r = row0 + wr0 + 16 * i + 4 * lg + v;
                     if (r < g.R) {
-                        if constexpr (f16ops(SCHEME))
-                            out[r + (long long)g.R * c] = make_float2(accR[i][j][v] * wunscale, accI[i][j][v] * wunscale);
-                        else
-                            out[r + (long long)g.R * c] = make_float2(accR[i][j][v], accI[i][j][v]);
+                        const long long o = r + (long long)g.R * c;
+                        if constexpr (D64) {
+                            const double xr = dR[i][j][v], xi = dI[i][j][v];
+                            const float hr = (float)xr, hi = (float)xi;
+                            out[o] = make_float2(hr, hi);
+                            out[o + g.P] = make_float2((float)(xr - (double)hr), (float)(xi - (double)hi));
+                        } else if constexpr (f16ops(SCHEME)) {
+                            out[o] = make_float2(accR[i][j][v] * wunscale, accI[i][j][v] * wunscale);
+                        } else {
+                            out[o] = make_float2(accR[i][j][v], accI[i][j][v]);
+                        }
                     }
                 }
             }
@@ -607,11 +711,17 @@ using CfgH = TileCfg<1, 11, 8, 1>;  // 128 x 176
 // 1.24-1.39.  The smaller tiles generate more operand elements per product
 // and have more work items per launch, each with its own set-up.
 static const TileInfo kTiles[] = {
-    {CfgA::RT, CfgA::CT, CfgA::NT, 1, 1.17f}, {CfgB::RT, CfgB::CT, CfgB::NT, 0, 1.28f},
-    {CfgC::RT, CfgC::CT, CfgC::NT, 0, 1.62f}, {CfgD::RT, CfgD::CT, CfgD::NT, 1, 2.30f},
-    {CfgE::RT, CfgE::CT, CfgE::NT, 0, 1.65f}, {CfgF::RT, CfgF::CT, CfgF::NT, 0, 6.30f},
-    {CfgG::RT, CfgG::CT, CfgG::NT, 1, 1.00f}, {CfgH::RT, CfgH::CT, CfgH::NT, 1, 1.33f},
+    {CfgA::RT, CfgA::CT, CfgA::NT, 1, 1.17f, 0}, {CfgB::RT, CfgB::CT, CfgB::NT, 0, 1.28f, 0},
+    {CfgC::RT, CfgC::CT, CfgC::NT, 0, 1.62f, 1}, {CfgD::RT, CfgD::CT, CfgD::NT, 1, 2.30f, 0},
+    {CfgE::RT, CfgE::CT, CfgE::NT, 0, 1.65f, 1}, {CfgF::RT, CfgF::CT, CfgF::NT, 0, 6.30f, 1},
+    {CfgG::RT, CfgG::CT, CfgG::NT, 1, 1.00f, 0}, {CfgH::RT, CfgH::CT, CfgH::NT, 1, 1.33f, 0},
 };
+
+// FP32D keeps 2 x WR x WC x 4 FP64 sums per lane beside the FP32 chains: only
+// the tiles with at most 8 MFMA blocks per wave (64 x 256, 128 x 128 / 8
+// waves, 16 x 256) fit the register file without spilling.
+template <int SCHEME, class T>
+constexpr bool tile_built() { return SCHEME != SCH_FP32D || T::WR * T::WC <= 8; }
 
 int gemm_num_tiles() { return (int)(sizeof(kTiles) / sizeof(kTiles[0])); }
 TileInfo gemm_tile_info(int t) { return kTiles[t]; }
@@ -627,22 +737,33 @@ static void launch_one(const AtomRec *rec, const int *slice_beg, float2 *partial
         phase_gemm_kernel<SCHEME, T, false><<<grid, dim3(T::NT), 0, st>>>(rec, slice_beg, partial, sq, slice_sq, g, slice0, (int)items);
 }
 
+template <int SCHEME, class T>
+static hipError_t launch_if_built(const AtomRec *rec, const int *slice_beg, float2 *partial, const float *sq,
+                                  const int *slice_sq, const GemmGeom &g, int slice0, int nslice, int nwg,
+                                  hipStream_t st) {
+    if constexpr (tile_built<SCHEME, T>()) {
+        launch_one<SCHEME, T>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st);
+        return hipGetLastError();
+    } else {
+        return hipErrorInvalidValue;
+    }
+}
+
 template <int SCHEME>
 static hipError_t launch_scheme(int tile, const AtomRec *rec, const int *slice_beg, float2 *partial, const float *sq,
                                 const int *slice_sq, const GemmGeom &g, int slice0, int nslice, int nwg,
                                 hipStream_t st) {
     switch (tile) {
-    case 0: launch_one<SCHEME, CfgA>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st); break;
-    case 1: launch_one<SCHEME, CfgB>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st); break;
-    case 2: launch_one<SCHEME, CfgC>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st); break;
-    case 3: launch_one<SCHEME, CfgD>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st); break;
-    case 4: launch_one<SCHEME, CfgE>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st); break;
-    case 5: launch_one<SCHEME, CfgF>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st); break;
-    case 6: launch_one<SCHEME, CfgG>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st); break;
-    case 7: launch_one<SCHEME, CfgH>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st); break;
+    case 0: return launch_if_built<SCHEME, CfgA>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st);
+    case 1: return launch_if_built<SCHEME, CfgB>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st);
+    case 2: return launch_if_built<SCHEME, CfgC>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st);
+    case 3: return launch_if_built<SCHEME, CfgD>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st);
+    case 4: return launch_if_built<SCHEME, CfgE>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st);
+    case 5: return launch_if_built<SCHEME, CfgF>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st);
+    case 6: return launch_if_built<SCHEME, CfgG>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st);
+    case 7: return launch_if_built<SCHEME, CfgH>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st);
     default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 hipError_t launch_phase_gemm(int scheme, int tile, const AtomRec *rec, const int *slice_beg, float2 *partial,
@@ -658,6 +779,7 @@ hipError_t launch_phase_gemm(int scheme, int tile, const AtomRec *rec, const int
     case SCH_BF16X3: return launch_scheme<SCH_BF16X3>(tile, rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st);
     case SCH_FP16X2: return launch_scheme<SCH_FP16X2>(tile, rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st);
     case SCH_FP16X3: return launch_scheme<SCH_FP16X3>(tile, rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st);
+    case SCH_FP32D: return launch_scheme<SCH_FP32D>(tile, rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st);
     default: return hipErrorInvalidValue;
     }
 }

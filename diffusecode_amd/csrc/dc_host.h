@@ -25,6 +25,8 @@ struct dc_ctx {
     int autotune = 1;               // batch: time tile shapes on the first call of a shape
     int graphs = 1;                 // batch: capture the population step as a hipGraph
     int adj_walk = 0;               // split adjoint column walk: 0 planned, else forced (short grids)
+    int dither = 1;                 // per-atom phase rotation of the phase GEMM (dither_turn)
+    int slice_atoms = 0;            // forced atoms per GEMM slice, 0 = planned
     BatchState *batch = nullptr;    // cached plan / buffers / graph of the last batch shape
     PostState *post = nullptr;      // FFT plans and buffers of the post-processing calls
     NufftState *nufft = nullptr;    // fine grid, plans and buffers of the NUFFT technique
@@ -120,6 +122,7 @@ struct Plan {
     GemmGeom gg{};
     int mapping = 0;
     int tile = 0;
+    int scheme = 0;              // GEMM scheme the plan is for (DC_PREC_* or SCHEME_FP32_ACC64)
 };
 
 // Slices over a list of (type, count) segments laid out consecutively.
@@ -137,6 +140,14 @@ int kchunk_for(const Plan &pl, long long natoms_batch, int prec);
 int kchunk_for(const Plan &pl, const std::vector<std::pair<int, int>> &segs, int prec);
 void add_segments(SliceSet &s, const std::vector<std::pair<int, int>> &segs, int kchunk, int &atom_off);
 int run_gemm(dc_ctx *c, const Plan &pl, int nslice, float2 *partial, const float *sq = nullptr);
+// atoms per slice: the context's forced length (dc_set_slice_atoms) or the planned one
+inline int slice_len(const dc_ctx *c, int planned) { return c->slice_atoms > 0 ? c->slice_atoms : planned; }
+// the plan's geometry with the context's dither salt
+inline GemmGeom launch_geom(const dc_ctx *c, const GemmGeom &g) {
+    GemmGeom r = g;
+    r.dither = c->dither ? 0x2545F491u : 0u;
+    return r;
+}
 void batch_release(dc_ctx *c);   // frees the cached batch plan, buffers and graph
 void post_release(dc_ctx *c);    // frees the FFT plans and buffers of the post-processing calls
 void nufft_release(dc_ctx *c);   // frees the NUFFT fine grid, plans and buffers

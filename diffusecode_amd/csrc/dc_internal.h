@@ -40,17 +40,27 @@ struct Grid {
 };
 
 // sin(theta)/lambda bin, four_stltab's expression (fourier_sup.f90:2663-2674,
-// skalpro metric.f90:103) evaluated left to right with no contraction so the
-// bins equal the host restatement bit for bit.
-__device__ inline int istl_point(const Grid &g, int i, int j, int l) {
+// skalpro metric.f90:90-105): h(k) is summed in FP64 and rounded to FP32 by
+// REAL() of default kind before the metric is applied in FP64; evaluated
+// left to right with no contraction so the bins equal the host restatement
+// bit for bit.
+__host__ __device__ inline double stl_h(double xm, double u, double v, double w, int i, int j, int l) {
+#pragma clang fp contract(off)
+    return (double)(float)(xm + u * (double)i + v * (double)j + w * (double)l);
+}
+// the bin as four_stltab stores it (NINT, no clamp; values beyond 2^30 saturate)
+__device__ inline int istl_raw(const Grid &g, int i, int j, int l) {
 #pragma clang fp contract(off)
     double h[3], t[3];
-    for (int k = 0; k < 3; ++k)
-        h[k] = g.xm[k] + g.uin[k] * (double)i + g.vin[k] * (double)j + g.win[k] * (double)l;
+    for (int k = 0; k < 3; ++k) h[k] = stl_h(g.xm[k], g.uin[k], g.vin[k], g.win[k], i, j, l);
     for (int r = 0; r < 3; ++r) t[r] = g.rten[r] * h[0] + g.rten[r + 3] * h[1] + g.rten[r + 6] * h[2];
     double q2 = (h[0] * t[0] + h[1] * t[1] + h[2] * t[2]) / 4.0;
     double v = __builtin_sqrt(q2) * (1.0 / CFINC);
-    int b = (int)__builtin_round(v);
+    return (int)__builtin_fmin(__builtin_round(v), 1073741824.0);
+}
+// the bin used to index cfact (the run checks the grid's largest bin first)
+__device__ inline int istl_point(const Grid &g, int i, int j, int l) {
+    const int b = istl_raw(g, i, j, l);
     return b < 0 ? 0 : (b > CFPKT ? CFPKT : b);
 }
 
@@ -58,7 +68,13 @@ struct TileInfo {
     int rt, ct, nt;
     int split_only;         // tile offered to the split schemes only (its FP32 variant spills)
     float eff;              // split schemes: time per padded output x atom, relative to 256 x 176
+    int acc64;              // built for SCHEME_FP32_ACC64
 };
+
+// GEMM schemes: DC_PREC_FP32 .. DC_PREC_FP16X3 (0..3) and the FP32 scheme
+// with FP64 partial sums, which writes two planes (hi, lo) per slice.
+constexpr int SCHEME_FP32_ACC64 = 4;
+inline int planes_per_slice(int scheme) { return scheme == SCHEME_FP32_ACC64 ? 2 : 1; }
 
 struct ReduceArgs {
     Grid g;

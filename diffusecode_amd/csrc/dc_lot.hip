@@ -274,14 +274,15 @@ __global__ void strucf_out_kernel(ReduceArgs a, double *__restrict__ tcsf, long 
     tcsf[2 * q + 1] = fi;
 }
 
-// Diagnostic: bins of the computed region (compared with the host restatement).
+// four_stltab over the full grid (fourier_sup.f90:2660-2685), abscissa fastest:
+// the unclamped bins, which the caller checks against CFPKT.
 __global__ void istl_kernel(Grid g, int *__restrict__ out) {
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long ntot = (long long)g.n[0] * g.n[1] * g.n[2];
     if (q >= ntot) return;
     const int i = (int)(q % g.n[0]);
     const long long qq = q / g.n[0];
-    out[q] = istl_point(g, i, (int)(qq % g.n[1]), (int)(qq / g.n[1]));
+    out[q] = istl_raw(g, i, (int)(qq % g.n[1]), (int)(qq / g.n[1]));
 }
 
 static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }

@@ -141,7 +141,7 @@ int model_forward(dc_ctx *c, const dc_model_desc *m, ModelPlan &mp) {
     const GemmGeom &gg = mp.pl.gg;
     SliceSet ss;
     int off = 0;
-    add_segments(ss, mp.segs, kchunk_for(mp.pl, mp.segs, c->prec), off);
+    add_segments(ss, mp.segs, slice_len(c, kchunk_for(mp.pl, mp.segs, c->prec)), off);
     if (ss.beg.empty()) ss.beg.push_back(0);
     const int nslice = (int)ss.type.size();
     ss.lot = {0, nslice};
@@ -342,7 +342,7 @@ int dc_model_jvp(dc_ctx *c, const dc_model_desc *m, const double *fcalc, int ndi
     if (m->bdw && (st = upload(c, c->m_bd, m->bdw, (size_t)m->natoms))) return st;
     if ((st = ensure(c, c->m_di, sizeof(double) * (size_t)P * std::max(ndir, 1) * (df ? 3 : 1)))) return st;
     // directions per batch so that the slice planes fit the budget
-    const int kchunk = kchunk_for(mp.pl, (long long)ns * 5, c->prec);
+    const int kchunk = slice_len(c, kchunk_for(mp.pl, (long long)ns * 5, c->prec));
     long long slices_per_dir = 0;
     for (auto &sg : mp.segs) slices_per_dir += 5 * ((sg.second + kchunk - 1) / kchunk);
     const double dir_bytes = (double)std::max<long long>(slices_per_dir, 1) * gg.P * sizeof(float2);

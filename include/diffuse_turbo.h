@@ -54,7 +54,7 @@ extern "C" {
 #define DC_PREC_FP32 0          /* plain FP32 MFMA (the drop-in)            */
 #define DC_PREC_BF16X3 1        /* split BF16 hi/lo, 3 MFMA products        */
 #define DC_PREC_FP16X2 2        /* split FP16 hi/lo on one side, 2 products */
-#define DC_PREC_FP16X3 3        /* split FP16 hi/lo on both sides, 3 products, slices of <= 512 atoms */
+#define DC_PREC_FP16X3 3        /* split FP16 hi/lo on both sides, 3 MFMA products */
 
 /* ---- lot shapes (discus/prog/diffuse_mod.f90:10-12) --------------------- */
 #define DC_LOT_OFF 1
@@ -78,6 +78,17 @@ int dc_get_precision(const dc_ctx *ctx);
 int dc_set_gemm_tile(dc_ctx *ctx, int tile);
 int dc_gemm_num_tiles(void);
 int dc_gemm_tile_shape(int tile, int *rows, int *cols, int *threads);
+/* Atoms per GEMM slice (one FP32 output plane each, summed in FP64): 0 =
+ * planned per problem and precision (default), n > 0 forces n.  No
+ * counterpart in DISCUS (whose sum runs atom by atom in FP64); a tuning and
+ * test knob, like dc_set_gemm_tile. */
+int dc_set_slice_atoms(dc_ctx *ctx, int n);
+/* Per-atom phase rotation of the phase GEMM (on by default): row operands
+ * carry +psi_n and column operands -psi_n, psi_n a fixed pseudo-random turn
+ * per atom, so the products are unchanged and each atom's operand rounding
+ * differs.  Off, atoms with equal phases (Bragg peaks of a crystal) add the
+ * same rounding error once per atom.  No counterpart in DISCUS; a test knob. */
+int dc_set_phase_dither(dc_ctx *ctx, int on);
 /* Polled between kernel launches of long runs; a non-zero value aborts the
  * call with DC_ERR_CANCEL, leaving outputs untouched (DISCUS's ier_ctrlc). */
 int dc_set_cancel_flag(dc_ctx *ctx, const volatile int *flag);
@@ -130,6 +141,19 @@ int dc_strucf(dc_ctx *ctx, const double *xat, int64_t nxat, int64_t ld_xat,
               const double *cfact_col, int64_t ncfact,
               const int *istl, const double xm[3], const double rten[9],
               int lform, double *tcsf);
+
+/*
+ * dc_four_stltab -- drop-in for four_stltab (discus/prog/fourier_sup.f90:
+ * 2637-2687): istl(i,j,l) = NINT(sqrt(skalpro(h,h,cr_rten)/4) / CFINC) over
+ * the num grid (abscissa fastest), h(k) = REAL(xm(k) + uin(k)*(i-1) +
+ * vin(k)*(j-1) + win(k)*(l-1)) summed in FP64 and rounded to FP32 as
+ * DISCUS's default-kind REAL() does.  Computed on the device, bit-identical
+ * to the bins every run of this library uses.  Returns DC_ERR_ISTL (-3,
+ * ier_typ ER_FOUR) when a bin exceeds CFPKT, as four_stltab does; istl then
+ * holds the unclamped bins.
+ */
+int dc_four_stltab(dc_ctx *ctx, const double xm[3], const double uin[3], const double vin[3],
+                   const double win[3], const int num[3], const double rten[9], int *istl);
 
 /* Crystal resident in HBM (crystal_mod.f90:25-138).  cr_iscat is 1-based,
  * 0 = vacancy.  dim0 = cr_dim0(:,1), dim_lo = cr_dim(:,1). */

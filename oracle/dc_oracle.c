@@ -248,9 +248,13 @@ int orc_fourier_lmn(const double *eck_in, const double *vi_in, const int *inc,
 
 /* ------------------------------------------------------------------------ */
 /* sin(theta)/lambda bin of one reciprocal point, the expression of
- * four_stltab (fourier_sup.f90:2663-2674) with skalpro = dot(h, rten*h)
- * (discus/prog/metric.f90:90-105).  Evaluated left to right without
- * contraction so that the GPU epilogue can reproduce it bit for bit. */
+ * four_stltab (fourier_sup.f90:2663-2674): h(k) = REAL(xm(k) + uin(k)*(i-1)
+ * + vin(k)*(j-1) + win(k)*(l-1)) -- the sum in FP64, then REAL() of default
+ * kind, i.e. rounded to FP32 (the reference builds without
+ * -fdefault-real-8, CMakeLists.txt:129), stored back into the FP64 h --
+ * and skalpro = dot(h, rten*h) (discus/prog/metric.f90:90-105).
+ * Evaluated left to right without contraction so that the GPU reproduces
+ * it bit for bit. */
 #pragma GCC push_options
 #pragma GCC optimize("fp-contract=off")
 int orc_istl_point(const double xm[3], const double uin[3], const double vin[3],
@@ -258,7 +262,7 @@ int orc_istl_point(const double xm[3], const double uin[3], const double vin[3],
 {
     double h[3], t[3];
     for (int k = 0; k < 3; ++k)
-        h[k] = xm[k] + uin[k] * (double)i + vin[k] * (double)j + win[k] * (double)l;
+        h[k] = (double)(float)(xm[k] + uin[k] * (double)i + vin[k] * (double)j + win[k] * (double)l);
     for (int r = 0; r < 3; ++r)
         t[r] = rten[r + 0] * h[0] + rten[r + 3] * h[1] + rten[r + 6] * h[2];
     double q2 = (h[0] * t[0] + h[1] * t[1] + h[2] * t[2]) / 4.0;

@@ -46,6 +46,8 @@ def test_null_context_arguments():
     L = _lib.lib()
     assert L.dc_set_precision(None, 0) == _lib.DC_ERR_ARG
     assert L.dc_set_gemm_tile(None, 0) == _lib.DC_ERR_ARG
+    assert L.dc_set_slice_atoms(None, 0) == _lib.DC_ERR_ARG
+    assert L.dc_set_phase_dither(None, 1) == _lib.DC_ERR_ARG
     assert L.dc_set_adjoint_walk(None, 0) == _lib.DC_ERR_ARG
     assert L.dc_crystal_set(None, None) == _lib.DC_ERR_ARG
     assert L.dc_four_run(None, None, None, None, None, None, None) == _lib.DC_ERR_ARG

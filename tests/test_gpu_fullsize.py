@@ -21,12 +21,12 @@ time):
   - the split BF16x3 adjoint (column walk over all 17 column tiles) against
     the FP32 adjoint.
 
-Bars: the forward's max|dI|/max(I) bars, 1e-6 (FP32) and 1e-5 (FP16x3), for
-each run against the oracle, and twice that between two runs.  BF16x3 is held
-to its per-product bound on coherent sums, 4e-5 (the rounding of the lo
-halves and the dropped lo*lo term leave ~2^-17 per real product, which a
-coherent sum does not average out): it meets 1e-5 only where the sum is not
-coherent.  Duality to 1e-5 of |Jv| |u|; the split adjoint to 1e-4 of max|g|
+Bars: the forward's max|dI|/max(I) bars, 1e-6 (FP32) and 1e-5 (the split
+schemes), for each run against the oracle, and twice that between two runs.
+The coherent sums hold them because the GEMM rotates every atom's operands
+by a pseudo-random phase (+psi on the row side, -psi on the column side), so
+the rounding of equal phases is no longer the same error added 256,000
+times (BF16x3 measured 2.3e-5 without the rotation, 5e-6 with it).  Duality to 1e-5 of |Jv| |u|; the split adjoint to 1e-4 of max|g|
 per parameter kind."""
 import dataclasses
 
@@ -40,7 +40,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-BAR = {"fp32": 1e-6, "fp16x3": 1e-5, "bf16x3": 4e-5}
+BAR = {"fp32": 1e-6, "fp16x3": 1e-5, "bf16x3": 1e-5}
 SHIFT = np.array([0.37, -0.21, 0.113])
 _CASE = {}
 

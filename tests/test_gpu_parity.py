@@ -147,6 +147,8 @@ def test_strucf_layer_form_factor(ctx, prec):
     e1, e2 = rel_intensity_err(F1, Fref), rel_intensity_err(F2, Fref)
     print(f"  host istl {e1:.3e} device istl {e2:.3e}", flush=True)
     assert e1 <= BAR[prec] and e2 <= BAR[prec]
+    # the device evaluates four_stltab's bins bit for bit: same GEMM, same bins
+    assert np.array_equal(F1, F2)
 
 
 def run_pair(ctx, c, prec, eck, vi, inc, **kw):
@@ -177,9 +179,22 @@ def test_four_run_rocksalt_hk0(ctx, prec):
     np.testing.assert_allclose(gpu["acsf"], 0.0)
 
 
+# max|dI|/max(I) bars of the Bragg-subtracted lot runs, where the diffuse
+# intensity |F_l - <F>|^2 is small beside |F_l|^2 at the Bragg pixels, so the
+# error of F_l (relative to |F_l|) is amplified by |F_l| / |F_l - <F>| (up to
+# ~30 in this case).  FP32 then keeps its partial sums in FP64 (the library
+# does this by itself for such runs) and FP16X3 (one rounding per 16 atoms)
+# hold the bars of the direct sums.  BF16X3 carries ~2^-17 per product (its
+# lo halves keep 8 bits and lo*lo is dropped): 2.3e-5 here, held to 4e-5; it
+# is the faster, lower-precision split scheme for such runs.
+LOTS_BAR = {"fp32": 1e-6, "fp16x3": 1e-5, "bf16x3": 4e-5}
+
+
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3", "fp16x3"])
 def test_four_run_lots_aver(ctx, prec):
-    """Box lots with periodic wrap, 'set aver' <F> subtraction, cross-lot variance."""
+    """Box lots with periodic wrap, 'set aver' <F> subtraction, cross-lot
+    variance: dsi, dvar and acsf against the exact oracle, each normalised by
+    its own maximum."""
     print(f"case four_run lots prec={prec}", flush=True)
     c = X.fcc_correlated(n=8, seed=7)
     eck, vi, inc = FR.grid_block((-2, -2, 0.5), (2, 2, 0.5), (41, 41, 1))
@@ -187,13 +202,10 @@ def test_four_run_lots_aver(ctx, prec):
     lbeg = np.floor(rng.random((6, 3)) * 8).astype(np.int32) + 1
     sel = (rng.random((8, 8, 8)) <= 0.5).astype(np.uint8)
     gpu, ref = run_pair(ctx, c, prec, eck, vi, inc, ilots=2, ls=(3, 3, 3), lbeg=lbeg, fave=0.5, sel_cell=sel)
-    for k in ("dsi", "dvar"):
+    for k in ("dsi", "dvar", "acsf"):
         err = float(np.abs(gpu[k] - ref[k]).max() / np.abs(ref[k]).max())
         print(f"  {k} max rel {err:.3e}", flush=True)
-        assert err <= 10 * BAR[prec]
-    err = float(np.abs(gpu["acsf"] - ref["acsf"]).max() / np.abs(ref["acsf"]).max())
-    print(f"  acsf max rel {err:.3e}", flush=True)
-    assert err <= BAR[prec] ** 0.5
+        assert err <= LOTS_BAR[prec]
 
 
 def test_four_run_ellipsoid_3d(ctx):

@@ -194,6 +194,51 @@ def test_stltab_cubic_closed_form_and_overflow():
     assert st == -3
 
 
+def _stl_bins(xm, u, v, w, num, rten, fp32):
+    """Independent numpy statement of four_stltab's bin (fourier_sup.f90:
+    2663-2674): h summed left to right in FP64, optionally rounded to FP32
+    (REAL() of default kind), metric applied left to right, NINT half away
+    from zero."""
+    i = np.arange(num[0])[:, None, None].astype(float)
+    j = np.arange(num[1])[None, :, None].astype(float)
+    l = np.arange(num[2])[None, None, :].astype(float)
+    h = [xm[k] + u[k] * i + v[k] * j + w[k] * l for k in range(3)]
+    if fp32:
+        h = [x.astype(np.float32).astype(np.float64) for x in h]
+    R = np.asarray(rten, float).reshape(3, 3, order="F")
+    t = [R[r, 0] * h[0] + R[r, 1] * h[1] + R[r, 2] * h[2] for r in range(3)]
+    val = np.sqrt((h[0] * t[0] + h[1] * t[1] + h[2] * t[2]) / 4.0) * (1.0 / 0.001)
+    fl = np.floor(val)
+    return np.where(val - fl >= 0.5, fl + 1, fl).astype(np.int64)
+
+
+def test_stltab_rounds_h_to_fp32():
+    """four_stltab stores h(k) = REAL(...) of default kind (FP32; the reference
+    has no -fdefault-real-8).  On the 161^3 block of the fcc a = 4 A
+    workload that rounding moves 2,316 of 4,173,281 bins by one; the oracle
+    must equal the FP32 statement everywhere, which the FP64 one is not."""
+    eck, vi, inc = FR.grid_block((-4, -4, -4), (4, 4, 4), (161, 161, 161))
+    rten = X.reciprocal_metric(4.0, 4.0, 4.0)
+    st, orc = O.stltab(eck[:, 0], vi[:, 0], vi[:, 1], vi[:, 2], inc, rten)
+    assert st == 0
+    b32 = _stl_bins(eck[:, 0], vi[:, 0], vi[:, 1], vi[:, 2], inc, rten, True)
+    b64 = _stl_bins(eck[:, 0], vi[:, 0], vi[:, 1], vi[:, 2], inc, rten, False)
+    assert np.array_equal(orc, b32)
+    assert int(np.count_nonzero(b32 != b64)) == 2316
+    # a general (triclinic) metric on a 2D layer, and a line of near-ties:
+    # h = 0.001 (2i + 1) sits on the bin boundaries of the a = 1 cube
+    rt = X.reciprocal_metric(5.1, 6.3, 7.7, 84.0, 97.0, 101.0)
+    e2, v2, i2 = FR.grid_block((-3.1, -2.2, 0.7), (2.9, 3.3, 0.7), (121, 97, 1))
+    st, orc2 = O.stltab(e2[:, 0], v2[:, 0], v2[:, 1], v2[:, 2], i2, rt)
+    assert st == 0 and np.array_equal(orc2, _stl_bins(e2[:, 0], v2[:, 0], v2[:, 1], v2[:, 2], i2, rt, True))
+    xm, u = np.array([0.001, 0.0, 0.0]), np.array([0.002, 0.0, 0.0])
+    z = np.zeros(3)
+    st, orc3 = O.stltab(xm, u, z, z, (4000, 1, 1), np.eye(3))
+    b3 = _stl_bins(xm, u, z, z, (4000, 1, 1), np.eye(3), True)
+    assert st == 0 and np.array_equal(orc3, b3)
+    assert np.count_nonzero(b3 != _stl_bins(xm, u, z, z, (4000, 1, 1), np.eye(3), False)) > 0
+
+
 def test_formtab_closed_form():
     c = X.orthorhombic(ncell=(2, 2, 2))
     cf = O.formtab(c.scat, c.occ, c.dw, ldbw=True)
