@@ -9,8 +9,10 @@ reduction with form factors, Friedel mirror) with the crystal resident in
 HBM and outputs left in HBM.
 
 Metric: atom.hkl pairs/s = sum_types N_type x 161^3 / wall time of a step,
-for the split-BF16x3 MFMA path; the plain FP32 MFMA path runs the same
-workload for the ratio.  The CPU leg (the only part that runs the oracle)
+for the split-precision MFMA path that holds max|dI|/max(I) <= 1e-5 on every
+input, FP16x3 (FP16 hi/lo halves on both sides, three products per pair on
+v_mfma_f32_16x16x32_f16, the same rate as the BF16 instruction); BF16x3,
+FP16x2 and the plain FP32 MFMA path run the same workload beside it.  The CPU leg (the only part that runs the oracle)
 times DISCUS's turbo algorithm on the host cores and checks a bounded sample
 of the GPU outputs against the exact FP64 sum (max|dI|/max(I)).
 
@@ -27,7 +29,7 @@ library's HIP events on its stream):
   post     the post-processing tail of four_run on a 161^3 grid
   nufft    the NUFFT technique on the headline workload
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--prec bf16x3]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--prec fp16x3]
   python bench.py --calibrate-copy 1073741824   (plain copy kernel of known
            size, for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)
 """
@@ -266,11 +268,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--prec", default="bf16x3")
+    ap.add_argument("--prec", default="fp16x3")
     ap.add_argument("--fp32-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--others", default="fp32,fp16x2,bf16x3,fp16x3",
+                    help="schemes timed beside the headline one (fp32 is always timed: it is the ratio's base)")
+    ap.add_argument("--no-coherent", action="store_true", help="skip the coherent-sum check runs")
     ap.add_argument("--extras", default="lots,adjoint,batch,post,nufft,peaks",
                     help="comma list of the extra measurements to run")
     ap.add_argument("--cpu-layers", type=int, default=40)
@@ -329,17 +334,15 @@ def main():
     t32, g32, _ = run_steps("fp32", fp32_steps, 1)
     v32 = pairs / t32
     log(f"fp32: {t32 * 1e3:.2f} ms/step, {v32 / 1e12:.2f} Tpairs/s, gemm {g32:.2f} ms/launch")
-    # the third scheme, FP16x2 (A split, B single FP16), on the same workload
-    t16, g16, _ = run_steps("fp16x2", fp32_steps, 1)
-    v16 = pairs / t16
-    dsi_16 = d_dsi.view(inc[2], inc[1], inc[0]).cpu().numpy()
-    log(f"fp16x2: {t16 * 1e3:.2f} ms/step, {v16 / 1e12:.2f} Tpairs/s, gemm {g16:.2f} ms/launch")
-    # FP16x3 (both sides split, 512-atom slices): the split scheme that holds
-    # 1e-5 on coherent sums as well
-    t163, g163, l163 = run_steps("fp16x3", fp32_steps, 1)
-    v163 = pairs / t163
-    dsi_163 = d_dsi.view(inc[2], inc[1], inc[0]).cpu().numpy()
-    log(f"fp16x3: {t163 * 1e3:.2f} ms/step, {v163 / 1e12:.2f} Tpairs/s, gemm {g163:.2f} ms/launch")
+    # the other split schemes on the same workload: FP16x2 (A split, B single
+    # FP16) and BF16x3 / FP16x3, whichever is not the headline
+    others = {}
+    for prec in ("fp16x2", "bf16x3", "fp16x3"):
+        if prec == args.prec or prec not in args.others.split(","):
+            continue
+        tt, gg_ms, _ = run_steps(prec, fp32_steps, 1)
+        others[prec] = {"t": tt, "gemm_ms": gg_ms, "dsi": d_dsi.view(inc[2], inc[1], inc[0]).cpu().numpy()}
+        log(f"{prec}: {tt * 1e3:.2f} ms/step, {pairs / tt / 1e12:.2f} Tpairs/s, gemm {gg_ms:.2f} ms/launch")
 
     # coherent sums: the same crystal shifted by a general vector.  Its sites
     # are no longer at 0 and 1/2, so at the Bragg peaks every cell adds the
@@ -348,7 +351,7 @@ def main():
     moved = dataclasses.replace(c, pos=c.pos + SHIFT)
     ctx.crystal_set(moved)
     coherent_samples = {}
-    for prec in ("bf16x3", "fp16x3", "fp16x2", "fp32"):
+    for prec in (() if args.no_coherent else ("fp16x3", "bf16x3", "fp16x2", "fp32")):
         ctx.set_precision(prec)
         ctx.four_run_dev(desc, csf=d_csf, dsi=d_dsi)
         torch.cuda.synchronize()
@@ -366,23 +369,20 @@ def main():
                 "flops_per_launch": 8.0 * KSPLIT[args.prec] * pairs_computed_per_launch,
                 "fp32_kernel": {"avg_launch_ms": round(g32, 3),
                                 "achieved": round(8.0 * pairs_computed_per_launch / (g32 * 1e-3) / 1e12, 2),
-                                "peak": PEAK["fp32"]},
-                "fp16x2_kernel": {"avg_launch_ms": round(g16, 3),
-                                  "achieved": round(16.0 * pairs_computed_per_launch / (g16 * 1e-3) / 1e12, 2),
-                                  "peak": PEAK["fp16x2"]},
-                "fp16x3_kernel": {"avg_launch_ms": round(g163, 3), "launches_per_step": l163,
-                                  "achieved": round(24.0 * natoms * fnum0 * inc[1] * inc[2] / l163 / (g163 * 1e-3)
-                                                    / 1e12, 2),
-                                  "peak": PEAK["fp16x3"]}}
+                                "peak": PEAK["fp32"]}}
+    for prec, o in others.items():
+        roofline[f"{prec}_kernel"] = {"avg_launch_ms": round(o["gemm_ms"], 3),
+                                      "achieved": round(8.0 * KSPLIT[prec] * pairs_computed_per_launch
+                                                        / (o["gemm_ms"] * 1e-3) / 1e12, 2),
+                                      "peak": PEAK[prec]}
 
     # bounded sample of the GPU output for the check in the CPU leg: two
     # l-layers (l = 0 and l = 1) subsampled every 4th point (h, k step 0.2)
     head_samples = {0.0: dsi_split[80, ::4, ::4].T.copy(), 1.0: dsi_split[100, ::4, ::4].T.copy()}
-    fp16_samples = {0.0: dsi_16[80, ::4, ::4].T.copy(), 1.0: dsi_16[100, ::4, ::4].T.copy()}
-    f163_samples = {0.0: dsi_163[80, ::4, ::4].T.copy(), 1.0: dsi_163[100, ::4, ::4].T.copy()}
+    other_samples = {p: {0.0: o["dsi"][80, ::4, ::4].T.copy(), 1.0: o["dsi"][100, ::4, ::4].T.copy()}
+                     for p, o in others.items()}
     max_err = None
-    err16 = None
-    err163 = None
+    other_err = {p: None for p in others}
     coherent = None
     ctx.close()
 
@@ -436,11 +436,10 @@ def main():
                 errs = [(np.abs(samples[l] - refs[l]).max(), refs[l].max()) for l in refs]
                 return float(max(e for e, _ in errs) / max(m for _, m in errs))
             max_err = sampled_err(head_samples)
-            log(f"sampled max|dI|/max(I) vs FP64 oracle: {max_err:.3e}")
-            err16 = sampled_err(fp16_samples)
-            log(f"fp16x2 sampled max|dI|/max(I): {err16:.3e}")
-            err163 = sampled_err(f163_samples)
-            log(f"fp16x3 sampled max|dI|/max(I): {err163:.3e}")
+            log(f"{args.prec} sampled max|dI|/max(I) vs FP64 oracle: {max_err:.3e}")
+            for p, smp in other_samples.items():
+                other_err[p] = sampled_err(smp)
+                log(f"{p} sampled max|dI|/max(I): {other_err[p]:.3e}")
             moved = dataclasses.replace(c, pos=c.pos + SHIFT)
             mrefs = {}
             for lval in (0.0, 1.0):
@@ -448,7 +447,7 @@ def main():
                 mrefs[lval] = O.four_run(moved, e2, v2, i2, cf)["dsi"][:, :, 0]
             coherent = {"workload": "the headline crystal shifted by t = (0.37, -0.21, 0.113) (all 256k atoms in "
                                     "phase at the Bragg peaks), layers l = 0 and 1 every 4th point vs FP64"}
-            for prec, smp in coherent_samples.items():
+            for prec, smp in coherent_samples.items():  # (empty with --no-coherent)
                 errs = [(np.abs(smp[l] - mrefs[l]).max(), mrefs[l].max()) for l in mrefs]
                 coherent[prec] = float(max(e for e, _ in errs) / max(m for _, m in errs))
             log(f"coherent sums, max|dI|/max(I): {json.dumps(coherent)}")
@@ -489,11 +488,14 @@ def main():
                                "3D block h,k,l in [-4,4] at 161^3 points, lots off, Friedel on",
                    "atoms": int(c.natoms), "grid": inc, "pairs_per_step": pairs, "precision": args.prec},
         "fp32_value": v32, "fp32_ms_per_step": t32 * 1e3, "speedup_vs_fp32": value / v32,
-        "fp16x2_value": v16, "fp16x2_ms_per_step": t16 * 1e3, "fp16x2_max_rel_err_sampled": err16,
-        "fp16x3_value": v163, "fp16x3_ms_per_step": t163 * 1e3, "fp16x3_max_rel_err_sampled": err163,
         "max_rel_err_sampled": max_err, "coherent_sum_max_rel_err": coherent,
         "roofline": roofline, "cpu_baseline": cpu,
     }
+    for p, o in others.items():
+        line[f"{p}_value"] = pairs / o["t"]
+        line[f"{p}_ms_per_step"] = o["t"] * 1e3
+        line[f"{p}_speedup_vs_fp32"] = (pairs / o["t"]) / v32
+        line[f"{p}_max_rel_err_sampled"] = other_err[p]
     line.update(extras)
     print(json.dumps(line), flush=True)
 

@@ -150,15 +150,19 @@ Plan plan_gemm(Grid &g, int forced, int only_mapping, int prec, long long natoms
             gg.R0 = g.f[0];
             gg.C0 = g.f[2] > 1 ? std::max(16, g.f[1]) : g.f[1];
             gg.C = gg.C0 * g.f[2];
-            ps[0] = 1; ps[1] = gg.R; ps[2] = (long long)gg.R * gg.C0;
         } else {
             gg.R0 = g.f[1] > 1 ? std::max(16, g.f[0]) : g.f[0];
             gg.R = gg.R0 * g.f[1];
             gg.C = g.f[2];
             gg.C0 = g.f[2];
-            ps[0] = 1; ps[1] = gg.R0; ps[2] = gg.R;
         }
-        gg.P = (long long)gg.R * gg.C;
+        gg.Rs = (int)padded(gg.R, 16);
+        if (mapping == 0) {
+            ps[0] = 1; ps[1] = gg.Rs; ps[2] = (long long)gg.Rs * gg.C0;
+        } else {
+            ps[0] = 1; ps[1] = gg.R0; ps[2] = gg.Rs;
+        }
+        gg.P = (long long)gg.Rs * gg.C;
         const bool acc64 = prec == SCHEME_FP32_ACC64;
         // FP32 with FP64 sums is built for a few tiles only; a forced tile
         // outside them is ignored for it

@@ -29,9 +29,12 @@ struct alignas(16) AtomRec {
 static_assert(sizeof(AtomRec) == 48, "AtomRec layout");
 
 // GEMM problem geometry.  R rows x C cols per slice, output element
-// (r, c) stored at partial[slice * P + r + R * c].
+// (r, c) stored at partial[slice * P + r + Rs * c]; the column stride Rs is R
+// rounded up to 16 rows, so that the 16-row blocks the epilogue writes are
+// whole 128-byte lines.  P = Rs * C.
 struct GemmGeom {
     int R, R0, C, C0;
+    int Rs;
     long long P;
     uint32_t dither;   // per-atom phase rotation salt (0: none), see dither_turn
 };

@@ -102,6 +102,20 @@ __device__ __forceinline__ f32x2 unpack2(uint32_t w) {
         return (f32x2){__builtin_bit_cast(float, w << 16), __builtin_bit_cast(float, w & 0xFFFF0000u)};
 }
 
+// Low FP16 halves of a split: given h = (f16(x), f16(y)) packed, returns
+// (f16(x - h.x), f16(y - h.y)) packed.  x - h.x is exact in FP32, so each
+// half is one rounding, as with a conversion of the FP32 difference; the
+// mixed-precision FMAs take 2 instructions where unpacking h, subtracting
+// and packing again takes 5.
+__device__ __forceinline__ uint32_t lo_halves_f16(uint32_t h, float x, float y) {
+    uint32_t r;
+    asm("v_fma_mixlo_f16 %0, -1.0, %1, %2 op_sel_hi:[0,1,0]\n\t"
+        "v_fma_mixhi_f16 %0, -1.0, %1, %3 op_sel:[0,1,0] op_sel_hi:[0,1,0]"
+        : "=&v"(r)
+        : "v"(h), "v"(x), "v"(y));
+    return r;
+}
+
 // Round the 64-bit phase to the 32-bit turn fraction used by sincos_turn.
 __device__ __forceinline__ uint32_t phase_top(uint64_t p) { return (uint32_t)((p + 0x80000000ull) >> 32); }
 
@@ -177,7 +191,9 @@ __device__ __forceinline__ float2 cis(uint64_t ph) {
 // ---------------------------------------------------------------------------
 // Tables of one stage, built by all threads (pt = thread index).  Each atom
 // a = pt % 16 (index katom of the record array) has NPT/16 helper threads
-// r = pt / 16:
+// r = pt / 16 (waves 0 and 1 build the low tables, 8 sincos per thread, the
+// other waves the high rows; spreading the same work evenly over all waves
+// measured 3-5% slower, 11.2 vs 10.7 ms on the headline):
 //   r < 8      low tables of side r/4, entries l = 4m .. 4m+3 (m = r % 4):
 //              TL0[l] = cis(l s + psi2), TL1[l] = cis(l s + w + psi2), each
 //              evaluated from its exact phase
@@ -269,8 +285,12 @@ __device__ __forceinline__ void elems_group(Elems<SCHEME, T> &el, const float2 *
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             hi[q] = pack2<SCHEME>(v[q].x, v[q].y);
-            const f32x2 hvv = unpack2<SCHEME>(hi[q]);
-            lo[q] = pack2<SCHEME>(v[q].x - hvv.x, v[q].y - hvv.y);
+            if constexpr (f16ops(SCHEME)) {
+                lo[q] = lo_halves_f16(hi[q], v[q].x, v[q].y);
+            } else {
+                const f32x2 hvv = unpack2<SCHEME>(hi[q]);
+                lo[q] = pack2<SCHEME>(v[q].x - hvv.x, v[q].y - hvv.y);
+            }
         }
         const int o = pk_off(row, g);
         if constexpr (isA) {
@@ -672,7 +692,7 @@ phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice
                 for (int v = 0; v < 4; ++v) {
                     const int r = row0 + wr0 + 16 * i + 4 * lg + v;
                     if (r < g.R) {
-                        const long long o = r + (long long)g.R * c;
+                        const long long o = r + (long long)g.Rs * c;
                         if constexpr (D64) {
                             const double xr = dR[i][j][v], xi = dI[i][j][v];
                             const float hr = (float)xr, hi = (float)xi;
