@@ -156,7 +156,9 @@ Plan plan_gemm(Grid &g, int forced, int only_mapping, int prec, long long natoms
             gg.C = g.f[2];
             gg.C0 = g.f[2];
         }
-        gg.Rs = (int)padded(gg.R, 16);
+        // plane column stride: whole 16-row blocks (128-byte lines) unless
+        // that grows the plane by more than 1/16 (short columns)
+        gg.Rs = (padded(gg.R, 16) - gg.R) * 16 <= gg.R ? (int)padded(gg.R, 16) : gg.R;
         if (mapping == 0) {
             ps[0] = 1; ps[1] = gg.Rs; ps[2] = (long long)gg.Rs * gg.C0;
         } else {

@@ -30,8 +30,8 @@ static_assert(sizeof(AtomRec) == 48, "AtomRec layout");
 
 // GEMM problem geometry.  R rows x C cols per slice, output element
 // (r, c) stored at partial[slice * P + r + Rs * c]; the column stride Rs is R
-// rounded up to 16 rows, so that the 16-row blocks the epilogue writes are
-// whole 128-byte lines.  P = Rs * C.
+// rounded up to 16 rows where that adds at most 1/16 to the plane, so that
+// the 16-row blocks the epilogue writes are whole 128-byte lines.  P = Rs * C.
 struct GemmGeom {
     int R, R0, C, C0;
     int Rs;
