@@ -349,6 +349,7 @@ namespace {
 struct GatherList {
     std::vector<int> idx;
     std::vector<int> oid;
+    std::vector<int4> runs;                  // lot gather: (first, src in type_idx, oid, -)
     std::vector<double> offs;
     std::vector<std::pair<int, int>> segs;   // (type 0-based, count)
 };
@@ -640,7 +641,10 @@ int dc_crystal_set(dc_ctx *c, const dc_crystal_desc *d) {
     c->h_iscat.assign(d->cr_iscat, d->cr_iscat + d->natoms);
     int st = 0;
     if ((st = upload(c, c->pos, c->h_pos.data(), c->h_pos.size()))) return st;
-    // per-type atom lists in crystal order (LOT_OFF gather, four_getatm :1839-1848)
+    // per-type atom lists in crystal order (LOT_OFF gather, four_getatm :1839-1848);
+    // for a DISCUS-ordered crystal (atom = cell * ncatoms + site) crystal order
+    // is (cell, site) order, so the same lists serve the lot gather, with the
+    // first position of every cell per type in tc_start
     std::vector<std::vector<int>> lists(c->nscat);
     for (int64_t i = 0; i < d->natoms; ++i) {
         int t = d->cr_iscat[i];
@@ -654,6 +658,20 @@ int dc_crystal_set(dc_ctx *c, const dc_crystal_desc *d) {
     }
     c->type_beg[c->nscat] = (int)flat.size();
     if ((st = upload(c, c->type_idx, flat.data(), flat.size()))) return st;
+    const int64_t ncell = (int64_t)c->icc[0] * c->icc[1] * c->icc[2];
+    c->tc_start.clear();
+    if (d->natoms == ncell * c->ncatoms && ncell * c->nscat < (int64_t)1 << 28) {
+        c->tc_start.assign((size_t)c->nscat * (ncell + 1), 0);
+        for (int t = 0; t < c->nscat; ++t) {
+            int *cs = c->tc_start.data() + (size_t)t * (ncell + 1);
+            int q = 0;
+            const std::vector<int> &L = lists[t];
+            for (int64_t cell = 0; cell <= ncell; ++cell) {
+                while (q < (int)L.size() && L[q] < cell * c->ncatoms) ++q;
+                cs[cell] = q;
+            }
+        }
+    }
     HIPCHK(hipStreamSynchronize(c->st));
     c->have_crystal = true;
     return DC_OK;
@@ -661,20 +679,32 @@ int dc_crystal_set(dc_ctx *c, const dc_crystal_desc *d) {
 
 namespace {
 
-// four_getatm for box/ellipsoid lots (fourier_sup.f90:1852-1968): per type,
-// the atoms of the lot in DISCUS order with the lot's periodic offsets.
-// Offsets are per (lot, wrap pattern): oid = lot*8 + wrapbits.
-long long gather_lot(const dc_ctx *c, const dc_run_desc *r, int lot, int t1, std::vector<int> &idx,
-                     std::vector<int> &oid, std::vector<double> &offs) {
+// four_getatm for box/ellipsoid lots (fourier_sup.f90:1852-1968): the cells
+// of lot `lot` in DISCUS's order (kk outer, ii inner) with periodic wrap, as
+// runs of consecutive cells (ii direction) that share a wrap pattern; appends
+// the lot's 8 offsets (one per wrap pattern, cr_dim0 + lbeg - 1 - icc where
+// wrapped) to offs and returns the number of cells.  The atoms of type t in
+// a run of cells are one contiguous range of the type's (cell, site)-ordered
+// list (tc_start).  four_getatm's scan of the atoms beyond cr_end
+// (:1886-1899) is absent: DISCUS's 'run' refuses lots unless cr_natoms equals
+// cells x sites (fourier.f90:644-651, DC_ERR_LOTS here), so that list is
+// always empty on this path.
+struct CellRun {
+    long long cell;   // first cell (0-based crystal cell index)
+    int n;            // consecutive cells along ii
+    int wrap;         // wrap bits of the run (1: a, 2: b, 4: c)
+};
+
+long long lot_cells(const dc_ctx *c, const dc_run_desc *r, int lot, std::vector<CellRun> &cruns,
+                    std::vector<double> &offs) {
     const int *lbeg = r->lbeg + 3 * lot;
     double x0[3];
     for (int k = 0; k < 3; ++k) x0[k] = (double)(float)r->ls[k] / 2.0;
-    const size_t off_base = offs.size() / 3;
     for (int wb = 0; wb < 8; ++wb)
         for (int k = 0; k < 3; ++k)
             offs.push_back(c->dim0[k] + lbeg[k] - 1 - (((wb >> k) & 1) ? (double)c->icc[k] : 0.0));
-    const long long cr_end = (long long)c->ncatoms * c->icc[0] * c->icc[1] * c->icc[2];
     long long ncell = 0;
+    cruns.clear();
     for (int kk = 0; kk < r->ls[2]; ++kk) {
         int ic2 = kk + lbeg[2], w2 = 0;
         double xt2 = ((double)kk - x0[2] + 0.5) * ((double)kk - x0[2] + 0.5) / (x0[2] * x0[2]);
@@ -683,38 +713,47 @@ long long gather_lot(const dc_ctx *c, const dc_run_desc *r, int lot, int t1, std
             int ic1 = jj + lbeg[1], w1 = 0;
             double xt1 = ((double)jj - x0[1] + 0.5) * ((double)jj - x0[1] + 0.5) / (x0[1] * x0[1]);
             if (ic1 > c->icc[1]) { ic1 -= c->icc[1]; w1 = 2; }
+            const long long row = (long long)(ic2 - 1) * c->icc[0] * c->icc[1] + (long long)(ic1 - 1) * c->icc[0];
+            bool open = false;
             for (int ii = 0; ii < r->ls[0]; ++ii) {
                 int ic0 = ii + lbeg[0], w0 = 0;
                 double xt0 = ((double)ii - x0[0] + 0.5) * ((double)ii - x0[0] + 0.5) / (x0[0] * x0[0]);
                 if (ic0 > c->icc[0]) { ic0 -= c->icc[0]; w0 = 1; }
-                if (r->ilots == DC_LOT_ELI && !(xt0 + xt1 + xt2 <= 1.0)) continue;
-                ++ncell;
-                const int o = (int)off_base + (w0 | w1 | w2);
-                const long long cell = ((long long)(ic2 - 1) * c->icc[0] * c->icc[1] + (long long)(ic1 - 1) * c->icc[0] + (ic0 - 1));
-                for (int is = 0; is < c->ncatoms; ++is) {
-                    long long ia = cell * c->ncatoms + is;
-                    if (c->h_iscat[ia] == t1) {
-                        idx.push_back((int)ia);
-                        oid.push_back(o);
-                    }
+                if (r->ilots == DC_LOT_ELI && !(xt0 + xt1 + xt2 <= 1.0)) {
+                    open = false;
+                    continue;
                 }
-                for (long long ir = cr_end; ir < c->natoms; ++ir) {
-                    if (c->h_iscat[ir] != t1) continue;
-                    const int ic[3] = {ic0, ic1, ic2};
-                    bool ok = true;
-                    for (int k = 0; k < 3 && ok; ++k) {
-                        int jcell = ic[k] + (int)std::llround(c->dim_lo[k]) - 1;
-                        ok = (int)c->h_pos[3 * ir + k] == jcell;
-                    }
-                    if (ok) {
-                        idx.push_back((int)ir);
-                        oid.push_back(o);
-                    }
+                ++ncell;
+                const long long cell = row + (ic0 - 1);
+                const int wrap = w0 | w1 | w2;
+                if (open && cruns.back().wrap == wrap && cruns.back().cell + cruns.back().n == cell) {
+                    cruns.back().n++;
+                } else {
+                    cruns.push_back({cell, 1, wrap});
+                    open = true;
                 }
             }
         }
     }
     return ncell;
+}
+
+// The runs of gathered atoms of type t1 (1-based) over the cell runs of a
+// lot; dst is the position of the lot's next atom in the launch's list.
+// Returns the number of atoms.
+long long lot_type_runs(const dc_ctx *c, const std::vector<CellRun> &cruns, int t1, int obase, long long dst,
+                        std::vector<int4> &runs) {
+    const long long ncell = (long long)c->icc[0] * c->icc[1] * c->icc[2];
+    const int *cs = c->tc_start.data() + (size_t)(t1 - 1) * (ncell + 1);
+    long long n = 0;
+    for (const CellRun &cr : cruns) {
+        const int b = cs[cr.cell], e = cs[cr.cell + cr.n];
+        if (e > b) {
+            runs.push_back(make_int4((int)(dst + n), c->type_beg[t1 - 1] + b, obase + cr.wrap, 0));
+            n += e - b;
+        }
+    }
+    return n;
 }
 
 // Run the phase GEMM over gathered segments and leave the slice planes in
@@ -737,6 +776,13 @@ int gemm_segments(dc_ctx *c, const Grid &g, const Plan &pl, GatherList &gl,
         pa.src = ptr<double>(c->pos);
         if (use_type_lists) {
             pa.idx = ptr<int>(c->type_idx);
+        } else if (!gl.runs.empty()) {
+            if ((st = upload(c, c->runs, gl.runs.data(), gl.runs.size()))) return st;
+            if ((st = upload(c, c->offs, gl.offs.data(), gl.offs.size()))) return st;
+            pa.idx = ptr<int>(c->type_idx);
+            pa.runs = ptr<int4>(c->runs);
+            pa.nruns = (int)gl.runs.size();
+            pa.offs = ptr<double>(c->offs);
         } else {
             if ((st = upload(c, c->idx, gl.idx.data(), gl.idx.size()))) return st;
             pa.idx = ptr<int>(c->idx);
@@ -790,6 +836,7 @@ int dc_four_run_dev(dc_ctx *c, const dc_run_desc *r, double *d_csf, double *d_ac
     if (lots) {
         if (c->natoms != (int64_t)c->icc[0] * c->icc[1] * c->icc[2] * c->ncatoms)
             return set_err(c, DC_ERR_LOTS, DC_ER_APPL, "Lots are on but the number of atoms differs from the crystal size");
+        if (c->tc_start.empty()) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "dc_four_run: crystal too large for lots");
         if (r->nlots < 1 || !r->lbeg) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "dc_four_run: lots need lbeg");
         for (int k = 0; k < 3; ++k)
             if (r->ls[k] < 1 || r->ls[k] > c->icc[k]) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "dc_four_run: lot size outside crystal");
@@ -967,24 +1014,22 @@ int dc_four_run_dev(dc_ctx *c, const dc_run_desc *r, double *d_csf, double *d_ac
         std::vector<int> lot_first_seg;
         int l1 = lot;
         long long n_batch = 0;
+        std::vector<CellRun> cruns;
         while (l1 < nlots) {
             std::vector<std::pair<int, int>> segs_one;
-            std::vector<int> idx_one, oid_one;
+            std::vector<int4> runs_one;
             std::vector<double> offs8;
             long long nc = ncell;
             if (!lots) {
                 for (int t = 0; t < c->nscat; ++t) segs_one.push_back({t, c->type_beg[t + 1] - c->type_beg[t]});
             } else {
                 const int obase = (int)(gl.offs.size() / 3);
+                nc = lot_cells(c, r, l1, cruns, offs8);
+                long long dst = n_batch;
                 for (int t = 1; t <= c->nscat; ++t) {
-                    std::vector<int> idx, oid;
-                    std::vector<double> offs;
-                    nc = gather_lot(c, r, l1, t, idx, oid, offs);   // oid in 0..7, 8 offsets
-                    if (t == 1) offs8 = offs;
-                    for (auto &o : oid) o += obase;
-                    idx_one.insert(idx_one.end(), idx.begin(), idx.end());
-                    oid_one.insert(oid_one.end(), oid.begin(), oid.end());
-                    segs_one.push_back({t - 1, (int)idx.size()});
+                    const long long n = lot_type_runs(c, cruns, t, obase, dst, runs_one);
+                    segs_one.push_back({t - 1, (int)n});
+                    dst += n;
                 }
             }
             long long n_one = 0;
@@ -994,8 +1039,7 @@ int dc_four_run_dev(dc_ctx *c, const dc_run_desc *r, double *d_csf, double *d_ac
             if (l1 > lot && est_slices * plane_bytes > PARTIAL_BUDGET) break;
             lot_first_seg.push_back((int)gl.segs.size());
             gl.offs.insert(gl.offs.end(), offs8.begin(), offs8.end());
-            gl.idx.insert(gl.idx.end(), idx_one.begin(), idx_one.end());
-            gl.oid.insert(gl.oid.end(), oid_one.begin(), oid_one.end());
+            gl.runs.insert(gl.runs.end(), runs_one.begin(), runs_one.end());
             gl.segs.insert(gl.segs.end(), segs_one.begin(), segs_one.end());
             n_batch += n_one;
             ncell = nc;

@@ -51,6 +51,13 @@ struct dc_ctx {
     double dim0[3] = {0, 0, 0}, dim_lo[3] = {0, 0, 0};
     std::vector<int> type_beg;      // per type offsets into d type list
     DevBuf type_idx;
+    // lot gather (DISCUS-ordered crystals, natoms == cells x sites): per type,
+    // the atoms in (cell, site) order on the device (tc_idx, type t from
+    // type_beg[t]) and per type the first position of every cell in that list
+    // (tc_start[t * (ncell + 1) + cell], host), so that a run of consecutive
+    // cells of a lot is one contiguous range
+    std::vector<int> tc_start;
+    DevBuf runs;
 
     std::vector<DevBuf *> buffers() {
         return {&rec, &slice_beg, &slice_type, &lot_slice, &partial, &cfact, &istl, &idx, &oid, &offs, &xat, &w,

@@ -13,6 +13,9 @@ struct PrepArgs {
     long long src_ld;
     const int *idx;         // gathered atom index (null: identity)
     const int *oid;         // offset id per gathered atom (null: none)
+    const int4 *runs;       // (first, src, oid, -) runs of gathered atoms: atom first + q takes
+                            // idx[src + q] and offset oid (null: idx/oid per atom); by first
+    int nruns;
     const double *offs;     // offset table (3 per id)
     const float *w;         // per-atom amplitude (null: 1)
     const float *bdw;       // per-atom isotropic B in AtomRec::pad (null: 0)

@@ -10,7 +10,9 @@ namespace dc {
 
 // ---------------------------------------------------------------------------
 // Record preparation.  Position of gathered atom k:
-//   x = src[3*idx[k] + c] - offs[oid[k]].c        (idx/oid/offs may be null)
+//   x = src[3*ia + c] - offs[o].c,  ia = idx[k], o = oid[k]  (idx/oid/offs may be null)
+// or, with runs (the lot gather, four_getatm), ia = idx[run.src + k - run.first] and
+// o = run.oid for the run holding k (binary search over the runs' first atoms);
 // or, with src_ld > 0, the Fortran xat(ld,3) layout x = src[k + ld*c].
 // Phases: p0 from the corner (mode CORNER) or from DISCUS's lmn/off_shift
 // decomposition (mode LMN, four_strucf_OMP_mod.f90:135-149,163-165).
@@ -18,14 +20,28 @@ __global__ void prep_records_kernel(PrepArgs a, AtomRec *__restrict__ out) {
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= a.n) return;
     double x[3];
+    long long src = k;     // caller-order index of the atom
     if (a.src_ld > 0) {
         for (int c = 0; c < 3; ++c) x[c] = a.src[k + a.src_ld * c];
     } else {
-        const long long i = a.idx ? (long long)a.idx[k] : k;
-        for (int c = 0; c < 3; ++c) x[c] = a.src[3 * i + c];
-        if (a.oid) {
-            const double *o = a.offs + 3 * (long long)a.oid[k];
-            for (int c = 0; c < 3; ++c) x[c] -= o[c];
+        long long o = -1;
+        if (a.runs) {
+            int lo = 0, hi = a.nruns - 1;   // last run with first <= k
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (a.runs[mid].x <= k) lo = mid; else hi = mid - 1;
+            }
+            const int4 r = a.runs[lo];
+            src = a.idx[r.y + (k - r.x)];
+            o = r.z;
+        } else {
+            src = a.idx ? (long long)a.idx[k] : k;
+            if (a.oid) o = a.oid[k];
+        }
+        for (int c = 0; c < 3; ++c) x[c] = a.src[3 * src + c];
+        if (o >= 0) {
+            const double *of = a.offs + 3 * o;
+            for (int c = 0; c < 3; ++c) x[c] -= of[c];
         }
     }
     uint64_t inc[3];
@@ -51,7 +67,6 @@ __global__ void prep_records_kernel(PrepArgs a, AtomRec *__restrict__ out) {
         r.ra = inc[0]; r.ra1 = inc[1];
         r.ca = inc[2]; r.ca1 = 0;
     }
-    const long long src = a.idx ? (long long)a.idx[k] : k;
     r.w = a.occ_src ? (float)a.occ_src[src] : (a.w ? a.w[k] : 1.0f);
     r.pad = a.bdw_src ? (float)a.bdw_src[src] : (a.bdw ? a.bdw[k] : 0.0f);
     out[k] = r;

@@ -1,6 +1,6 @@
-"""GPU parity at the benchmark sizes, through properties that do not depend on
-the size (the exact FP64 oracle cannot cover a whole 161^3 block in test
-time):
+"""GPU parity at the benchmark sizes, through sampled sub-grids and properties
+that do not depend on the size (the exact FP64 oracle cannot cover a whole
+161^3 block in test time):
 
 * the structure-factor run on fcc 40^3 cells (256,000 atoms, 3 types) over
   the 161^3 block h, k, l in [-4, 4] (the bench workload):
@@ -19,7 +19,15 @@ time):
   - duality of the derivative maps, <J v, u> = <v, J^T u>, between the JVP
     (forward-mode columns) and the VJP (adjoint) on the same F;
   - the split BF16x3 adjoint (column walk over all 17 column tiles) against
-    the FP32 adjoint.
+    the FP32 adjoint;
+  - chi^2 with weights on the sub-lattice of every 8th pixel (9^3), where the
+    FP64 oracle computes chi^2, I and the gradient, checked at 200 random
+    parameters;
+* the lots workload (4M atoms, 64 lots, 8 x 401^2): dsi and the cross-lot
+  variance of two sampled layers against the oracle's lots run;
+* the population (96 members, 1.78M atoms, 256^2 each): I, chi^2 and the
+  gradient of every 8th member and the largest one on the sub-lattice of
+  every 8th pixel (32^2).
 
 Bars: the forward's max|dI|/max(I) bars, 1e-6 (FP32) and 1e-5 (the split
 schemes), for each run against the oracle, and twice that between two runs.
@@ -132,3 +140,146 @@ def test_refinement_duality_and_split_adjoint(ctx):
     print(f"  duality |<Jv,u> - <v,J^T u>|/(|Jv||u|) = {dual:.2e}; split adjoint (walk {walk}) vs FP32 {serr:.2e}")
     assert dual <= 1e-5
     assert serr <= 1e-4
+
+
+# ---- the other benchmark configurations at their full sizes -----------------
+
+def _sublattice(inc, step):
+    """Mask of every step-th pixel per axis of a Fortran-ordered grid, and the
+    (eck, vi, inc) of that sub-lattice as a grid of its own."""
+    m = np.zeros(tuple(inc), bool)
+    sl = tuple(slice(None, None, step) if n > 1 else slice(None) for n in inc)
+    m[sl] = True
+    return m, sl
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16x3", "bf16x3"])
+def test_lots_fullsize(ctx, prec):
+    """The lots workload at full size: fcc 100^3 cells (4M atoms, 2 types,
+    10% substitution, correlated displacements), 64 periodic box lots of 8^3
+    cells, 8 hk-layers of 401^2; dsi and the FP64 cross-lot variance dvar of
+    layers l = 0 and l = 3.5 (every 10th h and k) against the oracle's exact
+    lots run with the same origins.  FP32 keeps its partial sums in FP64 for a
+    variance run."""
+    c = X.fcc_correlated(n=100, a=4.0, subst=0.10, sigma=0.08, seed=3)
+    eck, vi, inc = FR.grid_block((-10.0, -10.0, 0.0), (10.0, 10.0, 3.5), (401, 401, 8))
+    ls = (8, 8, 8)
+    lbeg = X.lot_origins(c.icc, 64, seed=11, lperiod=True, ls=ls)
+    cf = FR.formtab(c)
+    print(f"case lots fullsize {prec}", flush=True)
+    ctx.set_precision(prec)
+    try:
+        ctx.crystal_set(c)
+        desc, keep = ctx.run_desc(eck, vi, inc, c.rten(), cf, ilots=2, ls=ls, lbeg=lbeg, lperiod=True)
+        gpu = ctx.four_run(desc, want_var=True)
+    finally:
+        ctx.set_precision("fp32")
+    errs = {}
+    for k, lval in ((0, 0.0), (7, 3.5)):
+        e2, v2, i2 = FR.grid_block((-10.0, -10.0, lval), (10.0, 10.0, lval), (41, 41, 1))
+        ref = O.four_run(c, e2, v2, i2, cf, ilots=O.LOT_BOX, ls=ls, lbeg=lbeg, lperiod=True, want_var=True)
+        for q in ("dsi", "dvar"):
+            errs.setdefault(q, []).append((np.abs(gpu[q][::10, ::10, k] - ref[q][:, :, 0]).max(),
+                                           np.abs(ref[q]).max()))
+    bar = {"fp32": 1e-6, "fp16x3": 1e-5, "bf16x3": 4e-5}[prec]
+    for q, v in errs.items():
+        err = max(e for e, _ in v) / max(m for _, m in v)
+        print(f"  {prec} {q}: max|d|/max = {err:.2e}", flush=True)
+        assert err <= bar
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16x3"])
+def test_refinement_fullsize_gradient(ctx, prec):
+    """The refinement workload at full size: 32,768 atoms, 4 types, 65^3
+    block.  chi^2 against a perturbed copy (+1% noise) with weights 1/T on the
+    sub-lattice of every 8th pixel per axis and 0 elsewhere, so the FP64
+    oracle covers the same chi^2 on that 9^3 sub-grid: chi^2, the intensity
+    there, and 200 random parameters of the gradient wrt (x, y, z, occ, B)."""
+    from diffusecode_amd.refine import scattering_table
+    cr = X.orthorhombic()
+    occ, bdw = X.refinement_parameters(cr)
+    cf = scattering_table(cr.scat)
+    eck, vi, inc = FR.grid_block((-2.0, -2.0, -2.0), (2.0, 2.0, 2.0), (65, 65, 65))
+    mask, sl = _sublattice(inc, 8)
+    e9, v9, i9 = eck, vi * 8, [9, 9, 9]
+    p2, o2, b2 = X.perturb_parameters(cr.pos, occ, bdw)
+    print(f"case refinement fullsize {prec}", flush=True)
+    ctx.set_precision("fp32")
+    d2, k2 = ctx.model_desc(p2, cr.iscat, o2, b2, cf, eck, vi, inc, cr.rten())
+    I2, _ = ctx.model_intensity(d2)
+    rng = np.random.default_rng(21)
+    T = I2 * (1.0 + 0.01 * rng.standard_normal(I2.shape))
+    W = np.where(mask, 1.0 / np.maximum(T, 1e-3 * T.max()), 0.0)
+    ctx.set_precision(prec)
+    try:
+        d, keep = ctx.model_desc(cr.pos, cr.iscat, occ, bdw, cf, eck, vi, inc, cr.rten())
+        chi, I, g = ctx.model_chi2(d, T.ravel(order="F"), W.ravel(order="F"))
+    finally:
+        ctx.set_precision("fp32")
+    F9 = O.model_f(cr.pos, cr.iscat, occ, bdw, cf, e9, v9, i9, cr.rten())
+    I9 = np.abs(F9) ** 2
+    T9, W9 = T[sl], W[sl]
+    chi_ref = float((W9 * (I9 - T9) ** 2).sum())
+    gref = O.model_grad(cr.pos, cr.iscat, occ, bdw, cf, e9, v9, i9, cr.rten(), F9,
+                        (2.0 * W9 * (I9 - T9)).ravel(order="F"))
+    pick = rng.choice(g.size, 200, replace=False)
+    atoms, kinds = pick // 5, pick % 5
+    ierr = np.abs(I[sl] - I9).max() / I9.max()
+    cerr = abs(chi - chi_ref) / chi_ref
+    gerr = max(np.abs(g[atoms[kinds == k], k] - gref[atoms[kinds == k], k]).max() / np.abs(gref[:, k]).max()
+               for k in range(5))
+    print(f"  {prec}: I {ierr:.2e} chi2 {cerr:.2e} grad (200 parameters, of max|g| per kind) {gerr:.2e}", flush=True)
+    assert ierr <= (1e-6 if prec == "fp32" else 1e-5)
+    assert cerr <= (1e-4 if prec == "fp32" else 1e-3)
+    assert gerr <= (1e-4 if prec == "fp32" else 1e-3)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16x3"])
+def test_population_fullsize(ctx, prec):
+    """The batch workload at full size: 96 members (1.78M atoms, log-uniform
+    2k-60k), own 256^2 layer each, one population call.  Weights 1/T on the
+    sub-lattice of every 8th pixel per axis, 0 elsewhere; every 8th member and
+    the largest one against the FP64 oracle on that 32^2 sub-grid: I, chi^2
+    and the gradient."""
+    from diffusecode_amd.population import Population
+    crs = X.population()
+    eck, vi, inc = FR.grid_block((-4.0, -4.0, 0.0), (4.0, 4.0, 0.0), (256, 256, 1))
+    mask, sl = _sublattice(inc, 8)
+    e32, v32, i32 = eck, vi * 8, [32, 32, 1]
+    print(f"case population fullsize {prec}", flush=True)
+    ctx.set_precision("fp32")
+    pop = Population.from_crystals(crs, eck, vi, inc, ctx=ctx)
+    P, M = pop.npixels, len(crs)
+    _, I0, _ = pop.score(np.zeros((P, M), order="F"), want_icalc=True, want_grad=False)
+    rng = np.random.default_rng(23)
+    T = np.asfortranarray(I0 * (1.0 + 0.01 * rng.standard_normal(I0.shape)))
+    W = np.asfortranarray(np.where(mask.ravel(order="F")[:, None], 1.0 / np.maximum(T, 1e-3 * T.max(axis=0)), 0.0))
+    ctx.set_precision(prec)
+    try:
+        chi, I, g = pop.score(T, W, want_icalc=True, want_grad=True)
+    finally:
+        ctx.set_precision("fp32")
+    sizes = [len(m["types"]) for m in pop.members]
+    sample = sorted(set(list(range(0, M, 8)) + [int(np.argmax(sizes))]))
+    worst = [0.0, 0.0, 0.0]
+    for m in sample:
+        mb = pop.members[m]
+        F = O.model_f(mb["pos"], mb["types"], mb["occ"], mb["bdw"], mb["cfact"], e32, v32, i32, mb["rten"])
+        Iref = (np.abs(F) ** 2).ravel(order="F")
+        Tm = T[:, m].reshape(inc, order="F")[sl].ravel(order="F")
+        Wm = W[:, m].reshape(inc, order="F")[sl].ravel(order="F")
+        Im = I[:, m].reshape(inc, order="F")[sl].ravel(order="F")
+        chi_ref = float((Wm * (Iref - Tm) ** 2).sum())
+        gref = O.model_grad(mb["pos"], mb["types"], mb["occ"], mb["bdw"], mb["cfact"], e32, v32, i32, mb["rten"],
+                            F, 2.0 * Wm * (Iref - Tm))
+        gm = g[pop.atom_beg[m]:pop.atom_beg[m + 1]]
+        e = [np.abs(Im - Iref).max() / Iref.max(), abs(chi[m] - chi_ref) / chi_ref,
+             max(np.abs(gm[:, k] - gref[:, k]).max() / np.abs(gref[:, k]).max() for k in range(5))]
+        worst = [max(a, b) for a, b in zip(worst, e)]
+        print(f"  {prec} member {m} ({sizes[m]} atoms): I {e[0]:.2e} chi2 {e[1]:.2e} grad {e[2]:.2e}", flush=True)
+    # chi^2 and gradient bars scale with the forward bar (1e-6 / 1e-5 of
+    # max I): the weights 1/T (T down to 1e-3 max T) turn the absolute error
+    # of I into ~1e3 times that relative error on the weak pixels
+    assert worst[0] <= (1e-6 if prec == "fp32" else 1e-5)
+    assert worst[1] <= (1e-4 if prec == "fp32" else 1e-3)
+    assert worst[2] <= (1e-4 if prec == "fp32" else 1e-3)
