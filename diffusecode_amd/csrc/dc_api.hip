@@ -484,12 +484,17 @@ int dc_synchronize(dc_ctx *c) {
     return DC_OK;
 }
 
+}  // extern "C"
+
 // ----------------------------------------------------------------------------
-int dc_strucf(dc_ctx *c, const double *xat, int64_t nxat, int64_t ld_xat, const double uin[3],
-              const double vin[3], const double win[3], const double off_shift[9], const int lmn[6],
-              const int num[3], const int fnum[3], const double *cfact_col, int64_t ncfact, const int *istl,
-              const double xm[3], const double rten[9], int lform, double *tcsf) {
-    if (!c) return DC_ERR_ARG;
+namespace dc {
+
+// Validation and grid of the layer call (four_strucf's inputs).  Returns 0 or
+// a status; fills g and ntot.
+int strucf_setup(dc_ctx *c, const void *xat, int64_t nxat, int64_t ld_xat, const double uin[3], const double vin[3],
+                 const double win[3], const double off_shift[9], const int lmn[6], const int num[3],
+                 const int fnum[3], const void *cfact_col, int64_t ncfact, const void *istl, const double xm[3],
+                 const double rten[9], int lform, const void *tcsf, Grid &g, long long &ntot) {
     c->err.clear();
     c->err_typ = DC_ER_NONE;
     if (!uin || !vin || !win || !off_shift || !lmn || !num || !fnum || !tcsf || nxat < 0 || (nxat > 0 && (!xat || ld_xat < nxat)))
@@ -500,11 +505,7 @@ int dc_strucf(dc_ctx *c, const double *xat, int64_t nxat, int64_t ld_xat, const 
     if (lform && !istl && (!xm || !rten)) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "dc_strucf: need istl or xm+rten");
     if (nxat > INT32_MAX) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "dc_strucf: too many atoms");
     if (cancelled(c)) return set_err(c, DC_ERR_CANCEL, DC_ER_COMM, "cancelled");
-    HIPCHK(hipSetDevice(c->device));
-    std::memset(&c->stats, 0, sizeof(c->stats));
-    HIPCHK(hipEventRecord(c->ev[0], c->st));
-
-    Grid g{};
+    g = Grid{};
     for (int k = 0; k < 3; ++k) {
         g.n[k] = num[k];
         g.f[k] = fnum[k];
@@ -515,8 +516,80 @@ int dc_strucf(dc_ctx *c, const double *xat, int64_t nxat, int64_t ld_xat, const 
     }
     for (int k = 0; k < 9; ++k) g.rten[k] = rten ? rten[k] : 0.0;
     g.P = (long long)fnum[0] * fnum[1] * fnum[2];
-    const long long ntot = (long long)num[0] * num[1] * num[2];
+    ntot = (long long)num[0] * num[1] * num[2];
+    if (lform && !istl && max_corner_bin(g) >= ncfact)
+        return set_err(c, DC_ERR_ISTL, DC_ER_FOUR, "sin(theta)/lambda beyond table");
+    return 0;
+}
 
+// The layer call on device buffers: records of the nxat atoms (d_xat in
+// xat(ld,3) layout) into rec, the slices of the plan, and the phase GEMM in
+// batches of slices whose planes fit plane_bytes of partial, each batch
+// reduced into d_tcsf (cfact(istl) x FP64 sum; the first batch writes the
+// whole num grid, the next ones add).  slice_beg holds nslice + 1 ints on the
+// device.  Allocates nothing.
+int strucf_core(dc_ctx *c, const Grid &g0, long long ntot, const double *d_xat, int64_t nxat, int64_t ld_xat,
+                const double off_shift[9], const int lmn[6], const double2 *d_cfact, int64_t ncfact,
+                const int *d_istl, int lform, AtomRec *rec, int *slice_beg, float2 *partial, double plane_bytes,
+                double *d_tcsf) {
+    Grid g = g0;
+    const Plan pl = plan_gemm(g, c->tile, -1, c->prec, nxat);
+    const double one_plane = (double)sizeof(float2) * (double)pl.gg.P * planes_per_slice(pl.scheme);
+    if (one_plane > plane_bytes) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "dc_strucf: one slice plane exceeds the plane budget");
+    PrepArgs pa = base_prep(g, pl.mapping);
+    pa.mode = PREP_LMN;
+    for (int k = 0; k < 9; ++k) pa.vec[9 + k] = off_shift[k];
+    for (int k = 0; k < 6; ++k) pa.lmn[k] = lmn[k];
+    pa.src = d_xat;
+    pa.src_ld = std::max<int64_t>(ld_xat, 1);
+    pa.n = nxat;
+    HIPCHK(launch_prep(pa, rec, c->st));
+    SliceSet ss;
+    int off = 0;
+    add_segments(ss, {{0, (int)nxat}}, slice_len(c, kchunk_for(pl, nxat, pl.scheme)), off);
+    if (ss.beg.empty()) ss.beg.push_back(0);
+    const int nslice = (int)ss.type.size();
+    HIPCHK(hipMemcpyAsync(slice_beg, ss.beg.data(), sizeof(int) * ss.beg.size(), hipMemcpyHostToDevice, c->st));
+    ReduceArgs ra{};
+    ra.g = g;
+    ra.partial = partial;
+    ra.lform = lform;
+    ra.cfact = d_cfact;
+    ra.ncfact = (int)ncfact;
+    ra.istl = d_istl;
+    const int per = std::max(1, (int)std::min<double>(4096.0, std::floor(plane_bytes / one_plane)));
+    const GemmGeom gg = launch_geom(c, pl.gg);
+    HIPCHK(hipEventRecord(c->ev[2], c->st));
+    for (int s0 = 0; s0 < std::max(nslice, 1); s0 += per) {
+        if (cancelled(c)) return set_err(c, DC_ERR_CANCEL, DC_ER_COMM, "cancelled");
+        const int s1 = std::min(nslice, s0 + per);
+        if (s1 > s0) {
+            HIPCHK(launch_phase_gemm(pl.scheme, pl.tile, rec, slice_beg, partial, nullptr, gg, s0, s1 - s0, c->st));
+            c->stats.gemm_launches++;
+        }
+        ra.slice0 = s0;
+        HIPCHK(launch_strucf_out(ra, d_tcsf, ntot, s0, s1, s0 > 0, c->st));
+    }
+    HIPCHK(hipEventRecord(c->ev[3], c->st));
+    c->stats.slices = nslice;
+    c->stats.pairs = (double)nxat * (double)ntot;
+    return 0;
+}
+
+}  // namespace dc
+
+extern "C" {
+
+int dc_strucf(dc_ctx *c, const double *xat, int64_t nxat, int64_t ld_xat, const double uin[3],
+              const double vin[3], const double win[3], const double off_shift[9], const int lmn[6],
+              const int num[3], const int fnum[3], const double *cfact_col, int64_t ncfact, const int *istl,
+              const double xm[3], const double rten[9], int lform, double *tcsf) {
+    if (!c) return DC_ERR_ARG;
+    Grid g{};
+    long long ntot = 0;
+    int st = strucf_setup(c, xat, nxat, ld_xat, uin, vin, win, off_shift, lmn, num, fnum, cfact_col, ncfact, istl, xm,
+                          rten, lform, tcsf, g, ntot);
+    if (st) return st;
     if (lform && istl) {
         for (int l = 0; l < fnum[2]; ++l)
             for (int j = 0; j < fnum[1]; ++j)
@@ -524,67 +597,45 @@ int dc_strucf(dc_ctx *c, const double *xat, int64_t nxat, int64_t ld_xat, const 
                     int b = istl[i + (long long)num[0] * (j + (long long)num[1] * l)];
                     if (b < 0 || b >= ncfact) return set_err(c, DC_ERR_ISTL, DC_ER_FOUR, "istl outside cfact");
                 }
-    } else if (lform) {
-        if (max_corner_bin(g) >= ncfact) return set_err(c, DC_ERR_ISTL, DC_ER_FOUR, "sin(theta)/lambda beyond table");
     }
-
-    const Plan pl = plan_gemm(g, c->tile, -1, c->prec, nxat);
-    const GemmGeom &gg = pl.gg;
-    const int mapping = pl.mapping;
-
-    int st = 0;
-    // positions: pack xat(ld,3) columns into (n,3) Fortran layout on device
+    HIPCHK(hipSetDevice(c->device));
+    std::memset(&c->stats, 0, sizeof(c->stats));
+    HIPCHK(hipEventRecord(c->ev[0], c->st));
+    // positions: xat(ld,3) columns packed into (n,3) Fortran layout on device
     if ((st = ensure(c, c->xat, sizeof(double) * 3 * std::max<int64_t>(nxat, 1)))) return st;
     for (int col = 0; col < 3 && nxat > 0; ++col)
         HIPCHK(hipMemcpyAsync(ptr<double>(c->xat) + col * nxat, xat + col * ld_xat, sizeof(double) * nxat,
                               hipMemcpyHostToDevice, c->st));
-    if ((st = ensure(c, c->rec, sizeof(AtomRec) * std::max<int64_t>(nxat, 1)))) return st;
-    PrepArgs pa = base_prep(g, mapping);
-    pa.mode = PREP_LMN;
-    for (int k = 0; k < 9; ++k) pa.vec[9 + k] = off_shift[k];
-    for (int k = 0; k < 6; ++k) pa.lmn[k] = lmn[k];
-    pa.src = ptr<double>(c->xat);
-    pa.src_ld = std::max<int64_t>(nxat, 1);
-    pa.n = nxat;
-    HIPCHK(launch_prep(pa, ptr<AtomRec>(c->rec), c->st));
-
-    SliceSet ss;
-    int off = 0;
-    add_segments(ss, {{0, (int)nxat}}, slice_len(c, kchunk_for(pl, nxat, c->prec)), off);
-    if (ss.beg.empty()) ss.beg.push_back(0);
-    int nslice = (int)ss.type.size();
-    ss.lot = {0, nslice};
-    if ((st = upload(c, c->slice_beg, ss.beg.data(), ss.beg.size()))) return st;
-    if ((st = upload(c, c->lot_slice, ss.lot.data(), ss.lot.size()))) return st;
-    if ((st = ensure(c, c->partial, sizeof(float2) * gg.P * std::max(nslice, 1)))) return st;
-    if ((st = run_gemm(c, pl, nslice, ptr<float2>(c->partial)))) return st;
-
-    ReduceArgs ra{};
-    ra.g = g;
-    ra.partial = ptr<float2>(c->partial);
-    ra.slice0 = 0;
-    ra.lot_slice = ptr<int>(c->lot_slice);
-    ra.nlots = 1;
-    ra.lform = lform;
     if (lform) {
         if ((st = upload(c, c->cfact, cfact_col, 2 * (size_t)ncfact))) return st;
-        ra.cfact = ptr<double2>(c->cfact);
-        ra.ncfact = (int)ncfact;
-        if (istl) {
-            if ((st = upload(c, c->istl, istl, (size_t)ntot))) return st;
-            ra.istl = ptr<int>(c->istl);
-        }
+        if (istl && (st = upload(c, c->istl, istl, (size_t)ntot))) return st;
     }
+    // buffers for the records, the slice boundaries and the planes of all
+    // slices, or of as many as PARTIAL_BUDGET holds (the GEMM then runs in
+    // batches of slices)
+    Grid gp = g;
+    const Plan pl = plan_gemm(gp, c->tile, -1, c->prec, nxat);
+    const int klen = slice_len(c, kchunk_for(pl, nxat, pl.scheme));
+    const long long nslice = std::max<long long>(1, (nxat + klen - 1) / klen);
+    const double plane = (double)sizeof(float2) * (double)pl.gg.P;
+    const double planes = std::max(plane, std::min((double)PARTIAL_BUDGET, plane * (double)nslice));
+    if ((st = ensure(c, c->rec, sizeof(AtomRec) * std::max<int64_t>(nxat, 1)))) return st;
+    if ((st = ensure(c, c->slice_beg, sizeof(int) * (size_t)(nslice + 1)))) return st;
+    if ((st = ensure(c, c->partial, (size_t)planes))) return st;
     if ((st = ensure(c, c->out_tcsf, sizeof(double) * 2 * ntot))) return st;
-    HIPCHK(launch_strucf_out(ra, ptr<double>(c->out_tcsf), ntot, c->st));
+    if ((st = strucf_core(c, g, ntot, ptr<double>(c->xat), nxat, std::max<int64_t>(nxat, 1), off_shift, lmn,
+                          lform ? ptr<double2>(c->cfact) : nullptr, ncfact, lform && istl ? ptr<int>(c->istl) : nullptr,
+                          lform, ptr<AtomRec>(c->rec), ptr<int>(c->slice_beg), ptr<float2>(c->partial), planes,
+                          ptr<double>(c->out_tcsf))))
+        return st;
     HIPCHK(hipMemcpyAsync(tcsf, c->out_tcsf.p, sizeof(double) * 2 * ntot, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipEventRecord(c->ev[1], c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
     c->stats.total_ms = ms;
-    c->stats.pairs = (double)nxat * (double)ntot;
-    c->stats.slices = nslice;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+    c->stats.gemm_ms = ms;
     return DC_OK;
 }
 

@@ -118,7 +118,8 @@ hipError_t launch_reduce_lots(const ReduceArgs &a, hipStream_t st);
 hipError_t launch_reduce_sum(const ReduceArgs &a, double *out, int accumulate, hipStream_t st);
 hipError_t launch_aver_combine(const Grid &g, double *acsf, const double *lat, double norm, hipStream_t st);
 hipError_t launch_finalize(const FinalArgs &a, hipStream_t st);
-hipError_t launch_strucf_out(const ReduceArgs &a, double *tcsf, long long ntot, hipStream_t st);
+hipError_t launch_strucf_out(const ReduceArgs &a, double *tcsf, long long ntot, int s0, int s1, int accumulate,
+                             hipStream_t st);
 hipError_t launch_istl(const Grid &g, int *out, hipStream_t st);
 
 // ---- refinement model (dc_adjoint.hip) --------------------------------------

@@ -259,20 +259,26 @@ __global__ void finalize_kernel(FinalArgs a) {
 
 // Layer call output: tcsf over the num grid, zero outside the fnum region
 // (four_strucf_omp zeroes tcsf first, four_strucf_OMP_mod.f90:129).
-__global__ void strucf_out_kernel(ReduceArgs a, double *__restrict__ tcsf, long long ntot) {
+// tcsf over the full num grid from the slices [s0, s1) (planes relative to
+// a.slice0): cfact(istl) x their FP64 sum on the computed region, 0 outside;
+// accumulate != 0 adds to tcsf instead (the computed region only).
+__global__ void strucf_out_kernel(ReduceArgs a, double *__restrict__ tcsf, long long ntot, int s0, int s1,
+                                  int accumulate) {
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= ntot) return;
     const int i = (int)(q % a.g.n[0]);
     const long long qq = q / a.g.n[0];
     const int j = (int)(qq % a.g.n[1]), l = (int)(qq / a.g.n[1]);
     if (i >= a.g.f[0] || j >= a.g.f[1] || l >= a.g.f[2]) {
-        tcsf[2 * q] = 0.0;
-        tcsf[2 * q + 1] = 0.0;
+        if (!accumulate) {
+            tcsf[2 * q] = 0.0;
+            tcsf[2 * q + 1] = 0.0;
+        }
         return;
     }
     const long long po = i * a.g.ps[0] + j * a.g.ps[1] + l * a.g.ps[2];
     double fr = 0.0, fi = 0.0;
-    for (int s = a.lot_slice[0]; s < a.lot_slice[a.nlots]; ++s) {
+    for (int s = s0; s < s1; ++s) {
         const float2 v = a.partial[(long long)(s - a.slice0) * a.g.plane + po];
         fr += (double)v.x;
         fi += (double)v.y;
@@ -285,8 +291,13 @@ __global__ void strucf_out_kernel(ReduceArgs a, double *__restrict__ tcsf, long 
         fr = r;
         fi = im;
     }
-    tcsf[2 * q] = fr;
-    tcsf[2 * q + 1] = fi;
+    if (accumulate) {
+        tcsf[2 * q] += fr;
+        tcsf[2 * q + 1] += fi;
+    } else {
+        tcsf[2 * q] = fr;
+        tcsf[2 * q + 1] = fi;
+    }
 }
 
 // four_stltab over the full grid (fourier_sup.f90:2660-2685), abscissa fastest:
@@ -323,8 +334,9 @@ hipError_t launch_finalize(const FinalArgs &a, hipStream_t st) {
     finalize_kernel<<<nblk(a.ntot, 256), 256, 0, st>>>(a);
     return hipGetLastError();
 }
-hipError_t launch_strucf_out(const ReduceArgs &a, double *tcsf, long long ntot, hipStream_t st) {
-    strucf_out_kernel<<<nblk(ntot, 256), 256, 0, st>>>(a, tcsf, ntot);
+hipError_t launch_strucf_out(const ReduceArgs &a, double *tcsf, long long ntot, int s0, int s1, int accumulate,
+                             hipStream_t st) {
+    strucf_out_kernel<<<nblk(ntot, 256), 256, 0, st>>>(a, tcsf, ntot, s0, s1, accumulate);
     return hipGetLastError();
 }
 hipError_t launch_istl(const Grid &g, int *out, hipStream_t st) {
