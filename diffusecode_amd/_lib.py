@@ -33,7 +33,7 @@ CFPKT = 9999
 
 EXPORTS = [
     "dc_create", "dc_destroy", "dc_last_error", "dc_last_error_type", "dc_set_precision",
-    "dc_get_precision", "dc_set_gemm_tile", "dc_set_slice_atoms", "dc_set_phase_dither", "dc_set_adjoint_walk", "dc_gemm_num_tiles", "dc_gemm_tile_shape", "dc_set_cancel_flag", "dc_version", "dc_device_info", "dc_measure_peaks", "dc_strucf", "dc_four_stltab",
+    "dc_get_precision", "dc_set_gemm_tile", "dc_set_slice_atoms", "dc_set_phase_dither", "dc_set_adjoint_walk", "dc_gemm_num_tiles", "dc_gemm_tile_shape", "dc_set_cancel_flag", "dc_version", "dc_device_info", "dc_measure_peaks", "dc_strucf", "dc_strucf_dev_noalloc", "dc_four_stltab",
     "dc_crystal_set", "dc_four_run", "dc_four_run_dev", "dc_last_stats", "dc_synchronize",
     "dc_model_intensity", "dc_model_vjp", "dc_model_chi2", "dc_model_jvp",
     "dc_batch_chi2", "dc_set_autotune", "dc_set_graphs", "dc_batch_info",
@@ -137,6 +137,8 @@ def lib():
     L.dc_strucf.argtypes = [C.c_void_p, _dp, C.c_int64, C.c_int64, _dp, _dp, _dp, _dp, _ip, _ip, _ip, _dp,
                             C.c_int64, _ip, _dp, _dp, C.c_int, _dp]
     L.dc_four_stltab.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _ip, _dp, _ip]
+    L.dc_strucf_dev_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, _dp, _dp, _dp, _dp, _ip, _ip,
+                                        _ip, C.c_void_p, C.c_int64, C.c_void_p, _dp, _dp, C.c_int, C.c_void_p]
     L.dc_crystal_set.argtypes = [C.c_void_p, C.POINTER(CrystalDesc)]
     L.dc_four_run.argtypes = [C.c_void_p, C.POINTER(RunDesc), _dp, _dp, _dp, _dp, _dp]
     L.dc_four_run_dev.argtypes = [C.c_void_p, C.POINTER(RunDesc), C.c_void_p, C.c_void_p, C.c_void_p,
@@ -316,6 +318,19 @@ class Context:
                              _d(None if rten is None else _f64(rten, 9)), int(lform), _d(out))
         self.check(st)
         return out.view(np.complex128).reshape(tuple(num), order="F")
+
+    def strucf_dev_noalloc(self, d_xat, nxat, ld_xat, uin, vin, win, off_shift, lmn, num, fnum=None,
+                           d_cfact=None, ncfact=0, d_istl=None, xm=None, rten=None, d_tcsf=None):
+        """dc_strucf_dev_noalloc: the layer call on device buffers (raw device
+        addresses as ints: xat (ld,3) FP64, cfact complex, istl int32, tcsf
+        complex over num), allocating nothing."""
+        num = _i32(num, 3)
+        fnum = num.copy() if fnum is None else _i32(fnum, 3)
+        self.check(lib().dc_strucf_dev_noalloc(
+            self.h, d_xat, int(nxat), int(ld_xat), _d(_f64(uin, 3)), _d(_f64(vin, 3)), _d(_f64(win, 3)),
+            _d(_f64(off_shift, 9)), _i(_i32(lmn, 6)), _i(num), _i(fnum), d_cfact, int(ncfact), d_istl,
+            _d(None if xm is None else _f64(xm, 3)), _d(None if rten is None else _f64(rten, 9)),
+            int(d_cfact is not None), d_tcsf))
 
     def four_stltab(self, xm, uin, vin, win, num, rten):
         """four_stltab on the device: istl (num) int32, Fortran order (raises

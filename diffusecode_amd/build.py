@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdiffuse_turbo.so")
 SOURCES = ["dc_gemm.hip", "dc_lot.hip", "dc_api.hip", "dc_adjoint.hip", "dc_refine.hip", "dc_batch.hip", "dc_post.hip", "dc_nufft.hip",
-           "dc_peak.hip"]
+           "dc_peak.hip", "dc_noalloc.hip"]
 ROCM_LIB = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
 ARCH = os.environ.get("DC_OFFLOAD_ARCH", "gfx950")
 HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -384,6 +384,10 @@ int dc_create(dc_ctx **out, int device) {
             delete c;
             return DC_ERR_DEVICE;
         }
+    if (noalloc_attach(c) != 0) {
+        dc_destroy(c);
+        return DC_ERR_DEVICE;
+    }
     *out = c;
     return DC_OK;
 }

@@ -58,6 +58,8 @@ struct dc_ctx {
     // cells of a lot is one contiguous range
     std::vector<int> tc_start;
     DevBuf runs;
+    // this device's dc_scratch array (dc_noalloc.hip), looked up at creation
+    void *scratch = nullptr;
 
     std::vector<DevBuf *> buffers() {
         return {&rec, &slice_beg, &slice_type, &lot_slice, &partial, &cfact, &istl, &idx, &oid, &offs, &xat, &w,
@@ -158,6 +160,7 @@ inline GemmGeom launch_geom(const dc_ctx *c, const GemmGeom &g) {
 void batch_release(dc_ctx *c);   // frees the cached batch plan, buffers and graph
 void post_release(dc_ctx *c);    // frees the FFT plans and buffers of the post-processing calls
 void nufft_release(dc_ctx *c);   // frees the NUFFT fine grid, plans and buffers
+int noalloc_attach(dc_ctx *c);   // device address of this device's dc_scratch (dc_noalloc.hip)
 PrepArgs base_prep(const Grid &g, int mapping);
 
 }  // namespace dc

@@ -143,6 +143,30 @@ int dc_strucf(dc_ctx *ctx, const double *xat, int64_t nxat, int64_t ld_xat,
               int lform, double *tcsf);
 
 /*
+ * dc_strucf_dev_noalloc -- dc_strucf on device buffers that allocates
+ * nothing.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and result as dc_strucf,
+ * except that xat (ld_xat, 3), cfact_col, istl and tcsf are device pointers
+ * (istl full num grid, tcsf complex(8) over num, written over the whole grid).
+ * Its scratch -- the atom records, the slice boundaries and the slice planes,
+ * processed in batches of slices that fit -- lives in one array of
+ * DC_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; a problem
+ * whose records plus one slice plane (8 bytes per computed pixel) exceed it
+ * returns DC_ERR_ARG.  Every call on a GPU shares that GPU's array (nothing
+ * in it needs to start at zero), so calls that overlap on one GPU -- from
+ * several contexts, threads or streams -- race.  Synchronous on return.
+ */
+#define DC_NOALLOC_SCRATCH_BYTES (128 << 20)
+int dc_strucf_dev_noalloc(dc_ctx *ctx, const double *d_xat, int64_t nxat, int64_t ld_xat,
+                          const double uin[3], const double vin[3], const double win[3],
+                          const double off_shift[9], const int lmn[6],
+                          const int num[3], const int fnum[3],
+                          const double *d_cfact_col, int64_t ncfact,
+                          const int *d_istl, const double xm[3], const double rten[9],
+                          int lform, double *d_tcsf);
+
+/*
  * dc_four_stltab -- drop-in for four_stltab (discus/prog/fourier_sup.f90:
  * 2637-2687): istl(i,j,l) = NINT(sqrt(skalpro(h,h,cr_rten)/4) / CFINC) over
  * the num grid (abscissa fastest), h(k) = REAL(xm(k) + uin(k)*(i-1) +
