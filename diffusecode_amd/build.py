@@ -69,6 +69,10 @@ FORTRAN_EXAMPLE = os.path.join(HERE, "fortran", "example_strucf.f90")
 FORTRAN_EXE = os.path.join(OBJ, "fortran", "example_strucf")
 FORTRAN_EXAMPLE_POST = os.path.join(HERE, "fortran", "example_post.f90")
 FORTRAN_EXE_POST = os.path.join(OBJ, "fortran", "example_post")
+FORTRAN_EXAMPLE_REFINE = os.path.join(HERE, "fortran", "example_refine.f90")
+FORTRAN_EXE_REFINE = os.path.join(OBJ, "fortran", "example_refine")
+FORTRAN_EXAMPLE_DIFFEV = os.path.join(HERE, "fortran", "example_diffev.f90")
+FORTRAN_EXE_DIFFEV = os.path.join(OBJ, "fortran", "example_diffev")
 
 
 def build_fortran() -> str | None:
@@ -81,7 +85,8 @@ def build_fortran() -> str | None:
     os.makedirs(out, exist_ok=True)
     obj = os.path.join(out, "diffuse_turbo_mod.o")
     _run([fc, "-c", "-O2", "-fPIC", "-module-dir", out, FORTRAN_SRC, "-o", obj])
-    for src, exe in ((FORTRAN_EXAMPLE, FORTRAN_EXE), (FORTRAN_EXAMPLE_POST, FORTRAN_EXE_POST)):
+    for src, exe in ((FORTRAN_EXAMPLE, FORTRAN_EXE), (FORTRAN_EXAMPLE_POST, FORTRAN_EXE_POST),
+                     (FORTRAN_EXAMPLE_REFINE, FORTRAN_EXE_REFINE), (FORTRAN_EXAMPLE_DIFFEV, FORTRAN_EXE_DIFFEV)):
         _run([fc, "-O2", "-module-dir", out, "-I", out, src, obj, "-o", exe,
               "-L", HERE, "-ldiffuse_turbo", "-Wl,-rpath,$ORIGIN/../..", "-Wl,-rpath," + HERE])
     return FORTRAN_EXE

@@ -100,6 +100,41 @@ module diffuse_turbo_mod
       real(c_double)     :: faver_sign
    end type dc_qval_desc
 
+   ! dc_model_desc of include/diffuse_turbo.h: one refinement model (REFINE's
+   ! parameters per atom: positions, occupancies, isotropic B)
+   type, bind(C), public :: dc_model_desc
+      type(c_ptr)        :: pos            ! real(8) pos(3, natoms), fractional
+      type(c_ptr)        :: type           ! integer type(natoms), 1-based, 0 = vacancy
+      type(c_ptr)        :: occ            ! real(8) occ(natoms)
+      type(c_ptr)        :: bdw            ! real(8) bdw(natoms) or c_null_ptr
+      integer(c_int64_t) :: natoms
+      integer(c_int)     :: nscat
+      integer(c_int)     :: ncfact         ! rows of cfact (CFPKT + 1)
+      type(c_ptr)        :: cfact          ! complex(8) cfact(0:CFPKT, nscat)
+      real(c_double)     :: eck(12)
+      real(c_double)     :: vi(9)
+      integer(c_int)     :: inc(3)
+      real(c_double)     :: rten(9)
+   end type dc_model_desc
+
+   ! dc_batch_desc of include/diffuse_turbo.h: a DIFFEV generation, members
+   ! concatenated atom after atom, one shared hkl grid
+   type, bind(C), public :: dc_batch_desc
+      integer(c_int)     :: nmembers
+      type(c_ptr)        :: atom_beg       ! integer(8) atom_beg(0:nmembers), atom_beg(0) = 0
+      type(c_ptr)        :: pos            ! real(8) pos(3, natoms)
+      type(c_ptr)        :: type           ! integer type(natoms), 1-based within nscat
+      type(c_ptr)        :: occ            ! real(8) occ(natoms)
+      type(c_ptr)        :: bdw            ! real(8) bdw(natoms) or c_null_ptr
+      integer(c_int)     :: nscat
+      integer(c_int)     :: ncfact
+      type(c_ptr)        :: cfact          ! complex(8) cfact(0:CFPKT, nscat, nmembers)
+      type(c_ptr)        :: rten           ! real(8) rten(3, 3, nmembers)
+      real(c_double)     :: eck(12)
+      real(c_double)     :: vi(9)
+      integer(c_int)     :: inc(3)
+   end type dc_batch_desc
+
    ! Yell 1.0 HDF5 output (include/diffuse_yell.h, libdiffuse_yell.so):
    ! gen_hdf5_write's arguments (lib_f90/hdf_write.f90:12-16)
    type, bind(C), public :: dc_yell_desc
@@ -213,6 +248,74 @@ module diffuse_turbo_mod
          type(c_ptr), value :: type_occ, csf, dsi
       end function dc_four_run_nufft
 
+      ! ---- REFINE: the refinement model (refine/prog/refine_run.f90:491-600) ----
+      ! icalc (real) and fcalc (complex) over inc(1)*inc(2)*inc(3), c_null_ptr-able
+      integer(c_int) function dc_model_intensity(ctx, model, icalc, fcalc) bind(C, name='dc_model_intensity')
+         import :: c_ptr, c_int, dc_model_desc
+         type(c_ptr), value :: ctx
+         type(dc_model_desc), intent(in) :: model
+         type(c_ptr), value :: icalc, fcalc
+      end function dc_model_intensity
+
+      ! grad(5, natoms) = dL/d(x, y, z, occ, B) for gi = dL/dI per pixel
+      integer(c_int) function dc_model_vjp(ctx, model, fcalc, gi, grad) bind(C, name='dc_model_vjp')
+         import :: c_ptr, c_int, dc_model_desc
+         type(c_ptr), value :: ctx
+         type(dc_model_desc), intent(in) :: model
+         type(c_ptr), value :: fcalc, gi, grad
+      end function dc_model_vjp
+
+      ! di(P, ndir) [and df(P, ndir) complex] = dI/d(theta_k) for the
+      ! directions dparam(5, natoms, ndir): REFINE's derivative columns
+      integer(c_int) function dc_model_jvp(ctx, model, fcalc, ndir, dparam, di, df) bind(C, name='dc_model_jvp')
+         import :: c_ptr, c_int, dc_model_desc
+         type(c_ptr), value :: ctx
+         type(dc_model_desc), intent(in) :: model
+         type(c_ptr), value :: fcalc
+         integer(c_int), value :: ndir
+         type(c_ptr), value :: dparam, di, df
+      end function dc_model_jvp
+
+      ! chi2 = sum w (I - target)^2 (weight c_null_ptr = 1), icalc and grad(5, natoms) c_null_ptr-able
+      integer(c_int) function dc_model_chi2(ctx, model, target, weight, chi2, icalc, grad) bind(C, name='dc_model_chi2')
+         import :: c_ptr, c_int, c_double, dc_model_desc
+         type(c_ptr), value :: ctx
+         type(dc_model_desc), intent(in) :: model
+         type(c_ptr), value :: target, weight
+         real(c_double), intent(out) :: chi2
+         type(c_ptr), value :: icalc, grad
+      end function dc_model_chi2
+
+      ! ---- DIFFEV: one generation in one call (diffev/prog/diffev_kdo.f90:676) ----
+      ! target, weight, icalc (P, nmembers); chi2(nmembers); grad(5, natoms)
+      integer(c_int) function dc_batch_chi2(ctx, batch, target, weight, chi2, icalc, grad) bind(C, name='dc_batch_chi2')
+         import :: c_ptr, c_int, dc_batch_desc
+         type(c_ptr), value :: ctx
+         type(dc_batch_desc), intent(in) :: batch
+         type(c_ptr), value :: target, weight, chi2, icalc, grad
+      end function dc_batch_chi2
+
+      integer(c_int) function dc_set_autotune(ctx, on) bind(C, name='dc_set_autotune')
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: on
+      end function dc_set_autotune
+
+      integer(c_int) function dc_set_graphs(ctx, on) bind(C, name='dc_set_graphs')
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: on
+      end function dc_set_graphs
+
+      ! four_stltab on the device (fourier_sup.f90:2637-2687)
+      integer(c_int) function dc_four_stltab(ctx, xm, uin, vin, win, num, rten, istl) bind(C, name='dc_four_stltab')
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: xm(3), uin(3), vin(3), win(3), rten(9)
+         integer(c_int), intent(in) :: num(3)
+         integer(c_int), intent(out) :: istl(*)
+      end function dc_four_stltab
+
       ! in libdiffuse_yell.so (link -ldiffuse_yell where it is called)
       integer(c_int) function dc_write_yell(path, desc, qvalues, sigma) bind(C, name='dc_write_yell')
          import :: c_ptr, c_int, c_char, dc_yell_desc
@@ -225,6 +328,8 @@ module diffuse_turbo_mod
    public :: dc_create, dc_destroy, dc_set_precision, dc_last_error_type, dc_strucf
    public :: dc_crystal_set, dc_four_run, dc_four_post, dc_four_fill, dc_qval
    public :: dc_nufft1, dc_four_strucf_nufft, dc_four_run_nufft, dc_write_yell
+   public :: dc_model_intensity, dc_model_vjp, dc_model_jvp, dc_model_chi2
+   public :: dc_batch_chi2, dc_set_autotune, dc_set_graphs, dc_four_stltab
    public :: dc_four_strucf, dc_shutdown
 
    type(c_ptr), save :: the_ctx = c_null_ptr

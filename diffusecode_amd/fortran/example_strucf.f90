@@ -26,6 +26,9 @@ program example_strucf
       if (trim(arg) == 'bf16x3') then
          prec = DC_PREC_BF16X3
          bar = 1.0d-5
+      else if (trim(arg) == 'fp16x3') then
+         prec = DC_PREC_FP16X3
+         bar = 1.0d-5
       end if
    end if
    a = 0

@@ -82,3 +82,55 @@ def test_fortran_interface_module_compiles(tmp_path):
     p = subprocess.run([fc, "-c", "-module-dir", str(tmp_path), src, "-o", str(tmp_path / "m.o")],
                        capture_output=True, text=True)
     assert p.returncode == 0, p.stderr
+
+
+# descriptor fields, in declaration order, as both languages name them
+_LAYOUTS = {
+    "dc_model_desc": ["pos", "type", "occ", "bdw", "natoms", "nscat", "ncfact", "cfact", "eck", "vi", "inc", "rten"],
+    "dc_batch_desc": ["nmembers", "atom_beg", "pos", "type", "occ", "bdw", "nscat", "ncfact", "cfact", "rten", "eck",
+                      "vi", "inc"],
+    "dc_run_desc": ["eck", "vi", "inc", "rten", "cfact", "ncfact", "ilots", "ls", "nlots", "lbeg", "lperiod", "fave",
+                    "sel_cell", "four_friedel", "ano", "diff_scale"],
+    "dc_crystal_desc": ["cr_pos", "cr_iscat", "natoms", "icc", "ncatoms", "nscat", "dim0", "dim_lo"],
+}
+
+
+def test_fortran_descriptor_layouts_match_c(tmp_path):
+    """Every bind(C) descriptor of diffuse_turbo_mod has the size and field
+    offsets of its C struct (a Fortran host passes them by reference)."""
+    fc = shutil.which("flang") or "/opt/rocm/llvm/bin/flang"
+    cc = shutil.which("gcc")
+    src = os.path.join(ROOT, "diffusecode_amd", "fortran", "diffuse_turbo_mod.f90")
+    if not os.path.exists(fc) or not cc:
+        pytest.skip("flang or gcc not present")
+    fl = ["program layouts", "use iso_c_binding", "use diffuse_turbo_mod", "implicit none"]
+    for t in _LAYOUTS:
+        fl.append(f"type({t}), target :: v_{t}")
+    for t, fields in _LAYOUTS.items():
+        fl.append(f"write(*, '(a,1x,i0)') '{t} size', c_sizeof(v_{t})")
+        for f in fields:
+            fl.append(f"write(*, '(a,1x,i0)') '{t} {f}', "
+                      f"transfer(c_loc(v_{t}%{f}), 0_c_intptr_t) - transfer(c_loc(v_{t}), 0_c_intptr_t)")
+    fl.append("end program layouts")
+    (tmp_path / "layouts.f90").write_text("\n".join(fl) + "\n")
+    cl = ["#include <stdio.h>", "#include <stddef.h>", '#include "diffuse_turbo.h"', "int main(void) {"]
+    for t, fields in _LAYOUTS.items():
+        cl.append(f'printf("{t} size %zu\\n", sizeof({t}));')
+        for f in fields:
+            cl.append(f'printf("{t} {f} %zu\\n", offsetof({t}, {f}));')
+    cl.append("return 0; }")
+    (tmp_path / "layouts.c").write_text("\n".join(cl) + "\n")
+    p = subprocess.run([fc, "-c", "-module-dir", str(tmp_path), src, "-o", str(tmp_path / "m.o")],
+                       capture_output=True, text=True)
+    assert p.returncode == 0, p.stderr
+    libdir = os.path.dirname(_lib.lib()._name)
+    p = subprocess.run([fc, "-module-dir", str(tmp_path), "-I", str(tmp_path), str(tmp_path / "layouts.f90"),
+                        str(tmp_path / "m.o"), "-o", str(tmp_path / "lf"), "-L", libdir, "-ldiffuse_turbo",
+                        "-Wl,-rpath," + libdir], capture_output=True, text=True)
+    assert p.returncode == 0, p.stderr
+    p = subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(tmp_path / "layouts.c"), "-o",
+                        str(tmp_path / "lc")], capture_output=True, text=True)
+    assert p.returncode == 0, p.stderr
+    f = subprocess.run([str(tmp_path / "lf")], capture_output=True, text=True).stdout.split()
+    c = subprocess.run([str(tmp_path / "lc")], capture_output=True, text=True).stdout.split()
+    assert len(f) > 40 and f == c, (f, c)

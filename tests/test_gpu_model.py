@@ -295,3 +295,30 @@ def test_model_forward_weight_range(ctx, prec, bar, scale):
     e = rel_i(icalc, np.abs(F) ** 2)
     print(f"  {prec} occ x {scale}: max|dI|/max(I) = {e:.3e}")
     assert e <= bar
+
+
+def _run_fortran_example(name):
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "diffusecode_amd", "_build", "fortran", name)
+    assert os.path.exists(exe), "build the Fortran examples first (__graft_entry__.build)"
+    print(f"case fortran {name}", flush=True)
+    p = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(p.stdout, p.stderr, flush=True)
+    assert p.returncode == 0 and "PASS" in p.stdout
+
+
+def test_fortran_host_refine():
+    """A Fortran host fills dc_model_desc (bind(C)) and calls
+    dc_model_intensity, dc_model_jvp (REFINE's derivative columns, copied into
+    a gl_data(:,:,:,k)-shaped array) and dc_model_chi2: the columns against
+    central differences, the adjoint gradient against the columns."""
+    _run_fortran_example("example_refine")
+
+
+def test_fortran_host_diffev():
+    """A Fortran host fills dc_batch_desc (bind(C)) for a generation of three
+    members and scores it with dc_batch_chi2 twice (the second generation
+    replays the captured graph); each member against dc_model_chi2 alone."""
+    _run_fortran_example("example_diffev")
