@@ -168,12 +168,14 @@ Plan plan_gemm(Grid &g, int forced, int only_mapping, int prec, long long natoms
         const bool acc64 = prec == SCHEME_FP32_ACC64;
         // FP32 with FP64 sums is built for a few tiles only; a forced tile
         // outside them is ignored for it
-        const bool use_forced = forced >= 0 && (!acc64 || gemm_tile_info(forced).acc64);
+        const bool use_forced = forced >= 0 && (!acc64 || gemm_tile_info(forced).acc64) &&
+                                !(prec == DC_PREC_FP32 && gemm_tile_info(forced).no_fp32);
         for (int t = 0; t < nt; ++t) {
             if (use_forced && t != forced) continue;
             TileInfo ti = gemm_tile_info(t);
             if (!use_forced && ti.split_only && prec == DC_PREC_FP32) continue;
             if (acc64 && !ti.acc64) continue;
+            if ((acc64 || prec == DC_PREC_FP32) && ti.no_fp32) continue;
             const double work = (double)padded(gg.R, ti.rt) * (double)padded(gg.C, ti.ct);
             double cost;
             if (prec == DC_PREC_FP32 || acc64 || use_forced) {

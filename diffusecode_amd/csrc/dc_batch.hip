@@ -334,7 +334,8 @@ int enqueue(dc_ctx *c, BatchState &b, int tile, int nwg, bool prep, bool rest) {
 
 // Time every tile configuration and persistent grid size on this batch.
 int autotune(dc_ctx *c, BatchState &b) {
-    b.pl.tile = c->tile >= 0 ? c->tile : b.pl.tile;
+    const bool fp32_only = b.prec == DC_PREC_FP32;    // tiles built without an FP32 variant are skipped
+    if (c->tile >= 0 && !(fp32_only && gemm_tile_info(c->tile).no_fp32)) b.pl.tile = c->tile;
     b.nwg = 0;
     if (c->tile >= 0 || !c->autotune || b.nslice == 0) return 0;
     hipDeviceProp_t prop;
@@ -346,10 +347,12 @@ int autotune(dc_ctx *c, BatchState &b) {
     double wmin = 1e300;
     for (int t = 0; t < gemm_num_tiles(); ++t) {
         const TileInfo ti = gemm_tile_info(t);
+        if (fp32_only && ti.no_fp32) continue;
         wmin = std::min(wmin, (double)padded(b.pl.gg.R, ti.rt) * (double)padded(b.pl.gg.C, ti.ct));
     }
     for (int t = 0; t < gemm_num_tiles(); ++t) {
         const TileInfo ti = gemm_tile_info(t);
+        if (fp32_only && ti.no_fp32) continue;
         if ((double)padded(b.pl.gg.R, ti.rt) * (double)padded(b.pl.gg.C, ti.ct) > 1.5 * wmin) continue;
         for (int nwg : {0, ncu, 2 * ncu}) {
             int st = enqueue(c, b, t, nwg, false, false);    // warm-up

@@ -61,6 +61,8 @@
 //
 // Output: partial[slice * P + r + R * c] (float2), one plane per slice
 // (a slice = a contiguous atom range of one type of one lot).
+#include <type_traits>
+
 #include "dc_internal.h"
 
 namespace dc {
@@ -719,6 +721,7 @@ using CfgE = TileCfg<4, 2, 2, 4>;   // 128 x 128, 8 waves
 using CfgF = TileCfg<1, 4, 1, 4>;   //  16 x 256 (lines / few rows)
 using CfgG = TileCfg<2, 11, 8, 1>;  // 256 x 176 (flattened rows x one short axis, e.g. 161)
 using CfgH = TileCfg<1, 11, 8, 1>;  // 128 x 176
+using CfgI = TileCfg<2, 8, 8, 1>;   // 256 x 128 (column counts that are multiples of 128, e.g. 256^2 layers)
 
 // split_only: tiles whose FP32 variant (two accumulator sets) spills so much
 // that it is slower than the 96x256 tile (which spills a little and is still
@@ -731,17 +734,21 @@ using CfgH = TileCfg<1, 11, 8, 1>;  // 128 x 176
 // 1.24-1.39.  The smaller tiles generate more operand elements per product
 // and have more work items per launch, each with its own set-up.
 static const TileInfo kTiles[] = {
-    {CfgA::RT, CfgA::CT, CfgA::NT, 1, 1.17f, 0}, {CfgB::RT, CfgB::CT, CfgB::NT, 0, 1.28f, 0},
-    {CfgC::RT, CfgC::CT, CfgC::NT, 0, 1.62f, 1}, {CfgD::RT, CfgD::CT, CfgD::NT, 1, 2.30f, 0},
-    {CfgE::RT, CfgE::CT, CfgE::NT, 0, 1.65f, 1}, {CfgF::RT, CfgF::CT, CfgF::NT, 0, 6.30f, 1},
-    {CfgG::RT, CfgG::CT, CfgG::NT, 1, 1.00f, 0}, {CfgH::RT, CfgH::CT, CfgH::NT, 1, 1.33f, 0},
+    {CfgA::RT, CfgA::CT, CfgA::NT, 1, 1.17f, 0, 0}, {CfgB::RT, CfgB::CT, CfgB::NT, 0, 1.28f, 0, 0},
+    {CfgC::RT, CfgC::CT, CfgC::NT, 0, 1.62f, 1, 0}, {CfgD::RT, CfgD::CT, CfgD::NT, 1, 2.30f, 0, 0},
+    {CfgE::RT, CfgE::CT, CfgE::NT, 0, 1.65f, 1, 0}, {CfgF::RT, CfgF::CT, CfgF::NT, 0, 6.30f, 1, 0},
+    {CfgG::RT, CfgG::CT, CfgG::NT, 1, 1.00f, 0, 0}, {CfgH::RT, CfgH::CT, CfgH::NT, 1, 1.33f, 0, 0},
+    {CfgI::RT, CfgI::CT, CfgI::NT, 1, 1.10f, 0, 1},
 };
 
 // FP32D keeps 2 x WR x WC x 4 FP64 sums per lane beside the FP32 chains: only
 // the tiles with at most 8 MFMA blocks per wave (64 x 256, 128 x 128 / 8
 // waves, 16 x 256) fit the register file without spilling.
 template <int SCHEME, class T>
-constexpr bool tile_built() { return SCHEME != SCH_FP32D || T::WR * T::WC <= 8; }
+constexpr bool tile_built() {
+    if constexpr (std::is_same_v<T, CfgI>) return SCHEME != SCH_FP32 && SCHEME != SCH_FP32D;   // split schemes only
+    return SCHEME != SCH_FP32D || T::WR * T::WC <= 8;
+}
 
 int gemm_num_tiles() { return (int)(sizeof(kTiles) / sizeof(kTiles[0])); }
 TileInfo gemm_tile_info(int t) { return kTiles[t]; }
@@ -782,6 +789,7 @@ static hipError_t launch_scheme(int tile, const AtomRec *rec, const int *slice_b
     case 5: return launch_if_built<SCHEME, CfgF>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st);
     case 6: return launch_if_built<SCHEME, CfgG>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st);
     case 7: return launch_if_built<SCHEME, CfgH>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st);
+    case 8: return launch_if_built<SCHEME, CfgI>(rec, slice_beg, partial, sq, slice_sq, g, slice0, nslice, nwg, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -72,6 +72,7 @@ struct TileInfo {
     int split_only;         // tile offered to the split schemes only (its FP32 variant spills)
     float eff;              // split schemes: time per padded output x atom, relative to 256 x 176
     int acc64;              // built for SCHEME_FP32_ACC64
+    int no_fp32;            // no FP32 variant built (split schemes only)
 };
 
 // GEMM schemes: DC_PREC_FP32 .. DC_PREC_FP16X3 (0..3) and the FP32 scheme

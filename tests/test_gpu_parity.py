@@ -82,7 +82,7 @@ def _ntiles():
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3", "fp16x3"])
 @pytest.mark.parametrize("shape", [(41, 23, 19), (19, 37, 1), (17, 9, 5)])
-@pytest.mark.parametrize("tile", range(8))
+@pytest.mark.parametrize("tile", range(9))
 def test_every_tile_configuration(ctx, prec, shape, tile):
     """Each work-group tile shape (forced), with wrapping rows or columns
     (flattened axes), on a rocksalt crystal with vacancies and displacements
