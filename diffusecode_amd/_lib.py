@@ -36,7 +36,7 @@ EXPORTS = [
     "dc_get_precision", "dc_set_gemm_tile", "dc_set_slice_atoms", "dc_set_phase_dither", "dc_set_adjoint_walk", "dc_gemm_num_tiles", "dc_gemm_tile_shape", "dc_set_cancel_flag", "dc_version", "dc_device_info", "dc_measure_peaks", "dc_strucf", "dc_strucf_dev_noalloc", "dc_four_stltab",
     "dc_crystal_set", "dc_four_run", "dc_four_run_dev", "dc_last_stats", "dc_synchronize",
     "dc_model_intensity", "dc_model_vjp", "dc_model_chi2", "dc_model_jvp",
-    "dc_batch_chi2", "dc_set_autotune", "dc_set_graphs", "dc_batch_info",
+    "dc_batch_chi2", "dc_batch_chi2_dev", "dc_set_autotune", "dc_set_graphs", "dc_batch_info",
     "dc_four_post", "dc_four_post_dev", "dc_four_fill", "dc_four_fill_dev", "dc_four_accumulate_dev",
     "dc_qval", "dc_qval_dev", "dc_nufft1", "dc_nufft1_dev", "dc_four_strucf_nufft", "dc_four_strucf_nufft_dev",
     "dc_four_run_nufft", "dc_four_run_nufft_dev",
@@ -66,7 +66,8 @@ class RunDesc(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("pairs", C.c_double), ("gemm_ms", C.c_double), ("total_ms", C.c_double),
                 ("reduce_ms", C.c_double), ("gemm_launches", C.c_int64), ("slices", C.c_int64),
-                ("forward_ms", C.c_double), ("adjoint_ms", C.c_double), ("adjoint_walk", C.c_int64)]
+                ("forward_ms", C.c_double), ("adjoint_ms", C.c_double), ("adjoint_walk", C.c_int64),
+                ("gemm_scheme", C.c_int64)]
 
 
 class Peaks(C.Structure):
@@ -150,6 +151,8 @@ def lib():
     L.dc_model_chi2.argtypes = [C.c_void_p, C.POINTER(ModelDesc), _dp, _dp, _dp, _dp, _dp]
     L.dc_model_jvp.argtypes = [C.c_void_p, C.POINTER(ModelDesc), _dp, C.c_int, _dp, _dp, _dp]
     L.dc_batch_chi2.argtypes = [C.c_void_p, C.POINTER(BatchDesc), _dp, _dp, _dp, _dp, _dp]
+    L.dc_batch_chi2_dev.argtypes = [C.c_void_p, C.POINTER(BatchDesc), C.c_void_p, C.c_void_p, _dp, C.c_void_p,
+                                    C.c_void_p]
     L.dc_set_autotune.argtypes = [C.c_void_p, C.c_int]
     L.dc_set_graphs.argtypes = [C.c_void_p, C.c_int]
     L.dc_batch_info.argtypes = [C.c_void_p, _ip, _ip, _ip, _dp]
@@ -544,6 +547,22 @@ class Context:
         self.check(lib().dc_batch_chi2(self.h, C.byref(desc), _d(t), None if w is None else _d(w), _d(chi),
                                        None if ic is None else _d(ic), None if g is None else _d(g)))
         return chi, ic, None if g is None else g.reshape(n, 5)
+
+    def batch_chi2_dev(self, desc, pos, occ, bdw, target, weight=None, icalc=None, grad=None):
+        """Device variant: pos (N, 3), occ (N), bdw (N) or None, target and
+        weight (M, P) (member-major, abscissa fastest), icalc (M, P) and grad
+        (N, 5) float64 CUDA tensors or addresses; the other fields of desc
+        (atom_beg, types, cfact, rten, grid) are the host ones of batch_desc.
+        Returns chi2 (M,) as a host array."""
+        def p(t):
+            return None if t is None else (t if isinstance(t, int) else t.data_ptr())
+        d = BatchDesc.from_buffer_copy(desc)
+        d.pos = C.cast(C.c_void_p(p(pos)), _dp)
+        d.occ = C.cast(C.c_void_p(p(occ)), _dp)
+        d.bdw = C.cast(C.c_void_p(p(bdw)), _dp) if bdw is not None else None
+        chi = np.zeros(desc.nmembers)
+        self.check(lib().dc_batch_chi2_dev(self.h, C.byref(d), p(target), p(weight), _d(chi), p(icalc), p(grad)))
+        return chi
 
     # -- post-processing (four_run tail, four_accumulate, qval) ---------------
     @staticmethod

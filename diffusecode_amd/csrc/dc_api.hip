@@ -578,6 +578,7 @@ int strucf_core(dc_ctx *c, const Grid &g0, long long ntot, const double *d_xat, 
     }
     HIPCHK(hipEventRecord(c->ev[3], c->st));
     c->stats.slices = nslice;
+    c->stats.gemm_scheme = pl.scheme;
     c->stats.pairs = (double)nxat * (double)ntot;
     return 0;
 }
@@ -935,13 +936,19 @@ int dc_four_run_dev(dc_ctx *c, const dc_run_desc *r, double *d_csf, double *d_ac
     if (lots)
         natoms_launch = std::min<long long>(natoms_launch, (long long)std::max(r->nlots, 1) * r->ls[0] * r->ls[1] *
                                                                r->ls[2] * c->ncatoms);
-    // FP32 runs whose result is a catastrophic difference -- the <F>
-    // subtraction (fave != 0) or the cross-lot variance -- keep the GEMM's
-    // partial sums in FP64 (SCHEME_FP32_ACC64): at the Bragg pixels |F_l|
-    // exceeds |F_l - <F>| many times, and an FP32 running sum's rounding,
-    // relative to |F_l|, would stay in the difference.
+    // Runs whose result is a catastrophic difference -- the <F> subtraction
+    // (fave != 0) or the cross-lot variance -- amplify the relative error of
+    // F_l by |F_l| / |F_l - <F>| at the Bragg pixels.  FP32 keeps the GEMM's
+    // partial sums in FP64 there (SCHEME_FP32_ACC64): an FP32 running sum's
+    // rounding, relative to |F_l|, would stay in the difference.  BF16X3
+    // runs the FP16X3 products there (the same MFMA rate): BF16's 8-bit low
+    // halves leave about 2^-17 of every term, which the difference amplifies
+    // past 1e-5; FP16's 11-bit halves leave 2^-22.
     const bool want_var = d_fmean || d_dvar;
-    const int scheme = c->prec == DC_PREC_FP32 && (r->fave != 0.0 || want_var) ? SCHEME_FP32_ACC64 : c->prec;
+    const bool diff = r->fave != 0.0 || want_var;
+    const int scheme = !diff ? c->prec
+                       : c->prec == DC_PREC_FP32 ? SCHEME_FP32_ACC64
+                       : c->prec == DC_PREC_BF16X3 ? DC_PREC_FP16X3 : c->prec;
     const Plan pl = plan_gemm(g, c->tile, -1, scheme, natoms_launch);
     int st = 0;
     if ((st = upload(c, c->cfact, r->cfact, 2 * (size_t)r->ncfact * c->nscat))) return st;
@@ -1165,6 +1172,7 @@ int dc_four_run_dev(dc_ctx *c, const dc_run_desc *r, double *d_csf, double *d_ac
     c->stats.total_ms = ms;
     c->stats.pairs = pairs;
     c->stats.slices = nslices_total;
+    c->stats.gemm_scheme = scheme;
     return warn;
 }
 

@@ -37,16 +37,16 @@ struct BatchState {
     int walk = 1, nset = 0;              // adjoint column tiles per work-group, partial sets
     int walk_req = 0;                    // the context's adjoint walk setting at planning
     int64_t natoms = 0;
-    DevBuf pos, occ, bdw, idx, rec, sbeg, stype, ssq, msl, partial, cfact, rten, sq, target, weight, icalc, chi,
-        q, qs, atiles, part, grad;
+    DevBuf pos, occ, bdw, idx, rec, sbeg, stype, ssq, msl, sorder, partial, cfact, rten, sq, target, weight, icalc,
+        chi, q, qs, atiles, part, grad;
     bool primed = false;                 // records built and autotuned once
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     float tune_best_ms = 0.f;
 
     std::vector<DevBuf *> buffers() {
-        return {&pos, &occ, &bdw, &idx, &rec, &sbeg, &stype, &ssq, &msl, &partial, &cfact, &rten, &sq, &target,
-                &weight, &icalc, &chi, &q, &qs, &atiles, &part, &grad};
+        return {&pos, &occ, &bdw, &idx, &rec, &sbeg, &stype, &ssq, &msl, &sorder, &partial, &cfact, &rten, &sq,
+                &target, &weight, &icalc, &chi, &q, &qs, &atiles, &part, &grad};
     }
 };
 
@@ -227,6 +227,14 @@ int build_state(dc_ctx *c, const dc_batch_desc *d, BatchState &b) {
     if ((st = upload(c, b.stype, ss.type.data(), ss.type.size()))) return st;
     if ((st = upload(c, b.ssq, slice_sq.data(), slice_sq.size()))) return st;
     if ((st = upload(c, b.msl, mslice.data(), mslice.size()))) return st;
+    // GEMM launch order of the slices: longest first.  Members of uneven size
+    // give slices of uneven length, and in member order a long slice can
+    // start in the launch's last round of work-groups and run alone
+    std::vector<int> sorder(std::max(b.nslice, 1), 0);
+    for (int s = 0; s < b.nslice; ++s) sorder[s] = s;
+    std::stable_sort(sorder.begin(), sorder.begin() + b.nslice,
+                     [&](int x, int y) { return ss.beg[x + 1] - ss.beg[x] > ss.beg[y + 1] - ss.beg[y]; });
+    if ((st = upload(c, b.sorder, sorder.data(), sorder.size()))) return st;
     if ((st = upload(c, b.atiles, tiles.data(), tiles.size()))) return st;
     if ((st = ensure(c, b.pos, sizeof(double) * 3 * T))) return st;
     if ((st = ensure(c, b.occ, sizeof(double) * T))) return st;
@@ -267,9 +275,10 @@ int enqueue(dc_ctx *c, BatchState &b, int tile, int nwg, bool prep, bool rest) {
         pa.n = b.nsorted;
         if (b.nsorted > 0) HIPCHK(launch_prep(pa, ptr<AtomRec>(b.rec), c->st));
     }
+    GemmGeom lg = launch_geom(c, gg);
+    lg.order = ptr<int>(b.sorder);
     HIPCHK(launch_phase_gemm(c->prec, tile, ptr<AtomRec>(b.rec), ptr<int>(b.sbeg), ptr<float2>(b.partial),
-                             b.any_b ? ptr<float>(b.sq) : nullptr, launch_geom(c, gg), 0, b.nslice, c->st, nwg,
-                             ptr<int>(b.ssq)));
+                             b.any_b ? ptr<float>(b.sq) : nullptr, lg, 0, b.nslice, c->st, nwg, ptr<int>(b.ssq)));
     if (!rest) return 0;
     BatchReduceArgs ra{};
     ra.g = b.g;
@@ -402,8 +411,16 @@ int dc_batch_info(const dc_ctx *c, int *tile, int *nwg, int *graph, double *tune
     return DC_OK;
 }
 
-int dc_batch_chi2(dc_ctx *c, const dc_batch_desc *d, const double *target, const double *weight, double *chi2,
-                  double *icalc, double *grad) {
+}  // extern "C"
+
+namespace {
+
+// One population step.  dev = false: pos/occ/bdw of d, target, weight, icalc
+// and grad are host arrays; dev = true: they are device arrays (the inputs
+// are copied on the device into the buffers the graph reads, the outputs
+// out of them), and a non-null bdw counts as per-atom B.
+int batch_chi2(dc_ctx *c, const dc_batch_desc *d, const double *target, const double *weight, double *chi2,
+               double *icalc, double *grad, bool dev) {
     if (!c) return DC_ERR_ARG;
     c->err.clear();
     c->err_typ = DC_ER_NONE;
@@ -414,8 +431,10 @@ int dc_batch_chi2(dc_ctx *c, const dc_batch_desc *d, const double *target, const
     HIPCHK(hipSetDevice(c->device));
     std::memset(&c->stats, 0, sizeof(c->stats));
     const int64_t natoms = d->atom_beg[d->nmembers];
-    bool any_b = false;
-    if (d->bdw)
+    const hipMemcpyKind in = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const hipMemcpyKind out = dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    bool any_b = dev && d->bdw != nullptr;
+    if (d->bdw && !dev)
         for (int64_t k = 0; k < natoms && !any_b; ++k) any_b = d->type[k] > 0 && d->bdw[k] != 0.0;
     const bool want_grad = grad != nullptr, want_icalc = icalc != nullptr, have_weight = weight != nullptr;
     if (!same_shape(c->batch, c, d, want_grad, want_icalc, have_weight, any_b)) {
@@ -435,14 +454,14 @@ int dc_batch_chi2(dc_ctx *c, const dc_batch_desc *d, const double *target, const
     std::vector<float> sq;
     if ((st = member_sq(c, b, d, sq, any_b || want_grad))) return st;
     HIPCHK(hipEventRecord(c->ev[0], c->st));
-    HIPCHK(hipMemcpyAsync(b.pos.p, d->pos, sizeof(double) * 3 * natoms, hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(b.occ.p, d->occ, sizeof(double) * natoms, hipMemcpyHostToDevice, c->st));
-    if (any_b) HIPCHK(hipMemcpyAsync(b.bdw.p, d->bdw, sizeof(double) * natoms, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(b.pos.p, d->pos, sizeof(double) * 3 * natoms, in, c->st));
+    HIPCHK(hipMemcpyAsync(b.occ.p, d->occ, sizeof(double) * natoms, in, c->st));
+    if (any_b) HIPCHK(hipMemcpyAsync(b.bdw.p, d->bdw, sizeof(double) * natoms, in, c->st));
     HIPCHK(hipMemcpyAsync(b.cfact.p, d->cfact, sizeof(double) * 2 * d->ncfact * d->nscat * M, hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(b.rten.p, d->rten, sizeof(double) * 9 * M, hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(b.sq.p, sq.data(), sizeof(float) * sq.size(), hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(b.target.p, target, sizeof(double) * P * M, hipMemcpyHostToDevice, c->st));
-    if (weight) HIPCHK(hipMemcpyAsync(b.weight.p, weight, sizeof(double) * P * M, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(b.target.p, target, sizeof(double) * P * M, in, c->st));
+    if (weight) HIPCHK(hipMemcpyAsync(b.weight.p, weight, sizeof(double) * P * M, in, c->st));
 
     if (!b.primed) {
         // records of this call, then the tile / grid-size choice, then the graph
@@ -472,8 +491,8 @@ int dc_batch_chi2(dc_ctx *c, const dc_batch_desc *d, const double *target, const
     HIPCHK(hipEventRecord(c->ev[5], c->st));
     std::vector<double> part((size_t)b.nblk * M);
     HIPCHK(hipMemcpyAsync(part.data(), b.chi.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost, c->st));
-    if (icalc) HIPCHK(hipMemcpyAsync(icalc, b.icalc.p, sizeof(double) * P * M, hipMemcpyDeviceToHost, c->st));
-    if (grad) HIPCHK(hipMemcpyAsync(grad, b.grad.p, sizeof(double) * 5 * natoms, hipMemcpyDeviceToHost, c->st));
+    if (icalc) HIPCHK(hipMemcpyAsync(icalc, b.icalc.p, sizeof(double) * P * M, out, c->st));
+    if (grad) HIPCHK(hipMemcpyAsync(grad, b.grad.p, sizeof(double) * 5 * natoms, out, c->st));
     HIPCHK(hipEventRecord(c->ev[1], c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     for (size_t m = 0; m < M; ++m) {
@@ -489,9 +508,24 @@ int dc_batch_chi2(dc_ctx *c, const dc_batch_desc *d, const double *target, const
     c->stats.pairs = (double)b.nsorted * (double)P;
     c->stats.slices = b.nslice;
     c->stats.gemm_launches = 1;
+    c->stats.gemm_scheme = b.prec;
     c->stats.gemm_ms = b.tune_best_ms;
     c->stats.adjoint_walk = want_grad ? b.walk : 0;
     return DC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dc_batch_chi2(dc_ctx *c, const dc_batch_desc *d, const double *target, const double *weight, double *chi2,
+                  double *icalc, double *grad) {
+    return batch_chi2(c, d, target, weight, chi2, icalc, grad, false);
+}
+
+int dc_batch_chi2_dev(dc_ctx *c, const dc_batch_desc *d, const double *target, const double *weight, double *chi2,
+                      double *icalc, double *grad) {
+    return batch_chi2(c, d, target, weight, chi2, icalc, grad, true);
 }
 
 }  // extern "C"

@@ -37,6 +37,7 @@ struct GemmGeom {
     int Rs;
     long long P;
     uint32_t dither;   // per-atom phase rotation salt (0: none), see dither_turn
+    const int *order;  // launch order of the slices (null: slice0, slice0 + 1, ...)
 };
 
 // Per-atom phase rotation of the phase GEMM: a pseudo-random turn fraction

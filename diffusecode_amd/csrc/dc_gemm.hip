@@ -507,9 +507,12 @@ phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice
     __shared__ Smem<SCHEME, T> sm;
     const int nct = (g.C + T::CT - 1) / T::CT, nrt = (g.R + T::RT - 1) / T::RT;
     // work items (column tile fastest, then row tile, then slice); a launch
-    // with fewer work-groups than items walks them with a grid stride
+    // with fewer work-groups than items walks them with a grid stride.  The
+    // slices are taken in g.order when given (longest first, so that uneven
+    // slices leave no long item for the last round); planes stay in slice order
     for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
-        const int slice = slice0 + item / (nct * nrt);
+        const int si = item / (nct * nrt);
+        const int slice = slice0 + (g.order ? g.order[si] : si);
         const int kbeg = slice_beg[slice], kend = slice_beg[slice + 1];
         const int row0 = ((item / nct) % nrt) * T::RT, col0 = (item % nct) * T::CT;
         const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);

@@ -52,7 +52,9 @@ extern "C" {
 
 /* ---- arithmetic of the phase GEMM --------------------------------------- */
 #define DC_PREC_FP32 0          /* plain FP32 MFMA (the drop-in)            */
-#define DC_PREC_BF16X3 1        /* split BF16 hi/lo, 3 MFMA products        */
+#define DC_PREC_BF16X3 1        /* split BF16 hi/lo, 3 MFMA products; runs with
+                                   the <F> subtraction or the cross-lot variance
+                                   use the FP16X3 products (dc_four_run*)    */
 #define DC_PREC_FP16X2 2        /* split FP16 hi/lo on one side, 2 products */
 #define DC_PREC_FP16X3 3        /* split FP16 hi/lo on both sides, 3 MFMA products */
 
@@ -243,6 +245,8 @@ typedef struct dc_stats {
     double adjoint_ms;          /* refinement model: reverse pass (Q, GEMM, sums) */
     int64_t adjoint_walk;       /* column tiles per work-group of the last reverse
                                    pass (1: one work-group per column tile)       */
+    int64_t gemm_scheme;        /* products of the last phase GEMM: DC_PREC_* value,
+                                   or 4 = FP32 products with FP64 partial sums    */
 } dc_stats;
 int dc_last_stats(const dc_ctx *ctx, dc_stats *out);
 int dc_synchronize(dc_ctx *ctx);
@@ -341,6 +345,14 @@ typedef struct dc_batch_desc {
 
 int dc_batch_chi2(dc_ctx *ctx, const dc_batch_desc *batch, const double *target, const double *weight,
                   double *chi2, double *icalc, double *grad);
+/* The same step with the per-generation arrays in device memory: batch->pos,
+ * ->occ, ->bdw (non-NULL = per-atom B), target, weight, icalc and grad are
+ * device pointers; atom_beg, type, cfact, rten and chi2 stay host arrays.
+ * For a host loop that keeps a generation on the GPU (DIFFEV's trial
+ * vectors built on the device); replaces the same per-member macro runs as
+ * dc_batch_chi2.  Returns when chi2 is written. */
+int dc_batch_chi2_dev(dc_ctx *ctx, const dc_batch_desc *batch, const double *target, const double *weight,
+                      double *chi2, double *icalc, double *grad);
 /* Autotuner and hipGraph capture of batch calls (both on by default). */
 int dc_set_autotune(dc_ctx *ctx, int on);
 int dc_set_graphs(dc_ctx *ctx, int on);

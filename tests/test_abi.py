@@ -59,6 +59,8 @@ def test_null_context_arguments():
     assert L.dc_four_fill(None, None, None, None) == _lib.DC_ERR_ARG
     assert L.dc_four_accumulate_dev(None, 0, None, None, None, None, None) == _lib.DC_ERR_ARG
     assert L.dc_qval(None, None, None, None, None, None) == _lib.DC_ERR_ARG
+    assert L.dc_batch_chi2(None, None, None, None, None, None, None) == _lib.DC_ERR_ARG
+    assert L.dc_batch_chi2_dev(None, None, None, None, None, None, None) == _lib.DC_ERR_ARG
     assert L.dc_measure_peaks(None, 0, None) == _lib.DC_ERR_ARG
 
 

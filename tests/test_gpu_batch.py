@@ -176,3 +176,39 @@ def test_autotuner_choice(ctx):
     assert info["graph"] and info["tune_ms"] > 0.0
     chi2, _, _ = pop.score(T, want_grad=False)      # replay of the same shape
     np.testing.assert_array_equal(chi, chi2)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16x3"])
+def test_batch_device_arrays_match_host(prec):
+    """dc_batch_chi2_dev (generation arrays in HBM) gives the host entry
+    point's chi^2, intensities and gradients bit for bit, on two generations."""
+    import torch
+    dev = torch.device("cuda:0")
+    hc, dc_ = _lib.Context(precision=prec), _lib.Context(precision=prec)
+    try:
+        for c in (hc, dc_):
+            c.set_autotune(False)   # same tile in both contexts
+        pop = make_population(ctx=hc)
+        T = targets(pop)
+        W = 1.0 / np.maximum(T, 1e-3 * T.max(axis=0))
+        pos, occ, bdw = pop.params()
+        desc, keep = pop.desc()
+        M, P, n = len(pop.members), pop.npixels, pop.natoms
+        d_t = torch.from_numpy(np.ascontiguousarray(T.T)).to(dev)
+        d_w = torch.from_numpy(np.ascontiguousarray(W.T)).to(dev)
+        d_i = torch.zeros((M, P), dtype=torch.float64, device=dev)
+        d_g = torch.zeros((n, 5), dtype=torch.float64, device=dev)
+        rng = np.random.default_rng(1)
+        for gen in range(2):
+            p = pos + (0.01 * gen) * rng.standard_normal(pos.shape)
+            chi, I, g = pop.score(T, W, pos=p, want_icalc=True, want_grad=True)
+            d_pos = torch.from_numpy(np.ascontiguousarray(p)).to(dev)
+            d_occ = torch.from_numpy(np.ascontiguousarray(occ)).to(dev)
+            d_bdw = torch.from_numpy(np.ascontiguousarray(bdw)).to(dev)
+            chi_d = dc_.batch_chi2_dev(desc, d_pos, d_occ, d_bdw, d_t, d_w, icalc=d_i, grad=d_g)
+            assert np.array_equal(chi, chi_d), (gen, chi, chi_d)
+            assert np.array_equal(I, d_i.cpu().numpy().T), gen
+            assert np.array_equal(g, d_g.cpu().numpy()), gen
+    finally:
+        hc.close()
+        dc_.close()

@@ -182,12 +182,12 @@ def test_four_run_rocksalt_hk0(ctx, prec):
 # max|dI|/max(I) bars of the Bragg-subtracted lot runs, where the diffuse
 # intensity |F_l - <F>|^2 is small beside |F_l|^2 at the Bragg pixels, so the
 # error of F_l (relative to |F_l|) is amplified by |F_l| / |F_l - <F>| (up to
-# ~30 in this case).  FP32 then keeps its partial sums in FP64 (the library
-# does this by itself for such runs) and FP16X3 (one rounding per 16 atoms)
-# hold the bars of the direct sums.  BF16X3 carries ~2^-17 per product (its
-# lo halves keep 8 bits and lo*lo is dropped): 2.3e-5 here, held to 4e-5; it
-# is the faster, lower-precision split scheme for such runs.
-LOTS_BAR = {"fp32": 1e-6, "fp16x3": 1e-5, "bf16x3": 4e-5}
+# ~30 in this case).  The library switches such runs to products that hold the
+# direct sums' bars: FP32 keeps its partial sums in FP64, and BF16X3 (whose
+# 8-bit lo halves leave ~2^-17 per product: 2.3e-5 here with them) runs the
+# FP16X3 products, one rounding per 16 atoms at 2^-22 per product.
+LOTS_BAR = {"fp32": 1e-6, "fp16x3": 1e-5, "bf16x3": 1e-5}
+LOTS_SCHEME = {"fp32": 4, "fp16x3": 3, "bf16x3": 3}
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3", "fp16x3"])
@@ -202,6 +202,7 @@ def test_four_run_lots_aver(ctx, prec):
     lbeg = np.floor(rng.random((6, 3)) * 8).astype(np.int32) + 1
     sel = (rng.random((8, 8, 8)) <= 0.5).astype(np.uint8)
     gpu, ref = run_pair(ctx, c, prec, eck, vi, inc, ilots=2, ls=(3, 3, 3), lbeg=lbeg, fave=0.5, sel_cell=sel)
+    assert ctx.stats()["gemm_scheme"] == LOTS_SCHEME[prec]
     for k in ("dsi", "dvar", "acsf"):
         err = float(np.abs(gpu[k] - ref[k]).max() / np.abs(ref[k]).max())
         print(f"  {k} max rel {err:.3e}", flush=True)
