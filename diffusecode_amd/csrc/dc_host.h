@@ -65,7 +65,7 @@ struct dc_ctx {
         return {&rec, &slice_beg, &slice_type, &lot_slice, &partial, &cfact, &istl, &idx, &oid, &offs, &xat, &w,
                 &acsf, &dsi, &csf, &wm, &wv, &lat, &out_tcsf, &out[0], &out[1], &out[2], &out[3], &out[4],
                 &m_pos, &m_w, &m_b, &m_sq, &m_q, &m_part, &m_grad, &m_tiles, &m_gi, &m_f, &m_tgt, &m_qt, &m_qs, &m_bd, &m_jw, &m_jidx, &m_dsl, &m_di,
-                &pos, &iscat, &type_idx};
+                &pos, &iscat, &type_idx, &runs};
     }
 };
 
