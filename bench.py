@@ -488,16 +488,26 @@ def main():
         st, lmn, off = O.fourier_lmn(eck, vi, inc)
         num_s = [inc[0], inc[1], nl]
         fnum_s = [(inc[0] + 1) // 2, inc[1], nl]
+        # DISCUS's own four_strucf when its build is present (compiled from the
+        # reference sources, oracle/build_ref.py), else the oracle's bit-exact
+        # restatement of it
+        ref = O.ref_available()
         t = time.perf_counter()
         for t1 in range(1, c.nscat + 1):
             sel = c.iscat == t1
-            O.strucf_turbo(c.pos[sel], vi[:, 0], vi[:, 1], vi[:, 2], off, lmn, num_s, fnum=fnum_s,
-                           nthreads=nthr)
+            if ref:
+                O.ref_strucf(c.pos[sel], vi[:, 0], vi[:, 1], vi[:, 2], off, lmn, num_s, fnum=fnum_s,
+                             nthreads=nthr)
+            else:
+                O.strucf_turbo(c.pos[sel], vi[:, 0], vi[:, 1], vi[:, 2], off, lmn, num_s, fnum=fnum_s,
+                               nthreads=nthr)
         dt = time.perf_counter() - t
         cpairs = natoms * inc[0] * inc[1] * nl
-        cpu = {"value": cpairs / dt, "unit": "atom·hkl pairs/s", "cores": nthr, "kind": "port",
-               "sample": f"DISCUS turbo restatement (oracle/dc_oracle.c, OpenMP over abscissa rows, "
-                         f"Friedel-halved like DISCUS) on {nl} of 161 l-layers of the same workload, all "
+        what = ("DISCUS's four_strucf (four_strucf_OMP_mod.f90:19-188 with its OpenMP kernel, compiled from the "
+                "reference sources with flang -O3 -ffast-math -fopenmp, oracle/build_ref.py)" if ref else
+                "DISCUS turbo restatement (oracle/dc_oracle.c, OpenMP over abscissa rows)")
+        cpu = {"value": cpairs / dt, "unit": "atom·hkl pairs/s", "cores": nthr, "kind": "reference" if ref else "port",
+               "sample": f"{what}, Friedel-halved like DISCUS, on {nl} of 161 l-layers of the same workload, all "
                          f"{natoms} atoms: {cpairs:.3e} pairs in {dt:.1f}s",
                "host": cpu_info()}
         log(f"cpu turbo baseline: {cpu['value'] / 1e9:.2f} Gpairs/s on {nthr} threads")

@@ -20,7 +20,13 @@ Functions and what pins them (tests/test_oracle.py unless noted):
                  index against floor(iarg/64) mod 2^16 on hand-picked
                  values; closeness to strucf_exact within the table's
                  derived error bound (2^-16 truncation + 2^-23 per index
-                 step); exact equality with strucf_exact at h=0.
+                 step); exact equality with strucf_exact at h=0; and bit for
+                 bit against DISCUS's own four_strucf compiled from the
+                 reference sources (ref_strucf below; tests/test_ref_strucf.py
+                 and the vectors of tests/golden/ref_strucf_vectors.npz).
+  ref_strucf     Not a restatement: DISCUS's four_strucf itself, built by
+                 oracle/build_ref.py into oracle/_ref/ (present only where
+                 the reference sources were available at build time).
   fourier_lmn    Reconstruction identity eck(:,1) = sum lmn(1:3) vi +
                  sum lmn(4:6) off(:,j) to 1e-7, and +-h phase identity.
   stltab         Closed form s = |h|/(2a) on a cubic metric, nint bin.
@@ -173,6 +179,42 @@ def strucf_turbo(pos, uin, vin, win, off, lmn, num, fnum=None, cfact_col=None, i
                            _d(cexf), _d(cf), _i(ist), C.c_int(int(lform)), _d(out),
                            C.c_int(nthreads or nthreads_default()))
     return out.view(np.complex128).reshape(tuple(num), order="F")
+
+
+_REF_PATH = os.path.join(_HERE, "_ref", "libref_strucf.so")
+_ref = None
+
+
+def ref_available() -> bool:
+    """True when DISCUS's own kernel has been built into oracle/_ref
+    (oracle/build_ref.py, from the reference sources)."""
+    return os.path.exists(_REF_PATH)
+
+
+def ref_strucf(pos, uin, vin, win, off, lmn, num, fnum=None, cfact_col=None, istl=None, nthreads=1, variant=0):
+    """tcsf of DISCUS's own four_strucf (four_strucf_OMP_mod.f90:19-68),
+    compiled from the reference sources (oracle/build_ref.py): one type,
+    atoms pos (n, 3), the same arguments as strucf_turbo.  variant 0 = the
+    dispatcher, 1 = four_strucf_serial, 2 = four_strucf_omp."""
+    global _ref
+    if _ref is None:
+        # lazy binding: four GNU-extension and evaluator symbols of unreachable
+        # file-information code stay unresolved (oracle/build_ref.py)
+        _ref = C.CDLL(_REF_PATH, mode=os.RTLD_LAZY)
+        _ref.ref_strucf.restype = None
+    x = np.asfortranarray(np.asarray(pos, np.float64).reshape(-1, 3))
+    n = x.shape[0]
+    num = _i32(num, 3)
+    fnum = num if fnum is None else _i32(fnum, 3)
+    cf = np.ascontiguousarray(np.ones(1, np.complex128) if cfact_col is None else np.asarray(cfact_col, np.complex128))
+    ist = np.asfortranarray(np.zeros(tuple(num), np.int32) if istl is None else
+                            np.asarray(istl, np.int32).reshape(tuple(num), order="F"))
+    out = np.zeros(tuple(num), np.complex128, order="F")
+    _ref.ref_strucf(C.c_int(n), x.ctypes.data_as(_dp), _d(_f64(uin, 3)), _d(_f64(vin, 3)), _d(_f64(win, 3)),
+                    _d(_f64(off, 9)), _i(_i32(lmn, 6)), _i(num), _i(fnum), C.c_int(cf.size),
+                    cf.ctypes.data_as(_dp), ist.ctypes.data_as(C.POINTER(C.c_int)), C.c_int(int(cfact_col is not None)),
+                    C.c_int(int(nthreads)), C.c_int(int(variant)), out.ctypes.data_as(_dp))
+    return out
 
 
 def test_friedel(eck, vi, inc, ano=False, four_friedel=True):
