@@ -68,7 +68,9 @@ struct ColTables {
 template <int S>
 struct RowTablesT {
     float2 rh[S][NA];
-    float2 rl[NA][16];
+    // row-major with a padded row (8 float2): the stage builders read 4 rows
+    // x 8 atoms per half-wave, which then fall on distinct banks
+    float2 rl[16][NA + 8];
 };
 using RowTables = RowTablesT<MAXST>;
 
@@ -93,7 +95,7 @@ __device__ __forceinline__ void build_row_tables(RT &rt, const AtomRec *at, int 
     for (int x = threadIdx.x; x < NA * (16 + nst); x += NT) {
         const int n = x % NA, e = x / NA;
         if (e < 16)
-            rt.rl[n][e] = cis((uint64_t)e * at[n].ra);
+            rt.rl[e][n] = cis((uint64_t)e * at[n].ra);
         else
             rt.rh[e - 16][n] = cis((uint64_t)(16 * (e - 16)) * at[n].ra);
     }
@@ -105,7 +107,7 @@ __device__ __forceinline__ float2 row_element(const RT &rt, const AtomRec &ar, i
                                               float s2) {
     float2 v;
     if (tab) {
-        const float2 h = rt.rh[s][n], l = rt.rl[n][rr];
+        const float2 h = rt.rh[s][n], l = rt.rl[rr][n];
         v = make_float2(h.x * l.x - h.y * l.y, h.x * l.y + h.y * l.x);
     } else {
         v = cis((uint64_t)r * ar.ra);
@@ -395,7 +397,11 @@ template <class RT>
 __device__ __forceinline__ void gen_stage_split(StageA &dst, const RT &rt, const AtomRec *ats, const AdjArgs &a,
                                                 const float *__restrict__ sq, int st, int na, bool tab) {
     const int t = threadIdx.x;
-    const int at = t & (NA - 1), g = (t / NA) & 3, e = t / (4 * NA);
+    // e fastest: a wave's 4-byte stores of one plane cover 64 consecutive words
+    // (with the atom fastest they met on the same banks four at a time;
+    // measured with the padded row tables: population forward + adjoint
+    // 11.73 -> 11.43 ms, identical results)
+    const int e = t & 3, at = (t >> 2) & (NA - 1), g = t / (4 * NA);
     const int rr = 4 * g + e, r = st * SKR + rr;
     float2 v = make_float2(0.f, 0.f);
     float s2 = 0.0f;
