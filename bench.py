@@ -510,7 +510,7 @@ def main():
                "sample": f"{what}, Friedel-halved like DISCUS, on {nl} of 161 l-layers of the same workload, all "
                          f"{natoms} atoms: {cpairs:.3e} pairs in {dt:.1f}s",
                "host": cpu_info()}
-        log(f"cpu turbo baseline: {cpu['value'] / 1e9:.2f} Gpairs/s on {nthr} threads")
+        log(f"cpu baseline ({cpu['kind']}): {cpu['value'] / 1e9:.2f} Gpairs/s on {nthr} threads")
 
     line = {
         "metric": METRIC, "value": value, "unit": "atom·hkl pairs/s", "n_gpus": 1, "steps": args.steps,
