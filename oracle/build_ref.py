@@ -1,9 +1,10 @@
 """Build recipe for oracle/_ref/libref_strucf.so -- TEST INFRASTRUCTURE ONLY.
 
-Compiles DISCUS's per-type structure-factor kernel from the reference sources
-where they lie (discus/prog/four_strucf_OMP_mod.f90 and the modules it uses,
-39 files of discus/prog and lib_f90) with the image's flang, plus the C entry
-oracle/ref_strucf_bind.f90, into oracle/_ref/.  Nothing of the reference is
+Compiles DISCUS's per-type structure-factor kernel and its corner-phase
+construction from the reference sources where they lie
+(discus/prog/four_strucf_OMP_mod.f90, discus/prog/fourier_lmn_mod.f90 and the
+modules they use, 51 files of discus/prog and lib_f90) with the image's flang,
+plus the C entries of oracle/ref_strucf_bind.f90, into oracle/_ref/.  Nothing of the reference is
 copied: the sources are read from REF (default /root/reference) and only
 objects, module files and the shared library are written, under oracle/_ref/.
 
@@ -39,6 +40,7 @@ OUT = os.path.join(HERE, "_ref")
 LIB = os.path.join(OUT, "libref_strucf.so")
 REF = os.environ.get("DC_REFERENCE", "/root/reference")
 ROOT_SOURCE = "discus/prog/four_strucf_OMP_mod.f90"
+ROOT_SOURCES = [ROOT_SOURCE, "discus/prog/fourier_lmn_mod.f90"]
 FLAGS = ["-fPIC", "-cpp", "-O3", "-ffast-math", "-fopenmp"]
 INTRINSIC = {"omp_lib", "iso_c_binding", "iso_fortran_env", "ieee_arithmetic", "mpi"}
 # modules that several reference files define: the file the reference's own
@@ -98,7 +100,8 @@ def compile_order(ref=REF):
                 visit(mods[m])
         order.append(p)
 
-    visit(os.path.join(ref, ROOT_SOURCE))
+    for root in ROOT_SOURCES:
+        visit(os.path.join(ref, root))
     return order
 
 
@@ -127,7 +130,8 @@ def build(force=False, verbose=False):
         objs.append(o)
     libdir = os.path.join(os.path.dirname(os.path.dirname(fc)), "lib")
     tmp = LIB + ".tmp"
-    cmd = [fc, "-shared", "-fopenmp", "-o", tmp, *objs, "-Wl,-rpath," + libdir]
+    # fourier_lmn computes in REAL(16): flang calls libquadmath for it
+    cmd = [fc, "-shared", "-fopenmp", "-o", tmp, *objs, "-lquadmath", "-Wl,-rpath," + libdir]
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         sys.stderr.write(p.stdout + p.stderr)

@@ -24,11 +24,14 @@ Functions and what pins them (tests/test_oracle.py unless noted):
                  bit against DISCUS's own four_strucf compiled from the
                  reference sources (ref_strucf below; tests/test_ref_strucf.py
                  and the vectors of tests/golden/ref_strucf_vectors.npz).
-  ref_strucf     Not a restatement: DISCUS's four_strucf itself, built by
+  ref_strucf / ref_fourier_lmn
+                 Not restatements: DISCUS's four_strucf and fourier_lmn, built by
                  oracle/build_ref.py into oracle/_ref/ (present only where
                  the reference sources were available at build time).
   fourier_lmn    Reconstruction identity eck(:,1) = sum lmn(1:3) vi +
-                 sum lmn(4:6) off(:,j) to 1e-7, and +-h phase identity.
+                 sum lmn(4:6) off(:,j) to 1e-7, and +-h phase identity; and
+                 lmn / off equal to DISCUS's own fourier_lmn (ref_fourier_lmn,
+                 tests/test_ref_strucf.py, tests/golden/ref_lmn_vectors.json).
   stltab         Closed form s = |h|/(2a) on a cubic metric, nint bin.
   formtab        Closed-form 4-Gaussian + exp(-B s^2) occupancy factor.
   test_friedel / apply_friedel
@@ -215,6 +218,24 @@ def ref_strucf(pos, uin, vin, win, off, lmn, num, fnum=None, cfact_col=None, ist
                     cf.ctypes.data_as(_dp), ist.ctypes.data_as(C.POINTER(C.c_int)), C.c_int(int(cfact_col is not None)),
                     C.c_int(int(nthreads)), C.c_int(int(variant)), out.ctypes.data_as(_dp))
     return out
+
+
+def ref_fourier_lmn(eck, vi, inc):
+    """(ier_num, lmn, off) of DISCUS's own fourier_lmn
+    (discus/prog/fourier_lmn_mod.f90:4-279), from the same build as
+    ref_strucf."""
+    global _ref
+    if _ref is None:
+        _ref = C.CDLL(_REF_PATH, mode=os.RTLD_LAZY)
+        _ref.ref_strucf.restype = None
+    _ref.ref_fourier_lmn.restype = C.c_int
+    e = np.asfortranarray(np.asarray(eck, np.float64).reshape(3, 4))
+    v = np.asfortranarray(np.asarray(vi, np.float64).reshape(3, 3))
+    lmn = np.zeros(6, np.int32)
+    off = np.zeros((3, 3), order="F")
+    st = _ref.ref_fourier_lmn(e.ctypes.data_as(_dp), v.ctypes.data_as(_dp), _i(_i32(inc, 3)),
+                              lmn.ctypes.data_as(C.POINTER(C.c_int)), off.ctypes.data_as(_dp))
+    return int(st), lmn, off
 
 
 def test_friedel(eck, vi, inc, ano=False, four_friedel=True):

@@ -88,3 +88,34 @@ contains
       tcsf_c = tcsf
    end subroutine ref_strucf
 end module ref_strucf_bind
+
+!*******************************************************************************
+! ref_fourier_lmn -- DISCUS's fourier_lmn (discus/prog/fourier_lmn_mod.f90:4-279):
+! the integer corner multiples lmn and the offset vectors off of a grid
+! (eck(3,4), vi(3,3), inc(3)).  Returns DISCUS's ier_num (0 = success).
+!*******************************************************************************
+module ref_lmn_bind
+   use iso_c_binding
+   implicit none
+contains
+   integer(c_int) function ref_fourier_lmn(eck_c, vi_c, inc_c, lmn_c, off_c) bind(C, name="ref_fourier_lmn")
+      use fourier_lmn_mod
+      use errlist_mod
+      real(c_double), intent(in) :: eck_c(3, 4), vi_c(3, 3)
+      integer(c_int), intent(in) :: inc_c(3)
+      integer(c_int), intent(out) :: lmn_c(6)
+      real(c_double), intent(out) :: off_c(3, 3)
+      integer :: inc(3), lmn(6)
+      real(kind=8) :: eck(3, 4), vi(3, 3), off(3, 3)
+      eck = eck_c
+      vi = vi_c
+      inc = inc_c
+      ier_num = 0
+      lmn = 0
+      off = 0.0d0
+      call fourier_lmn(eck, vi, inc, lmn, off)
+      lmn_c = lmn
+      off_c = off
+      ref_fourier_lmn = ier_num
+   end function ref_fourier_lmn
+end module ref_lmn_bind

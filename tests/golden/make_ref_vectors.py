@@ -1,4 +1,6 @@
-"""Writes ref_strucf_vectors.npz: input/output vectors of DISCUS's own
+"""Writes ref_strucf_vectors.npz and ref_lmn_vectors.json.
+
+ref_strucf_vectors.npz: input/output vectors of DISCUS's own
 four_strucf (discus/prog/four_strucf_OMP_mod.f90:19-68 with its OpenMP,
 serial and 1-D kernels and tcsf_form_iso, fourier_form.f90:147-177), run from
 oracle/_ref/libref_strucf.so, which oracle/build_ref.py compiles from the
@@ -11,9 +13,15 @@ Friedel-reduced block (fnum < num), a 1-D line (the 1-D kernel), a form
 factor through istl, atoms far from the origin and at negative coordinates
 (the NINT / INT arithmetic of the increments), and two thread counts.
 
+ref_lmn_vectors.json: DISCUS's own fourier_lmn (discus/prog/fourier_lmn_mod.f90:4-279,
+same build): the corner multiples lmn, the offset vectors off and ier_num of
+layers, volumes, lines, points, oblique layers and a grid with a zero
+increment.
+
 Run from the repository root after `python oracle/build_ref.py`:
     python tests/golden/make_ref_vectors.py
 """
+import json
 import os
 import sys
 
@@ -67,6 +75,39 @@ def cases():
     return out
 
 
+def lmn_grids():
+    """Grids for DISCUS's fourier_lmn: layers, volumes, lines along and off the
+    axes, single points, grids through hkl = 0 and away from it, oblique
+    increment vectors, and a zero increment (ier_num -132)."""
+    rng = np.random.default_rng(5)
+    out = []
+    for lo, hi, n in [((-5, -5, 0), (5, 5, 0), (101, 101, 1)), ((-4, -4, -4), (4, 4, 4), (161, 161, 161)),
+                      ((-10, -10, 0.5), (10, 10, 0.5), (401, 401, 1)), ((0, 0, 1.5), (3, 2, 1.5), (31, 21, 1)),
+                      ((-2, -3, 0.25), (2, 3, 3.75), (41, 61, 15)), ((-3, 0.25, 0.5), (3, 0.25, 0.5), (301, 1, 1)),
+                      ((0.3, -2, 1.1), (0.3, 2, 1.1), (1, 81, 1)), ((0.5, 0.5, 0.5), (0.5, 0.5, 0.5), (1, 1, 1)),
+                      ((1, 2, 3), (1, 2, 3), (1, 1, 1)), ((-1.5, -1.5, -1.5), (1.5, 1.5, 1.5), (7, 7, 7))]:
+        eck, vi, inc = grid(lo, hi, n)
+        out.append((eck, vi, inc))
+    for _ in range(6):
+        # oblique 2-D layers: eck(:,1) + i u + j v with general u, v
+        u = rng.normal(0, 0.05, 3)
+        v = rng.normal(0, 0.05, 3)
+        c = rng.normal(0, 2.0, 3)
+        n = [int(rng.integers(5, 40)), int(rng.integers(5, 40)), 1]
+        eck = np.zeros((3, 4))
+        eck[:, 0] = c
+        eck[:, 1] = c + (n[0] - 1) * u
+        eck[:, 2] = c + (n[1] - 1) * v
+        eck[:, 3] = c
+        vi = np.stack([u, v, np.zeros(3)], axis=1)
+        out.append((eck, vi, n))
+    eck, vi, inc = grid((-1, -1, 0), (1, 1, 0), (11, 11, 1))
+    vi = vi.copy()
+    vi[:, 1] = 0.0          # zero increment along an axis with 11 points
+    out.append((eck, vi, inc))
+    return out
+
+
 def main():
     if not O.ref_available():
         raise SystemExit("oracle/_ref/libref_strucf.so missing: run python oracle/build_ref.py")
@@ -104,6 +145,15 @@ def main():
     data["cases"] = np.array(names)
     np.savez_compressed(os.path.join(HERE, "ref_strucf_vectors.npz"), **data)
     print("wrote", len(names), "cases")
+    lmn_cases = []
+    for eck, vi, inc in lmn_grids():
+        st, lmn, off = O.ref_fourier_lmn(eck, vi, inc)
+        lmn_cases.append({"eck": np.asarray(eck).tolist(), "vi": np.asarray(vi).tolist(), "inc": [int(v) for v in inc],
+                          "ier_num": st, "lmn": [int(v) for v in lmn], "off": np.asarray(off).tolist()})
+    with open(os.path.join(HERE, "ref_lmn_vectors.json"), "w") as f:
+        json.dump({"source": "DISCUS fourier_lmn (discus/prog/fourier_lmn_mod.f90:4-279), compiled from the "
+                             "reference sources by oracle/build_ref.py", "cases": lmn_cases}, f, indent=1)
+    print("wrote", len(lmn_cases), "fourier_lmn cases")
 
 
 if __name__ == "__main__":

@@ -97,3 +97,49 @@ def test_oracle_turbo_matches_reference_on_random_crystals():
         a = O.strucf_turbo(pos, u, v, w, off, lmn, num)
         b = O.ref_strucf(pos, u, v, w, off, lmn, num, nthreads=4)
         assert np.array_equal(a, b), trial
+
+
+LMN_FIX = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_lmn_vectors.json")
+
+
+def _lmn_cases():
+    import json
+    return json.load(open(LMN_FIX))["cases"]
+
+
+@pytest.mark.parametrize("case", _lmn_cases(), ids=[f"lmn{i}" for i in range(len(_lmn_cases()))])
+def test_oracle_fourier_lmn_matches_discus_vectors(case):
+    """The oracle's fourier_lmn (in __float128) gives DISCUS's ier_num and
+    integer corner multiples exactly, and its offset vectors to a few units in
+    the last place (tests/golden/ref_lmn_vectors.json, from DISCUS's own
+    fourier_lmn, which works in REAL(16) through matrix_mod's inverse: the two
+    quad-precision evaluation orders differ in the last bits of some oblique
+    offsets, up to 1e-14 of the largest offset, far below what moves an
+    integer phase increment)."""
+    st, lmn, off = O.fourier_lmn(np.array(case["eck"]), np.array(case["vi"]), case["inc"])
+    assert st == case["ier_num"]
+    if st == 0:
+        assert [int(v) for v in lmn] == case["lmn"]
+        ref = np.array(case["off"])
+        np.testing.assert_allclose(np.asarray(off), ref, rtol=1e-14, atol=1e-14 * np.abs(ref).max())
+
+
+@needs_ref
+def test_oracle_fourier_lmn_matches_reference_on_random_layers():
+    rng = np.random.default_rng(23)
+    for trial in range(40):
+        u, v = rng.normal(0, 0.08, 3), rng.normal(0, 0.08, 3)
+        c = rng.normal(0, 3.0, 3)
+        n = [int(rng.integers(2, 60)), int(rng.integers(1, 60)), 1]
+        eck = np.zeros((3, 4))
+        eck[:, 0] = c
+        eck[:, 1] = c + (n[0] - 1) * u
+        eck[:, 2] = c + (n[1] - 1) * v
+        eck[:, 3] = c
+        vi = np.stack([u, v, np.zeros(3)], axis=1)
+        a = O.fourier_lmn(eck, vi, n)
+        b = O.ref_fourier_lmn(eck, vi, n)
+        assert a[0] == b[0], trial
+        if a[0] == 0:
+            assert np.array_equal(np.asarray(a[1]), b[1]), (trial, a[1], b[1])
+            np.testing.assert_allclose(np.asarray(a[2]), b[2], rtol=1e-14, atol=1e-14 * np.abs(b[2]).max())
