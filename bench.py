@@ -123,7 +123,9 @@ def bench_lots(_lib, X, FR, torch, prec):
     ctx.close()
     # bounded sample for the check in the CPU leg: layer l = 0, every 10th point (h, k step 0.5)
     dsi = bufs["dsi"].cpu().numpy().reshape(inc, order="F")
-    LOTS_CHECK.update(sample=dsi[::10, ::10, 0].copy(), crystal=c, lbeg=lbeg, ls=ls)
+    dvar = bufs["dvar"].cpu().numpy().reshape(inc, order="F")
+    LOTS_CHECK.update(sample=dsi[::10, ::10, 0].copy(), sample_dvar=dvar[::10, ::10, 0].copy(), crystal=c, lbeg=lbeg,
+                      ls=ls)
     ms = float(np.median([s["total_ms"] for s in stats]))
     gemm = float(np.median([s["gemm_ms"] for s in stats]))
     red = float(np.median([s["reduce_ms"] for s in stats]))
@@ -476,10 +478,15 @@ def main():
             if LOTS_CHECK and "lots" in extras:
                 cl = LOTS_CHECK["crystal"]
                 e3, v3, i3 = FR.grid_block((-10.0, -10.0, 0.0), (10.0, 10.0, 0.0), (41, 41, 1))
-                ref = O.four_run(cl, e3, v3, i3, FR.formtab(cl), ilots=O.LOT_BOX, ls=LOTS_CHECK["ls"],
-                                 lbeg=LOTS_CHECK["lbeg"], lperiod=True)["dsi"][:, :, 0]
+                lr = O.four_run(cl, e3, v3, i3, FR.formtab(cl), ilots=O.LOT_BOX, ls=LOTS_CHECK["ls"],
+                                lbeg=LOTS_CHECK["lbeg"], lperiod=True, want_var=True)
+                ref, refv = lr["dsi"][:, :, 0], lr["dvar"][:, :, 0]
                 extras["lots"]["max_rel_err_sampled"] = float(np.abs(LOTS_CHECK["sample"] - ref).max() / ref.max())
-                log(f"lots sampled max|dI|/max(I): {extras['lots']['max_rel_err_sampled']:.3e}")
+                # the FP64 cross-lot variance <|F|^2> - |<F>|^2 of the same run
+                extras["lots"]["dvar_max_rel_err_sampled"] = float(np.abs(LOTS_CHECK["sample_dvar"] - refv).max()
+                                                                   / np.abs(refv).max())
+                log(f"lots sampled max|dI|/max(I): {extras['lots']['max_rel_err_sampled']:.3e}, "
+                    f"dvar {extras['lots']['dvar_max_rel_err_sampled']:.3e}")
             if nufft_samples is not None:
                 extras["nufft"]["max_rel_err_sampled"] = sampled_err(nufft_samples)
                 log(f"nufft sampled max|dI|/max(I): {extras['nufft']['max_rel_err_sampled']:.3e}")
