@@ -33,6 +33,15 @@ constexpr int KCHUNK_MAX = 16384;
 // atoms in phase at the Bragg peaks: FP16X3 4.0e-7 over 4,096-atom slices,
 // 4.9e-5 without the rotation).
 constexpr int KCHUNK_FP32 = 1024;
+// Longest slice the planner's search (kchunk_for) may pick for the split and
+// FP64-sum schemes: the headline 161^3 block's 85,334-atom type segments
+// then split 5 ways (15 slices x 51 tiles = 765 items, 3 rounds of 256 CUs
+// with 3 idle slots) instead of 10 ways (30 x 51 = 1,530 items, 6 rounds):
+// half the planes to write and reduce (reduction 0.122 -> 0.061 ms), the
+// same GEMM time, and on the coherent sums of the shifted crystal FP16X3
+// 8.7e-7, BF16X3 3.3e-6, FP16X2 1.3e-6 at 17,067 atoms against 1.7e-6,
+// 5.0e-6, 7.5e-7 at 8,534 (max|dI|/max(I) vs FP64).
+constexpr int KCHUNK_SEARCH_MAX = 32768;
 constexpr double PARTIAL_BUDGET = 6.0e9;   // bytes of slice planes per batch
 
 }  // namespace dc
@@ -220,7 +229,7 @@ int kchunk_for(const Plan &pl, long long natoms_batch, int prec) {
 // an item's set-up, and W the items weighted by (length + 64) / (L + 64):
 // exact when the slices are equal, close to the fluid limit when short
 // slices fill the gaps.  The candidates are the lengths in [KCHUNK_MIN,
-// KCHUNK_MAX] at which some segment's slice count changes; the shortest
+// KCHUNK_SEARCH_MAX] at which some segment's slice count changes; the shortest
 // predicted time wins, ties going to the longer slices (fewer planes to
 // reduce).  FP32 searches between half its fixed slice cap and the cap.
 // Launches that already have 16 rounds of items or more, and
@@ -233,7 +242,7 @@ int kchunk_for(const Plan &pl, const std::vector<std::pair<int, int>> &segs, int
     // FP32 caps its slices (accumulator rounding on coherent sums); shorter
     // slices are allowed, down to half the cap
     const bool capped = prec == DC_PREC_FP32;
-    const int kmax = capped ? k0 : KCHUNK_MAX;
+    const int kmax = capped ? k0 : KCHUNK_SEARCH_MAX;
     const int kmin = capped ? k0 / 2 : KCHUNK_MIN;
     const TileInfo ti = gemm_tile_info(pl.tile);
     const long long wgs = (long long)((pl.gg.R + ti.rt - 1) / ti.rt) * ((pl.gg.C + ti.ct - 1) / ti.ct);
@@ -309,7 +318,9 @@ void add_segments(SliceSet &s, const std::vector<std::pair<int, int>> &segs, int
 int run_gemm(dc_ctx *c, const Plan &pl, int nslice, float2 *partial, const float *sq) {
     const GemmGeom gg = launch_geom(c, pl.gg);
     if (nslice <= 0) return 0;
-    HIPCHK(hipEventRecord(c->ev[2], c->st));
+    int st = 0;
+    hipEvent_t e0, e1;
+    if ((st = mark(c, e0))) return st;
     // launch in chunks of slices so that the cancel flag is polled
     const int per = 4096;
     for (int s0 = 0; s0 < nslice; s0 += per) {
@@ -320,11 +331,8 @@ int run_gemm(dc_ctx *c, const Plan &pl, int nslice, float2 *partial, const float
                                  c->st));
         c->stats.gemm_launches++;
     }
-    HIPCHK(hipEventRecord(c->ev[3], c->st));
-    HIPCHK(hipEventSynchronize(c->ev[3]));
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
-    c->stats.gemm_ms += ms;
+    if ((st = mark(c, e1))) return st;
+    span(c, e0, e1, &c->stats.gemm_ms);
     return 0;
 }
 
@@ -405,6 +413,7 @@ void dc_destroy(dc_ctx *c) {
         if (b->p) (void)hipFree(b->p);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto &e : c->tev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->st);
     delete c;
 }
@@ -478,8 +487,12 @@ int dc_device_info(const dc_ctx *c, char *name, int len, int *num_cu) {
     return DC_OK;
 }
 
-int dc_last_stats(const dc_ctx *c, dc_stats *out) {
-    if (!c || !out) return DC_ERR_ARG;
+int dc_last_stats(const dc_ctx *cc, dc_stats *out) {
+    if (!cc || !out) return DC_ERR_ARG;
+    // timings still in flight are part of the last call's statistics
+    dc_ctx *c = const_cast<dc_ctx *>(cc);
+    int st = resolve_spans(c);
+    if (st) return st;
     *out = c->stats;
     return DC_OK;
 }
@@ -606,7 +619,7 @@ int dc_strucf(dc_ctx *c, const double *xat, int64_t nxat, int64_t ld_xat, const 
                 }
     }
     HIPCHK(hipSetDevice(c->device));
-    std::memset(&c->stats, 0, sizeof(c->stats));
+    stats_begin(c);
     HIPCHK(hipEventRecord(c->ev[0], c->st));
     // positions: xat(ld,3) columns packed into (n,3) Fortran layout on device
     if ((st = ensure(c, c->xat, sizeof(double) * 3 * std::max<int64_t>(nxat, 1)))) return st;
@@ -816,9 +829,13 @@ long long lot_type_runs(const dc_ctx *c, const std::vector<CellRun> &cruns, int 
 
 // Run the phase GEMM over gathered segments and leave the slice planes in
 // c->partial.  Positions come from the resident crystal (idx/oid/offs) or,
-// when explicit != null, from an explicit (n,3) list.
+// when explicit != null, from an explicit (n,3) list.  group_seg (ascending
+// segment indices, the last one = gl.segs.size()) marks groups of segments
+// (the lots of a batch): their slice boundaries go to c->lot_slice, uploaded
+// with the slice tables before the launch.
 int gemm_segments(dc_ctx *c, const Grid &g, const Plan &pl, GatherList &gl,
-                  const double *explicit_pos, SliceSet &ss, int &nslice, bool use_type_lists) {
+                  const double *explicit_pos, SliceSet &ss, int &nslice, bool use_type_lists,
+                  const std::vector<int> &group_seg) {
     const GemmGeom &gg = pl.gg;
     const int mapping = pl.mapping;
     int st = 0;
@@ -872,6 +889,10 @@ int gemm_segments(dc_ctx *c, const Grid &g, const Plan &pl, GatherList &gl,
     nslice = ngemm * pps;
     if ((st = upload(c, c->slice_beg, ss.beg.data(), ss.beg.size()))) return st;
     if ((st = upload(c, c->slice_type, ss.type.data(), std::max<size_t>(ss.type.size(), 1)))) return st;
+    ss.seg_first.push_back(nslice);
+    std::vector<int> gsl;
+    for (int v : group_seg) gsl.push_back(ss.seg_first[v]);
+    if ((st = upload(c, c->lot_slice, gsl.data(), gsl.size()))) return st;
     if ((st = ensure(c, c->partial, sizeof(float2) * gg.P * std::max(nslice, 1)))) return st;
     return run_gemm(c, pl, ngemm, ptr<float2>(c->partial));
 }
@@ -910,7 +931,7 @@ int dc_four_run_dev(dc_ctx *c, const dc_run_desc *r, double *d_csf, double *d_ac
     if (r->fave != 0.0 && !r->sel_cell) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "dc_four_run: fave needs sel_cell");
     if (cancelled(c)) return set_err(c, DC_ERR_CANCEL, DC_ER_COMM, "cancelled");
     HIPCHK(hipSetDevice(c->device));
-    std::memset(&c->stats, 0, sizeof(c->stats));
+    stats_begin(c);
     HIPCHK(hipEventRecord(c->ev[0], c->st));
 
     Grid g{};
@@ -953,19 +974,17 @@ int dc_four_run_dev(dc_ctx *c, const dc_run_desc *r, double *d_csf, double *d_ac
     int st = 0;
     if ((st = upload(c, c->cfact, r->cfact, 2 * (size_t)r->ncfact * c->nscat))) return st;
 
-    // compact-region state
+    // compact-region state: every element is written before it is read (<F>
+    // by its reduction, dsi and the moments by the first lot batch's
+    // reduction, which starts them from zero), so nothing is cleared
     if ((st = ensure(c, c->acsf, sizeof(double) * 2 * g.P))) return st;
     if ((st = ensure(c, c->dsi, sizeof(double) * g.P))) return st;
     if ((st = ensure(c, c->csf, sizeof(double) * 2 * g.P))) return st;
     if (want_var) {
         if ((st = ensure(c, c->wm, sizeof(double) * 2 * g.P))) return st;
         if ((st = ensure(c, c->wv, sizeof(double) * g.P))) return st;
-        HIPCHK(hipMemsetAsync(c->wm.p, 0, sizeof(double) * 2 * g.P, c->st));
-        HIPCHK(hipMemsetAsync(c->wv.p, 0, sizeof(double) * g.P, c->st));
     }
-    HIPCHK(hipMemsetAsync(c->acsf.p, 0, sizeof(double) * 2 * g.P, c->st));
-    HIPCHK(hipMemsetAsync(c->dsi.p, 0, sizeof(double) * g.P, c->st));
-    HIPCHK(hipMemsetAsync(c->csf.p, 0, sizeof(double) * 2 * g.P, c->st));
+    const double *acsf = r->fave != 0.0 ? ptr<double>(c->acsf) : nullptr;   // null: <F> = 0
 
     double pairs = 0.0;
     long long nslices_total = 0;
@@ -1000,14 +1019,12 @@ int dc_four_run_dev(dc_ctx *c, const dc_run_desc *r, double *d_csf, double *d_ac
         }
         SliceSet ss;
         int nslice = 0;
-        if ((st = gemm_segments(c, g, pl, gl, nullptr, ss, nslice, false))) return st;
+        if ((st = gemm_segments(c, g, pl, gl, nullptr, ss, nslice, false, {0, (int)gl.segs.size()}))) return st;
         pairs += (double)gl.idx.size() * (double)ntot;
         nslices_total += nslice;
         ReduceArgs ra{};
         ra.g = g;
         ra.partial = ptr<float2>(c->partial);
-        std::vector<int> lot_b = {0, nslice};
-        if ((st = upload(c, c->lot_slice, lot_b.data(), 2))) return st;
         ra.lot_slice = ptr<int>(c->lot_slice);
         ra.slice_type = ptr<int>(c->slice_type);
         ra.nlots = 1;
@@ -1049,15 +1066,13 @@ int dc_four_run_dev(dc_ctx *c, const dc_run_desc *r, double *d_csf, double *d_ac
         const Plan pll = plan_gemm(gl_full, c->tile, -1, scheme, (long long)(lp.size() / 3));
         GatherList gll;
         gll.segs.push_back({0, (int)(lp.size() / 3)});
-        if ((st = gemm_segments(c, gl_full, pll, gll, lp.data(), ss, nslice, false))) return st;
+        if ((st = gemm_segments(c, gl_full, pll, gll, lp.data(), ss, nslice, false, {0, 1}))) return st;
         pairs += (double)(lp.size() / 3) * (double)ntot;
         nslices_total += nslice;
         if ((st = ensure(c, c->lat, sizeof(double) * 2 * ntot))) return st;
         ReduceArgs rl{};
         rl.g = gl_full;
         rl.partial = ptr<float2>(c->partial);
-        lot_b = {0, nslice};
-        if ((st = upload(c, c->lot_slice, lot_b.data(), 2))) return st;
         rl.lot_slice = ptr<int>(c->lot_slice);
         rl.nlots = 1;
         rl.lform = 0;
@@ -1113,13 +1128,9 @@ int dc_four_run_dev(dc_ctx *c, const dc_run_desc *r, double *d_csf, double *d_ac
         lot_first_seg.push_back((int)gl.segs.size());
         SliceSet ss;
         int nslice = 0;
-        if ((st = gemm_segments(c, g, pl, gl, nullptr, ss, nslice, !lots))) return st;
+        if ((st = gemm_segments(c, g, pl, gl, nullptr, ss, nslice, !lots, lot_first_seg))) return st;
         pairs += (double)n_batch * (double)ntot;
         nslices_total += nslice;
-        ss.seg_first.push_back(nslice);
-        std::vector<int> lot_sl;
-        for (int v : lot_first_seg) lot_sl.push_back(ss.seg_first[v]);
-        if ((st = upload(c, c->lot_slice, lot_sl.data(), lot_sl.size()))) return st;
         ReduceArgs ra{};
         ra.g = g;
         ra.partial = ptr<float2>(c->partial);
@@ -1131,18 +1142,17 @@ int dc_four_run_dev(dc_ctx *c, const dc_run_desc *r, double *d_csf, double *d_ac
         ra.lform = 1;
         ra.cfact = ptr<double2>(c->cfact);
         ra.ncfact = r->ncfact;
-        ra.acsf = ptr<double>(c->acsf);
+        ra.acsf = acsf;
         ra.dsi = ptr<double>(c->dsi);
         ra.csf = ptr<double>(c->csf);
         ra.wm = want_var ? ptr<double>(c->wm) : nullptr;
         ra.wv = want_var ? ptr<double>(c->wv) : nullptr;
-        HIPCHK(hipEventRecord(c->ev[2], c->st));
+        ra.fresh = lot == 0;
+        hipEvent_t r0, r1;
+        if ((st = mark(c, r0))) return st;
         HIPCHK(launch_reduce_lots(ra, c->st));
-        HIPCHK(hipEventRecord(c->ev[3], c->st));
-        HIPCHK(hipEventSynchronize(c->ev[3]));
-        float rms = 0.f;
-        HIPCHK(hipEventElapsedTime(&rms, c->ev[2], c->ev[3]));
-        c->stats.reduce_ms += rms;
+        if ((st = mark(c, r1))) return st;
+        span(c, r0, r1, &c->stats.reduce_ms);
         lot = l1;
     }
 
@@ -1152,7 +1162,7 @@ int dc_four_run_dev(dc_ctx *c, const dc_run_desc *r, double *d_csf, double *d_ac
     fa.g = g;
     fa.ntot = ntot;
     fa.csf = ptr<double>(c->csf);
-    fa.acsf = ptr<double>(c->acsf);
+    fa.acsf = acsf;
     fa.dsi = ptr<double>(c->dsi);
     fa.wm = want_var ? ptr<double>(c->wm) : nullptr;
     fa.wv = want_var ? ptr<double>(c->wv) : nullptr;
@@ -1170,6 +1180,7 @@ int dc_four_run_dev(dc_ctx *c, const dc_run_desc *r, double *d_csf, double *d_ac
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
     c->stats.total_ms = ms;
+    if ((st = resolve_spans(c))) return st;
     c->stats.pairs = pairs;
     c->stats.slices = nslices_total;
     c->stats.gemm_scheme = scheme;

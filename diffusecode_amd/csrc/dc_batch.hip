@@ -429,7 +429,7 @@ int batch_chi2(dc_ctx *c, const dc_batch_desc *d, const double *target, const do
     if (!target || !chi2) return set_err(c, DC_ERR_ARG, DC_ER_LIB, "batch: null target or chi2");
     if (cancelled(c)) return set_err(c, DC_ERR_CANCEL, DC_ER_COMM, "cancelled");
     HIPCHK(hipSetDevice(c->device));
-    std::memset(&c->stats, 0, sizeof(c->stats));
+    stats_begin(c);
     const int64_t natoms = d->atom_beg[d->nmembers];
     const hipMemcpyKind in = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     const hipMemcpyKind out = dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;

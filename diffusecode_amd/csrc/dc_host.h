@@ -35,6 +35,16 @@ struct dc_ctx {
     const volatile int *cancel = nullptr;
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     dc_stats stats{};
+    // timed spans of the current call whose events the host has not waited
+    // for: (start, end, stats field the elapsed ms are added to); tev is the
+    // pool their events come from (tev_used in use)
+    struct Span {
+        hipEvent_t a, b;
+        double *dst;
+    };
+    std::vector<Span> spans;
+    std::vector<hipEvent_t> tev;
+    size_t tev_used = 0;
     // device buffers of the Fourier run
     DevBuf rec, slice_beg, slice_type, lot_slice, partial, cfact, istl, idx, oid, offs, xat, w;
     DevBuf acsf, dsi, csf, wm, wv, lat, out_tcsf, out[5];
@@ -121,6 +131,41 @@ int upload(dc_ctx *c, DevBuf &b, const T *src, size_t count) {
     int st = ensure(c, b, sizeof(T) * count);
     if (st) return st;
     if (count) HIPCHK(hipMemcpyAsync(b.p, src, sizeof(T) * count, hipMemcpyHostToDevice, c->st));
+    return 0;
+}
+
+// Deferred timing: mark() records a pool event on the context stream;
+// span() files (start, end) to be added to a stats field once the host has
+// waited for the stream, so that timing a kernel does not stall the host
+// between launches.  stats_begin() starts a call's statistics (dropping the
+// spans of an earlier call), resolve_spans() waits for the last span and
+// adds every span's elapsed time.
+inline int mark(dc_ctx *c, hipEvent_t &e) {
+    if (c->tev_used == c->tev.size()) {
+        hipEvent_t n = nullptr;
+        HIPCHK(hipEventCreate(&n));
+        c->tev.push_back(n);
+    }
+    e = c->tev[c->tev_used++];
+    HIPCHK(hipEventRecord(e, c->st));
+    return 0;
+}
+inline void span(dc_ctx *c, hipEvent_t a, hipEvent_t b, double *dst) { c->spans.push_back({a, b, dst}); }
+inline void stats_begin(dc_ctx *c) {
+    c->stats = dc_stats{};
+    c->spans.clear();
+    c->tev_used = 0;
+}
+inline int resolve_spans(dc_ctx *c) {
+    if (c->spans.empty()) return 0;
+    HIPCHK(hipEventSynchronize(c->spans.back().b));
+    for (const auto &s : c->spans) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, s.a, s.b));
+        *s.dst += ms;
+    }
+    c->spans.clear();
+    c->tev_used = 0;
     return 0;
 }
 

@@ -94,6 +94,7 @@ struct ReduceArgs {
     const int *istl;        // optional bins (full-grid layout), null = compute
     const double *acsf;     // compact region, null = zero
     double *dsi, *csf, *wm, *wv;
+    int fresh;              // dsi, wm, wv start from zero (not read)
 };
 
 struct FinalArgs {

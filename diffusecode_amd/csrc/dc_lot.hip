@@ -99,15 +99,17 @@ __device__ __forceinline__ int pixel_bin(const Grid &g, const int *istl, long lo
 // the batch, F_l(p) = sum_{slices s of lot l} cfact(type_s, bin p) * partial_s(p)
 // (FP64), then (four_run, fourier_sup.f90:224-254)
 //   csf = F_l - acsf ; dsi += |csf|^2 ; Welford moments of F_l.
+// fresh: the first batch, whose dsi and moments start from zero.
 __global__ void reduce_lots_kernel(ReduceArgs a) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.g.P) return;
     const int bin = a.lform ? pixel_bin(a.g, a.istl, p, a.ncfact) : 0;
     double ar = 0.0, ai = 0.0;
     if (a.acsf) { ar = a.acsf[2 * p]; ai = a.acsf[2 * p + 1]; }
-    double dsi = a.dsi[p];
-    double mr = a.wm ? a.wm[2 * p] : 0.0, mi = a.wm ? a.wm[2 * p + 1] : 0.0;
-    double m2 = a.wv ? a.wv[p] : 0.0;
+    const bool rd = !a.fresh;
+    double dsi = rd ? a.dsi[p] : 0.0;
+    double mr = a.wm && rd ? a.wm[2 * p] : 0.0, mi = a.wm && rd ? a.wm[2 * p + 1] : 0.0;
+    double m2 = a.wv && rd ? a.wv[p] : 0.0;
     double cr = 0.0, ci = 0.0;
     const long long po = plane_off(a.g, p);
     int s = a.lot_slice[0];

@@ -87,7 +87,7 @@ int dc_strucf_dev_noalloc(dc_ctx *c, const double *d_xat, int64_t nxat, int64_t 
         return set_err(c, DC_ERR_ARG, DC_ER_LIB, msg);
     }
     unsigned char *base = static_cast<unsigned char *>(c->scratch);
-    std::memset(&c->stats, 0, sizeof(c->stats));
+    stats_begin(c);
     HIPCHK(hipEventRecord(c->ev[0], c->st));
     st = strucf_core(c, g, ntot, d_xat, nxat, std::max<int64_t>(ld_xat, 1), off_shift, lmn,
                      lform ? reinterpret_cast<const double2 *>(d_cfact_col) : nullptr, ncfact, lform ? d_istl : nullptr,

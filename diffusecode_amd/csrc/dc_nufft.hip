@@ -1040,7 +1040,7 @@ int dc_four_run_nufft_dev(dc_ctx *c, const dc_run_desc *r, const double *type_oc
     if (max_corner_bin(g) > std::min(CFPKT, r->ncfact - 1))
         return set_err(c, DC_ERR_ISTL, DC_ER_FOUR, "sin(theta)/lambda beyond the form factor table");
     HIPCHK(hipSetDevice(c->device));
-    std::memset(&c->stats, 0, sizeof(c->stats));
+    stats_begin(c);
     HIPCHK(hipEventRecord(c->ev[0], c->st));
     NufftState &s = nu_state(c);
     const long long ntot = (long long)g.n[0] * g.n[1] * g.n[2];

@@ -154,7 +154,6 @@ int model_forward(dc_ctx *c, const dc_model_desc *m, ModelPlan &mp) {
     if ((st = ensure(c, c->dsi, sizeof(double) * mp.g.P))) return st;
     HIPCHK(hipEventRecord(c->ev[4], c->st));
     if ((st = run_gemm(c, mp.pl, nslice, ptr<float2>(c->partial), mp.any_b ? ptr<float>(c->m_sq) : nullptr))) return st;
-    HIPCHK(hipMemsetAsync(c->dsi.p, 0, sizeof(double) * mp.g.P, c->st));
     ReduceArgs ra{};
     ra.g = mp.g;
     ra.partial = ptr<float2>(c->partial);
@@ -167,10 +166,12 @@ int model_forward(dc_ctx *c, const dc_model_desc *m, ModelPlan &mp) {
     ra.ncfact = m->ncfact;
     ra.dsi = ptr<double>(c->dsi);
     ra.csf = ptr<double>(c->m_f);
+    ra.fresh = 1;
     if (nslice > 0) {
         HIPCHK(launch_reduce_lots(ra, c->st));
     } else {
         HIPCHK(hipMemsetAsync(c->m_f.p, 0, sizeof(double) * 2 * mp.g.P, c->st));
+        HIPCHK(hipMemsetAsync(c->dsi.p, 0, sizeof(double) * mp.g.P, c->st));
     }
     HIPCHK(hipEventRecord(c->ev[5], c->st));
     HIPCHK(hipEventSynchronize(c->ev[5]));
@@ -283,7 +284,7 @@ int model_begin(dc_ctx *c, const dc_model_desc *m, ModelPlan &mp) {
     if ((st = check_model(c, m))) return st;
     if (cancelled(c)) return set_err(c, DC_ERR_CANCEL, DC_ER_COMM, "cancelled");
     HIPCHK(hipSetDevice(c->device));
-    std::memset(&c->stats, 0, sizeof(c->stats));
+    stats_begin(c);
     HIPCHK(hipEventRecord(c->ev[0], c->st));
     if ((st = plan_model(c, m, mp))) return st;
     return upload_model(c, m, mp);
