@@ -203,27 +203,40 @@ __device__ __forceinline__ float2 cis(uint64_t ph) {
 // ---------------------------------------------------------------------------
 // Tables of one stage, built by all threads (pt = thread index).  Each atom
 // a = pt % 16 (index katom of the record array) has NPT/16 helper threads
-// r = pt / 16 (waves 0 and 1 build the low tables, 8 sincos per thread, the
+// r = pt / 16 (waves 0 and 1 build the low tables, 3 sincos per thread, the
 // other waves the high rows; spreading the same work evenly over all waves
 // measured 3-5% slower, 11.2 vs 10.7 ms on the headline):
 //   r < 8      low tables of side r/4, entries l = m, m+4, m+8, m+12
 //              (m = r % 4): TL0[l] = cis(l s + psi2), TL1[l] = cis(l s + w +
-//              psi2), each evaluated from its exact phase.  Each store of a
-//              wave then covers 4 consecutive rows (512 contiguous bytes);
-//              with rows 4m .. 4m+3 per thread its 4 row groups sat 512 bytes
-//              apart on the same banks (measured A/B: headline 10.75 -> 10.52
-//              ms, population GEMM 2.78 -> 2.63 ms, identical results)
+//              psi2).  Split schemes (TWO_LEVEL): helper m evaluates
+//              cis(m s + psi2), cis(4m s) and cis(4m s + w) from their exact
+//              phases; the four helpers of an atom and side (lanes a + 16 m
+//              of one wave) exchange the last two by lane shuffles, and
+//              TL_d[m + 4j] is the product cis(m s + psi2) * cis(4j s + d w):
+//              one FP32 rounding more, far below the split's 2^-22, in place
+//              of 8 sincos per thread (headline FP16X3 GEMM 10.70 -> 10.62
+//              ms, 4 interleaved A/B pairs; coherent sums of the shifted
+//              crystal 8.7e-7 -> 8.4e-7, BF16X3 3.3e-6 both).  The FP32
+//              schemes evaluate all 8 entries from their exact phases: their
+//              Bragg-subtracted runs hold 1e-6, which the extra rounding
+//              exceeds (1.1e-6 on the lots + <F> test).
+//              Each store of a wave covers 4 consecutive rows (512
+//              contiguous bytes); with rows 4m .. 4m+3 per thread its 4 row
+//              groups sat 512 bytes apart on the same banks (measured A/B:
+//              headline 10.75 -> 10.52 ms, population GEMM 2.78 -> 2.63 ms,
+//              identical results)
 //   r >= 8     high-table rows h = r-8, r-8+NPT/16-8, ... of [A (RH) | B (CH)],
 //              A rows rotated by psi - psi2 and B rows by -psi - psi2
 // so every A element carries +psi and every B element -psi (dither_turn):
 // the products are unchanged, their rounding errors differ per atom.
-template <class T>
+template <class T, bool TWO_LEVEL>
 __device__ __forceinline__ void build_tables(Tables<T> &tb, const int *__restrict__ a_k16,
                                              const int *__restrict__ b_k16, const AtomRec &ar, bool live,
                                              const Side &sa, const Side &sb, int pt, float wscale, uint32_t katom,
                                              uint32_t salt) {
     const int a = pt & (KS - 1), r = pt >> 4;
     constexpr int NR = T::NPT / KS;           // helper threads per atom
+    static_assert(NR > 8, "the low tables take waves 0 and 1, the high rows need more threads");
     const uint64_t psi = dither_turn(katom, salt);
     const uint64_t psi2 = psi * 0xD6E8FEB86659FD93ull;
     if (r == 0) tb.dmp[a] = live ? ar.pad * 1.4426950408889634f : 0.0f;
@@ -234,11 +247,27 @@ __device__ __forceinline__ void build_tables(Tables<T> &tb, const int *__restric
         const uint64_t wrp = sideb ? ar.ca1 - (uint64_t)sb.period * ar.ca : ar.ra1 - (uint64_t)sa.period * ar.ra;
         float2 *t0 = sideb ? tb.bl[0] : tb.al[0];
         float2 *t1 = sideb ? tb.bl[1] : tb.al[1];
+        if constexpr (TWO_LEVEL) {
+            const float2 base = cis((uint64_t)m * step + psi2);
+            const uint64_t hs = (uint64_t)(4 * m) * step;
+            const float2 g0 = cis(hs), g1 = cis(hs + wrp);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const uint64_t ph = (uint64_t)(m + 4 * j) * step + psi2;
-            t0[(m + 4 * j) * KS + a] = cis(ph);
-            t1[(m + 4 * j) * KS + a] = cis(ph + wrp);
+            for (int j = 0; j < 4; ++j) {
+                // helper j of this atom and side: lane a + 16 j of this wave
+                // (waves 0 and 1 are exactly the threads r < 8)
+                const int src = a + 16 * j;
+                const float2 h0 = make_float2(__shfl(g0.x, src), __shfl(g0.y, src));
+                const float2 h1 = make_float2(__shfl(g1.x, src), __shfl(g1.y, src));
+                t0[(m + 4 * j) * KS + a] = cmul(base, h0);
+                t1[(m + 4 * j) * KS + a] = cmul(base, h1);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint64_t ph = (uint64_t)(m + 4 * j) * step + psi2;
+                t0[(m + 4 * j) * KS + a] = cis(ph);
+                t1[(m + 4 * j) * KS + a] = cis(ph + wrp);
+            }
         }
     } else {
         const float w = live ? ar.w * wscale : 0.0f;   // wscale: power of two, exact
@@ -635,10 +664,10 @@ phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice
                 wunscale = ldexpf(1.0f, e);
             }
         }
-        build_tables<T>(sm.t[0], sm.a_k16, sm.b_k16, r0, kbeg + at < kend, sa, sb, pt, wscale, kbeg + at, g.dither);
+        build_tables<T, !fp32ops(SCHEME)>(sm.t[0], sm.a_k16, sm.b_k16, r0, kbeg + at < kend, sa, sb, pt, wscale, kbeg + at, g.dither);
         __syncthreads();
         build_elems<SCHEME, T, DAMP>(sm.e[0], sm.t[0], sm.a_wpos, sm.b_wpos, sm.s_a, sm.s_b, wave);
-        if (nst > 1) build_tables<T>(sm.t[1], sm.a_k16, sm.b_k16, r1, kbeg + KS + at < kend, sa, sb, pt, wscale, kbeg + KS + at, g.dither);
+        if (nst > 1) build_tables<T, !fp32ops(SCHEME)>(sm.t[1], sm.a_k16, sm.b_k16, r1, kbeg + KS + at < kend, sa, sb, pt, wscale, kbeg + KS + at, g.dither);
         __syncthreads();
         for (int s = 0; s < nst; ++s) {
             const int cur = s & 1, nxt = cur ^ 1;
@@ -663,7 +692,7 @@ phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice
             const bool mfma_first = split3(SCHEME) && T::WC > 4 && ((wave >> 2) & 1) != 0;
             if (mfma_first && live) mfma_stage<SCHEME, T>(sm.e[cur], accR, accI, wr0, wc0);
             if (s + 1 < nst) build_elems<SCHEME, T, DAMP>(sm.e[nxt], sm.t[nxt], sm.a_wpos, sm.b_wpos, sm.s_a, sm.s_b, wave);
-            if (s + 2 < nst) build_tables<T>(sm.t[cur], sm.a_k16, sm.b_k16, rt, kbeg + (s + 2) * KS + at < kend, sa, sb, pt, wscale,
+            if (s + 2 < nst) build_tables<T, !fp32ops(SCHEME)>(sm.t[cur], sm.a_k16, sm.b_k16, rt, kbeg + (s + 2) * KS + at < kend, sa, sb, pt, wscale,
                                               kbeg + (s + 2) * KS + at, g.dither);
             if (!mfma_first && live) {
                 if constexpr (D64)
