@@ -848,12 +848,24 @@ __global__ void batch_reduce_kernel(BatchReduceArgs a) {
         int bin = istl_point(g, i, j, l);
         bin = bin >= a.ncfact ? a.ncfact - 1 : bin;
         const long long po = i * g.ps[0] + j * g.ps[1] + l * g.ps[2];
+        // the member's slices come in runs of one type: sum a run's planes
+        // (four loads in flight), then apply that type's form factor once
         double fr = 0.0, fi = 0.0;
-        for (int s = s0; s < s1; ++s) {
-            const float2 v = a.partial[(long long)s * g.plane + po];
-            const double2 u = cf[(long long)a.slice_type[s] * a.ncfact + bin];
-            fr += u.x * (double)v.x - u.y * (double)v.y;
-            fi += u.x * (double)v.y + u.y * (double)v.x;
+        int s = s0;
+        while (s < s1) {
+            const int t = a.slice_type[s];
+            int e = s + 1;
+            while (e < s1 && a.slice_type[e] == t) ++e;
+            double sr = 0.0, si = 0.0;
+#pragma unroll 4
+            for (; s < e; ++s) {
+                const float2 v = a.partial[(long long)s * g.plane + po];
+                sr += (double)v.x;
+                si += (double)v.y;
+            }
+            const double2 u = cf[(long long)t * a.ncfact + bin];
+            fr += u.x * sr - u.y * si;
+            fi += u.x * si + u.y * sr;
         }
         const long long mp = (long long)m * P + p;
         const double I = fr * fr + fi * fi;

@@ -690,15 +690,27 @@ phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice
             // MFMA block costs registers.  So BF16X3 and FP16X3 on the
             // wide-column tiles.
             const bool mfma_first = split3(SCHEME) && T::WC > 4 && ((wave >> 2) & 1) != 0;
-            if (mfma_first && live) mfma_stage<SCHEME, T>(sm.e[cur], accR, accI, wr0, wc0);
+            // Split schemes raise the wave's issue priority while it issues
+            // its MFMAs, so that the SIMD's other wave, generating operands,
+            // does not delay them (population GEMM, 256 x 128 damped FP16X3:
+            // 2.665 -> 2.617 ms, 6 interleaved A/B pairs; headline unchanged)
+            constexpr bool PRIO = !fp32ops(SCHEME);
+            if (mfma_first && live) {
+                if constexpr (PRIO) __builtin_amdgcn_s_setprio(1);
+                mfma_stage<SCHEME, T>(sm.e[cur], accR, accI, wr0, wc0);
+                if constexpr (PRIO) __builtin_amdgcn_s_setprio(0);
+            }
             if (s + 1 < nst) build_elems<SCHEME, T, DAMP>(sm.e[nxt], sm.t[nxt], sm.a_wpos, sm.b_wpos, sm.s_a, sm.s_b, wave);
             if (s + 2 < nst) build_tables<T, !fp32ops(SCHEME)>(sm.t[cur], sm.a_k16, sm.b_k16, rt, kbeg + (s + 2) * KS + at < kend, sa, sb, pt, wscale,
                                               kbeg + (s + 2) * KS + at, g.dither);
             if (!mfma_first && live) {
-                if constexpr (D64)
+                if constexpr (D64) {
                     mfma_stage_d<T>(sm.e[cur], accR, accI, dR, dI, wr0, wc0);
-                else
+                } else {
+                    if constexpr (PRIO) __builtin_amdgcn_s_setprio(1);
                     mfma_stage<SCHEME, T>(sm.e[cur], accR, accI, wr0, wc0);
+                    if constexpr (PRIO) __builtin_amdgcn_s_setprio(0);
+                }
             }
             if constexpr (TWO) {
                 if ((s & 3) == 3 || s + 1 == nst) {
