@@ -685,7 +685,9 @@ phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice
             }
             // The three steps of a stage touch disjoint buffers (elements of
             // s+1 from tables of s+1, tables of s+2, MFMAs on elements of s),
-            // so their order is free.  Three-product schemes: waves 4..7 issue
+            // so their order is free; the tables come before the elements
+            // (headline FP16X3 GEMM -0.6%, population -1.3%, measured A/B,
+            // identical results).  Three-product schemes: waves 4..7 issue
             // the MFMAs first, so the two waves sharing a SIMD (w and w+4)
             // overlap one's operand generation with the other's matrix work
             // instead of both generating, then both multiplying.  Measured on
@@ -705,9 +707,9 @@ phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice
                 mfma_stage<SCHEME, T>(sm.e[cur], accR, accI, wr0, wc0);
                 if constexpr (PRIO) __builtin_amdgcn_s_setprio(0);
             }
-            if (s + 1 < nst) build_elems<SCHEME, T, DAMP>(sm.e[nxt], sm.t[nxt], sm.a_wpos, sm.b_wpos, sm.s_a, sm.s_b, wave);
             if (s + 2 < nst) build_tables<T, !fp32ops(SCHEME)>(sm.t[cur], sm.a_k16, sm.b_k16, rt, kbeg + (s + 2) * KS + at < kend, sa, sb, pt, wscale,
                                               kbeg + (s + 2) * KS + at, g.dither);
+            if (s + 1 < nst) build_elems<SCHEME, T, DAMP>(sm.e[nxt], sm.t[nxt], sm.a_wpos, sm.b_wpos, sm.s_a, sm.s_b, wave);
             if (!mfma_first && live) {
                 if constexpr (D64) {
                     mfma_stage_d<T>(sm.e[cur], accR, accI, dR, dI, wr0, wc0);
