@@ -33,6 +33,9 @@ struct BatchState {
     Plan pl;
     int nwg = 0;
     std::vector<int> order;
+    std::vector<std::pair<int, int>> segs;   // (type, atoms) per member and type, member-major
+    std::vector<int> seg_member;
+    int kchunk = 0;                          // atoms per GEMM slice
     int nsorted = 0, nslice = 0, ntiles = 0, ntc = 0, nblk = 0;
     int walk = 1, nset = 0;              // adjoint column tiles per work-group, partial sets
     int walk_req = 0;                    // the context's adjoint walk setting at planning
@@ -144,6 +147,42 @@ int member_sq(dc_ctx *c, const BatchState &b, const dc_batch_desc *d, std::vecto
     return 0;
 }
 
+// GEMM slices of at most kchunk atoms over the member-major, type-sorted
+// segments: slice tables, each member's slice range and metric offset, the
+// launch order and the plane buffer.
+int make_slices(dc_ctx *c, BatchState &b, int kchunk) {
+    const GemmGeom &gg = b.pl.gg;
+    SliceSet ss;
+    std::vector<int> mslice, slice_sq;
+    int off = 0;
+    size_t sg = 0;
+    for (int m = 0; m < b.nmembers; ++m) {
+        mslice.push_back((int)ss.type.size());
+        for (; sg < b.segs.size() && b.seg_member[sg] == m; ++sg) {
+            add_segments(ss, {b.segs[sg]}, kchunk, off);
+            while ((int)slice_sq.size() < (int)ss.type.size()) slice_sq.push_back(m * (gg.R + gg.C));
+        }
+    }
+    mslice.push_back((int)ss.type.size());
+    if (ss.beg.empty()) ss.beg.push_back(0);
+    b.nslice = (int)ss.type.size();
+    b.kchunk = kchunk;
+    int st;
+    if ((st = upload(c, b.sbeg, ss.beg.data(), ss.beg.size()))) return st;
+    if ((st = upload(c, b.stype, ss.type.data(), ss.type.size()))) return st;
+    if ((st = upload(c, b.ssq, slice_sq.data(), slice_sq.size()))) return st;
+    if ((st = upload(c, b.msl, mslice.data(), mslice.size()))) return st;
+    // GEMM launch order of the slices: longest first.  Members of uneven size
+    // give slices of uneven length, and in member order a long slice can
+    // start in the launch's last round of work-groups and run alone
+    std::vector<int> sorder(std::max(b.nslice, 1), 0);
+    for (int s = 0; s < b.nslice; ++s) sorder[s] = s;
+    std::stable_sort(sorder.begin(), sorder.begin() + b.nslice,
+                     [&](int x, int y) { return ss.beg[x + 1] - ss.beg[x] > ss.beg[y + 1] - ss.beg[y]; });
+    if ((st = upload(c, b.sorder, sorder.data(), sorder.size()))) return st;
+    return ensure(c, b.partial, sizeof(float2) * gg.P * std::max(b.nslice, 1));
+}
+
 int build_state(dc_ctx *c, const dc_batch_desc *d, BatchState &b) {
     b.nmembers = d->nmembers;
     b.nscat = d->nscat;
@@ -173,11 +212,12 @@ int build_state(dc_ctx *c, const dc_batch_desc *d, BatchState &b) {
     b.pl = plan_gemm(g, c->tile, 0, c->prec, d->atom_beg[d->nmembers]);
     const GemmGeom &gg = b.pl.gg;
 
-    // member-major, type-sorted atom order; slices; adjoint tiles
-    SliceSet ss;
-    std::vector<int> mslice, slice_sq, tiles;
-    std::vector<std::pair<int, int>> segs_all;
-    std::vector<int> seg_member;
+    // member-major, type-sorted atom order; adjoint tiles; slices
+    std::vector<int> tiles;
+    std::vector<std::pair<int, int>> &segs_all = b.segs;
+    std::vector<int> &seg_member = b.seg_member;
+    segs_all.clear();
+    seg_member.clear();
     b.order.clear();
     for (int m = 0; m < d->nmembers; ++m) {
         std::vector<int> cnt(d->nscat + 1, 0);
@@ -191,24 +231,15 @@ int build_state(dc_ctx *c, const dc_batch_desc *d, BatchState &b) {
         }
     }
     b.nsorted = (int)b.order.size();
-    const int kchunk = slice_len(c, kchunk_for(b.pl, b.nsorted, b.prec));
     int off = 0;
     const int na = adjoint_tile_atoms();
-    size_t sg = 0;
-    for (int m = 0; m < d->nmembers; ++m) {
-        mslice.push_back((int)ss.type.size());
-        for (; sg < segs_all.size() && seg_member[sg] == m; ++sg) {
-            const int a0 = off;
-            add_segments(ss, {segs_all[sg]}, kchunk, off);
-            while ((int)slice_sq.size() < (int)ss.type.size()) slice_sq.push_back(m * (gg.R + gg.C));
-            for (int k = 0; k < segs_all[sg].second; k += na)
-                tiles.insert(tiles.end(), {a0 + k, a0 + std::min(segs_all[sg].second, k + na),
-                                           m * d->nscat + segs_all[sg].first, m});
-        }
+    for (size_t sg = 0; sg < segs_all.size(); ++sg) {
+        const int m = seg_member[sg];
+        for (int k = 0; k < segs_all[sg].second; k += na)
+            tiles.insert(tiles.end(), {off + k, off + std::min(segs_all[sg].second, k + na),
+                                       m * d->nscat + segs_all[sg].first, m});
+        off += segs_all[sg].second;
     }
-    mslice.push_back((int)ss.type.size());
-    if (ss.beg.empty()) ss.beg.push_back(0);
-    b.nslice = (int)ss.type.size();
     b.ntiles = (int)tiles.size() / 4;
     b.ntc = (gg.C + adjoint_tile_cols() - 1) / adjoint_tile_cols();
     {
@@ -223,24 +254,12 @@ int build_state(dc_ctx *c, const dc_batch_desc *d, BatchState &b) {
     int st;
     const size_t P = (size_t)g.P, M = (size_t)d->nmembers, T = (size_t)std::max<int64_t>(b.natoms, 1);
     if ((st = upload(c, b.idx, b.order.data(), b.order.size()))) return st;
-    if ((st = upload(c, b.sbeg, ss.beg.data(), ss.beg.size()))) return st;
-    if ((st = upload(c, b.stype, ss.type.data(), ss.type.size()))) return st;
-    if ((st = upload(c, b.ssq, slice_sq.data(), slice_sq.size()))) return st;
-    if ((st = upload(c, b.msl, mslice.data(), mslice.size()))) return st;
-    // GEMM launch order of the slices: longest first.  Members of uneven size
-    // give slices of uneven length, and in member order a long slice can
-    // start in the launch's last round of work-groups and run alone
-    std::vector<int> sorder(std::max(b.nslice, 1), 0);
-    for (int s = 0; s < b.nslice; ++s) sorder[s] = s;
-    std::stable_sort(sorder.begin(), sorder.begin() + b.nslice,
-                     [&](int x, int y) { return ss.beg[x + 1] - ss.beg[x] > ss.beg[y + 1] - ss.beg[y]; });
-    if ((st = upload(c, b.sorder, sorder.data(), sorder.size()))) return st;
+    if ((st = make_slices(c, b, slice_len(c, kchunk_for(b.pl, b.nsorted, b.prec))))) return st;
     if ((st = upload(c, b.atiles, tiles.data(), tiles.size()))) return st;
     if ((st = ensure(c, b.pos, sizeof(double) * 3 * T))) return st;
     if ((st = ensure(c, b.occ, sizeof(double) * T))) return st;
     if ((st = ensure(c, b.bdw, sizeof(double) * T))) return st;
     if ((st = ensure(c, b.rec, sizeof(AtomRec) * std::max(b.nsorted, 1)))) return st;
-    if ((st = ensure(c, b.partial, sizeof(float2) * gg.P * std::max(b.nslice, 1)))) return st;
     if ((st = ensure(c, b.cfact, sizeof(double) * 2 * (size_t)d->ncfact * d->nscat * M))) return st;
     if ((st = ensure(c, b.rten, sizeof(double) * 9 * M))) return st;
     if ((st = ensure(c, b.sq, sizeof(float) * (size_t)(gg.R + gg.C) * M))) return st;
@@ -262,24 +281,10 @@ int build_state(dc_ctx *c, const dc_batch_desc *d, BatchState &b) {
     return 0;
 }
 
-// Kernels of one population step, in stream order (captured into the graph).
-int enqueue(dc_ctx *c, BatchState &b, int tile, int nwg, bool prep, bool rest) {
+// The reduction of the slice planes: every member's I, its chi^2 partial sums
+// and, when gradients are wanted, the Q planes.
+int reduce_planes(dc_ctx *c, BatchState &b) {
     const GemmGeom &gg = b.pl.gg;
-    if (prep) {
-        PrepArgs pa = base_prep(b.g, 0);
-        pa.src = ptr<double>(b.pos);
-        pa.src_ld = 0;
-        pa.idx = ptr<int>(b.idx);
-        pa.occ_src = ptr<double>(b.occ);
-        pa.bdw_src = b.any_b ? ptr<double>(b.bdw) : nullptr;
-        pa.n = b.nsorted;
-        if (b.nsorted > 0) HIPCHK(launch_prep(pa, ptr<AtomRec>(b.rec), c->st));
-    }
-    GemmGeom lg = launch_geom(c, gg);
-    lg.order = ptr<int>(b.sorder);
-    HIPCHK(launch_phase_gemm(c->prec, tile, ptr<AtomRec>(b.rec), ptr<int>(b.sbeg), ptr<float2>(b.partial),
-                             b.any_b ? ptr<float>(b.sq) : nullptr, lg, 0, b.nslice, c->st, nwg, ptr<int>(b.ssq)));
-    if (!rest) return 0;
     BatchReduceArgs ra{};
     ra.g = b.g;
     ra.R = gg.R;
@@ -300,7 +305,32 @@ int enqueue(dc_ctx *c, BatchState &b, int tile, int nwg, bool prep, bool rest) {
     ra.chi_part = ptr<double>(b.chi);
     ra.q = b.want_grad ? ptr<float2>(b.q) : nullptr;
     HIPCHK(launch_batch_reduce(ra, b.nblk, c->st));
-    if (b.want_grad) {
+    return 0;
+}
+
+// Kernels of one population step, in stream order (captured into the graph).
+// rest: 0 = records (prep) and GEMM only, 1 = and the reduction (I, chi^2,
+// Q planes), 2 = and the adjoint when gradients are wanted.
+int enqueue(dc_ctx *c, BatchState &b, int tile, int nwg, bool prep, int rest) {
+    const GemmGeom &gg = b.pl.gg;
+    if (prep) {
+        PrepArgs pa = base_prep(b.g, 0);
+        pa.src = ptr<double>(b.pos);
+        pa.src_ld = 0;
+        pa.idx = ptr<int>(b.idx);
+        pa.occ_src = ptr<double>(b.occ);
+        pa.bdw_src = b.any_b ? ptr<double>(b.bdw) : nullptr;
+        pa.n = b.nsorted;
+        if (b.nsorted > 0) HIPCHK(launch_prep(pa, ptr<AtomRec>(b.rec), c->st));
+    }
+    GemmGeom lg = launch_geom(c, gg);
+    lg.order = ptr<int>(b.sorder);
+    HIPCHK(launch_phase_gemm(c->prec, tile, ptr<AtomRec>(b.rec), ptr<int>(b.sbeg), ptr<float2>(b.partial),
+                             b.any_b ? ptr<float>(b.sq) : nullptr, lg, 0, b.nslice, c->st, nwg, ptr<int>(b.ssq)));
+    if (rest < 1) return 0;
+    int st = reduce_planes(c, b);
+    if (st) return st;
+    if (rest >= 2 && b.want_grad) {
         HIPCHK(launch_fill_zero(ptr<double>(b.grad), 5 * b.natoms, c->st));
         AdjArgs aa{};
         aa.rec = ptr<AtomRec>(b.rec);
@@ -341,7 +371,16 @@ int enqueue(dc_ctx *c, BatchState &b, int tile, int nwg, bool prep, bool rest) {
     return 0;
 }
 
-// Time every tile configuration and persistent grid size on this batch.
+// Time every tile configuration and persistent grid size on this batch at
+// the planned slice length, then the best tile at half and at twice that
+// length (its three grid sizes each).  A candidate costs its GEMM plus the
+// reduction of its slice planes, whose number the slice length sets: longer
+// slices mean fewer planes to write and reduce, shorter ones more, smaller
+// work items to balance over the CUs.  Another length must beat the planned
+// one by 2%, more than the spread of these short timings: the slice length
+// changes the FP32 summation order, so a choice made on noise would change
+// the results' last bits from one context to the next for nothing (on the
+// 96-member population the planned length wins).
 int autotune(dc_ctx *c, BatchState &b) {
     const bool fp32_only = b.prec == DC_PREC_FP32;    // tiles built without an FP32 variant are skipped
     if (c->tile >= 0 && !(fp32_only && gemm_tile_info(c->tile).no_fp32)) b.pl.tile = c->tile;
@@ -350,8 +389,23 @@ int autotune(dc_ctx *c, BatchState &b) {
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, c->device));
     const int ncu = prop.multiProcessorCount;
-    float best = 1e30f;
-    int bt = b.pl.tile, bn = 0;
+    // mean of 2 launches after a warm-up: the GEMM of (tile, nwg), or the reduction
+    auto time_ms = [&](int t, int nwg, bool reduce, float &ms) -> int {
+        int st = reduce ? reduce_planes(c, b) : enqueue(c, b, t, nwg, false, 0);
+        if (st) return st;
+        HIPCHK(hipEventRecord(c->ev[2], c->st));
+        for (int r = 0; r < 2; ++r)
+            if ((st = reduce ? reduce_planes(c, b) : enqueue(c, b, t, nwg, false, 0))) return st;
+        HIPCHK(hipEventRecord(c->ev[3], c->st));
+        HIPCHK(hipEventSynchronize(c->ev[3]));
+        HIPCHK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+        ms *= 0.5f;
+        return 0;
+    };
+    float best = 1e30f, best_gemm = 0.f, red = 0.f;
+    int bt = b.pl.tile, bn = 0, bk = b.kchunk;
+    int st = time_ms(0, 0, true, red);
+    if (st) return st;
     // candidates: tiles whose padded work is within 1.5x of the least padded one
     double wmin = 1e300;
     for (int t = 0; t < gemm_num_tiles(); ++t) {
@@ -364,25 +418,39 @@ int autotune(dc_ctx *c, BatchState &b) {
         if (fp32_only && ti.no_fp32) continue;
         if ((double)padded(b.pl.gg.R, ti.rt) * (double)padded(b.pl.gg.C, ti.ct) > 1.5 * wmin) continue;
         for (int nwg : {0, ncu, 2 * ncu}) {
-            int st = enqueue(c, b, t, nwg, false, false);    // warm-up
-            if (st) return st;
-            HIPCHK(hipEventRecord(c->ev[2], c->st));
-            for (int r = 0; r < 2; ++r)
-                if ((st = enqueue(c, b, t, nwg, false, false))) return st;
-            HIPCHK(hipEventRecord(c->ev[3], c->st));
-            HIPCHK(hipEventSynchronize(c->ev[3]));
-            float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
-            if (ms < best) {
-                best = ms;
+            float g = 0.f;
+            if ((st = time_ms(t, nwg, false, g))) return st;
+            if (g + red < best) {
+                best = g + red;
+                best_gemm = g;
                 bt = t;
                 bn = nwg;
             }
         }
     }
+    // slice length, on the best tile (a forced length is kept)
+    if (c->slice_atoms <= 0) {
+        const int k0 = b.kchunk;
+        for (int kc : {(k0 / 2 + 15) / 16 * 16, 2 * k0}) {
+            if (kc < 256 || kc == k0) continue;
+            if ((st = make_slices(c, b, kc))) return st;
+            if ((st = time_ms(0, 0, true, red))) return st;
+            for (int nwg : {0, ncu, 2 * ncu}) {
+                float g = 0.f;
+                if ((st = time_ms(bt, nwg, false, g))) return st;
+                if (g + red < (bk == k0 ? 0.98f : 1.0f) * best) {
+                    best = g + red;
+                    best_gemm = g;
+                    bn = nwg;
+                    bk = kc;
+                }
+            }
+        }
+        if (b.kchunk != bk && (st = make_slices(c, b, bk))) return st;
+    }
     b.pl.tile = bt;
     b.nwg = bn;
-    b.tune_best_ms = best / 2.0f;
+    b.tune_best_ms = best_gemm;
     return 0;
 }
 
@@ -465,12 +533,12 @@ int batch_chi2(dc_ctx *c, const dc_batch_desc *d, const double *target, const do
 
     if (!b.primed) {
         // records of this call, then the tile / grid-size choice, then the graph
-        if ((st = enqueue(c, b, b.pl.tile, 0, true, false))) return st;
+        if ((st = enqueue(c, b, b.pl.tile, 0, true, 0))) return st;
         if ((st = autotune(c, b))) return st;
         b.primed = true;
         if (c->graphs) {
             HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-            st = enqueue(c, b, b.pl.tile, b.nwg, true, true);
+            st = enqueue(c, b, b.pl.tile, b.nwg, true, 2);
             hipGraph_t graph = nullptr;
             hipError_t e = hipStreamEndCapture(c->st, &graph);
             if (st) {
@@ -486,7 +554,7 @@ int batch_chi2(dc_ctx *c, const dc_batch_desc *d, const double *target, const do
     if (b.exec) {
         HIPCHK(hipGraphLaunch(b.exec, c->st));
     } else {
-        if ((st = enqueue(c, b, b.pl.tile, b.nwg, true, true))) return st;
+        if ((st = enqueue(c, b, b.pl.tile, b.nwg, true, 2))) return st;
     }
     HIPCHK(hipEventRecord(c->ev[5], c->st));
     std::vector<double> part((size_t)b.nblk * M);

@@ -353,7 +353,13 @@ int dc_batch_chi2(dc_ctx *ctx, const dc_batch_desc *batch, const double *target,
  * dc_batch_chi2.  Returns when chi2 is written. */
 int dc_batch_chi2_dev(dc_ctx *ctx, const dc_batch_desc *batch, const double *target, const double *weight,
                       double *chi2, double *icalc, double *grad);
-/* Autotuner and hipGraph capture of batch calls (both on by default). */
+/* Autotuner and hipGraph capture of batch calls (both on by default).  The
+ * autotuner times the GEMM tiles and persistent grid sizes on the first call
+ * of a batch shape, and the planned slice length against half and twice it;
+ * a slice length other than the planned one (taken only when it is faster
+ * by 2% or more) changes the FP32 summation order, so results can then
+ * differ from another context's in the last bits.  dc_set_slice_atoms fixes
+ * the length, dc_set_autotune(ctx, 0) the whole plan. */
 int dc_set_autotune(dc_ctx *ctx, int on);
 int dc_set_graphs(dc_ctx *ctx, int on);
 /* Column walk of the reverse pass (every precision) on grids of at most 80
