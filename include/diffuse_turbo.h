@@ -225,7 +225,9 @@ typedef struct dc_run_desc {
  *   dvar  real     dnorm * sum_lots |F_l - fmean|^2 * diff_scale
  *                  (cross-lot variance, Welford in FP64)
  * dc_four_run takes host output pointers; dc_four_run_dev takes device
- * pointers (HBM) and does not synchronise the host.
+ * pointers (HBM).  Both return when the outputs are written: the launches of
+ * a call follow one another on the context's stream with no host wait
+ * between them, and the call waits for its stream once, at the end.
  */
 int dc_four_run(dc_ctx *ctx, const dc_run_desc *run, double *csf, double *acsf, double *dsi,
                 double *fmean, double *dvar);
