@@ -54,6 +54,22 @@ __host__ __device__ inline uint64_t dither_turn(uint32_t k, uint32_t salt) {
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     return z ^ (z >> 31);
 }
+// The same rotation from a 32-bit avalanche hash (the murmur3 finaliser) in
+// the top 32 bits of the turn: about a third of the VALU work of the 64-bit
+// one, which every thread of a GEMM stage evaluates for its atom (split
+// schemes: headline GEMM -1.9%).  The FP32 schemes keep the 64-bit hash:
+// their results stay as they were, one of their 1e-6 tests sitting within
+// 10% of the bar.
+__host__ __device__ inline uint64_t dither_turn32(uint32_t k, uint32_t salt) {
+    if (salt == 0) return 0;
+    uint32_t h = k * 0x9E3779B1u + salt;
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    return (uint64_t)h << 32;
+}
 
 // fraction of a FP64 value as 64-bit fixed point (exact for |t| < 2^11:
 // every mantissa bit of t - floor(t) lands inside the 64 fraction bits).

@@ -208,7 +208,7 @@ __device__ __forceinline__ float2 cis(uint64_t ph) {
 // measured 3-5% slower, 11.2 vs 10.7 ms on the headline):
 //   r < 8      low tables of side r/4, entries l = m, m+4, m+8, m+12
 //              (m = r % 4): TL0[l] = cis(l s + psi2), TL1[l] = cis(l s + w +
-//              psi2).  Split schemes (TWO_LEVEL): helper m evaluates
+//              psi2).  Split schemes (SPLIT): helper m evaluates
 //              cis(m s + psi2), cis(4m s) and cis(4m s + w) from their exact
 //              phases; the four helpers of an atom and side (lanes a + 16 m
 //              of one wave) exchange the last two by lane shuffles, and
@@ -229,7 +229,7 @@ __device__ __forceinline__ float2 cis(uint64_t ph) {
 //              A rows rotated by psi - psi2 and B rows by -psi - psi2
 // so every A element carries +psi and every B element -psi (dither_turn):
 // the products are unchanged, their rounding errors differ per atom.
-template <class T, bool TWO_LEVEL>
+template <class T, bool SPLIT>
 __device__ __forceinline__ void build_tables(Tables<T> &tb, const int *__restrict__ a_k16,
                                              const int *__restrict__ b_k16, const AtomRec &ar, bool live,
                                              const Side &sa, const Side &sb, int pt, float wscale, uint32_t katom,
@@ -237,7 +237,7 @@ __device__ __forceinline__ void build_tables(Tables<T> &tb, const int *__restric
     const int a = pt & (KS - 1), r = pt >> 4;
     constexpr int NR = T::NPT / KS;           // helper threads per atom
     static_assert(NR > 8, "the low tables take waves 0 and 1, the high rows need more threads");
-    const uint64_t psi = dither_turn(katom, salt);
+    const uint64_t psi = SPLIT ? dither_turn32(katom, salt) : dither_turn(katom, salt);
     const uint64_t psi2 = psi * 0xD6E8FEB86659FD93ull;
     if (r == 0) tb.dmp[a] = live ? ar.pad * 1.4426950408889634f : 0.0f;
     if (r < 8) {
@@ -247,7 +247,7 @@ __device__ __forceinline__ void build_tables(Tables<T> &tb, const int *__restric
         const uint64_t wrp = sideb ? ar.ca1 - (uint64_t)sb.period * ar.ca : ar.ra1 - (uint64_t)sa.period * ar.ra;
         float2 *t0 = sideb ? tb.bl[0] : tb.al[0];
         float2 *t1 = sideb ? tb.bl[1] : tb.al[1];
-        if constexpr (TWO_LEVEL) {
+        if constexpr (SPLIT) {
             const float2 base = cis((uint64_t)m * step + psi2);
             const uint64_t hs = (uint64_t)(4 * m) * step;
             const float2 g0 = cis(hs), g1 = cis(hs + wrp);
