@@ -235,7 +235,9 @@ int dc_four_run_dev(dc_ctx *ctx, const dc_run_desc *run, double *d_csf, double *
                     double *d_dsi, double *d_fmean, double *d_dvar);
 
 /* Statistics of the last call: atom.hkl pairs (full grid), device time of
- * the phase-GEMM kernels and of the whole call (ms, HIP events). */
+ * the phase-GEMM kernels and of the whole call (ms, HIP events).  The kernel
+ * timings are event pairs read after the call's launches complete:
+ * dc_last_stats waits for the context's stream when some are still pending. */
 typedef struct dc_stats {
     double pairs;
     double gemm_ms;
