@@ -41,7 +41,8 @@ struct BatchState {
     int walk_req = 0;                    // the context's adjoint walk setting at planning
     int64_t natoms = 0;
     DevBuf pos, occ, bdw, idx, rec, sbeg, stype, ssq, msl, sorder, partial, cfact, rten, sq, target, weight, icalc,
-        chi, q, qs, atiles, part, grad;
+        chi, q, qs, atiles, part, grad, wq;
+    bool queue = false;                  // GEMM items from a device counter (autotuner's choice)
     bool primed = false;                 // records built and autotuned once
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -49,7 +50,7 @@ struct BatchState {
 
     std::vector<DevBuf *> buffers() {
         return {&pos, &occ, &bdw, &idx, &rec, &sbeg, &stype, &ssq, &msl, &sorder, &partial, &cfact, &rten, &sq,
-                &target, &weight, &icalc, &chi, &q, &qs, &atiles, &part, &grad};
+                &target, &weight, &icalc, &chi, &q, &qs, &atiles, &part, &grad, &wq};
     }
 };
 
@@ -311,7 +312,11 @@ int reduce_planes(dc_ctx *c, BatchState &b) {
 // Kernels of one population step, in stream order (captured into the graph).
 // rest: 0 = records (prep) and GEMM only, 1 = and the reduction (I, chi^2,
 // Q planes), 2 = and the adjoint when gradients are wanted.
-int enqueue(dc_ctx *c, BatchState &b, int tile, int nwg, bool prep, int rest) {
+__global__ void zero_counter_kernel(int *p) {
+    if (threadIdx.x == 0) *p = 0;
+}
+
+int enqueue(dc_ctx *c, BatchState &b, int tile, int nwg, bool prep, int rest, bool queue) {
     const GemmGeom &gg = b.pl.gg;
     if (prep) {
         PrepArgs pa = base_prep(b.g, 0);
@@ -325,6 +330,13 @@ int enqueue(dc_ctx *c, BatchState &b, int tile, int nwg, bool prep, int rest) {
     }
     GemmGeom lg = launch_geom(c, gg);
     lg.order = ptr<int>(b.sorder);
+    if (queue) {
+        int st = ensure(c, b.wq, 256);
+        if (st) return st;
+        zero_counter_kernel<<<1, 64, 0, c->st>>>(ptr<int>(b.wq));
+        HIPCHK(hipGetLastError());
+        lg.queue = ptr<int>(b.wq);
+    }
     HIPCHK(launch_phase_gemm(c->prec, tile, ptr<AtomRec>(b.rec), ptr<int>(b.sbeg), ptr<float2>(b.partial),
                              b.any_b ? ptr<float>(b.sq) : nullptr, lg, 0, b.nslice, c->st, nwg, ptr<int>(b.ssq)));
     if (rest < 1) return 0;
@@ -371,9 +383,12 @@ int enqueue(dc_ctx *c, BatchState &b, int tile, int nwg, bool prep, int rest) {
     return 0;
 }
 
-// Time every tile configuration and persistent grid size on this batch at
-// the planned slice length, then the best tile at half and at twice that
-// length (its three grid sizes each).  A candidate costs its GEMM plus the
+// Time every tile configuration on this batch at the planned slice length,
+// each with one work-group per item, #CU or 2 #CU persistent work-groups
+// (static item assignment), or #CU work-groups taking items from a device
+// counter (longest first, each as it finishes one: the cfg 5 population's
+// GEMM 2.49 -> 2.43 ms and a steadier step); then the best tile at half and
+// at twice that length, with the same four launch forms.  A candidate costs its GEMM plus the
 // reduction of its slice planes, whose number the slice length sets: longer
 // slices mean fewer planes to write and reduce, shorter ones more, smaller
 // work items to balance over the CUs.  Another length must beat the planned
@@ -391,11 +406,13 @@ int autotune(dc_ctx *c, BatchState &b) {
     const int ncu = prop.multiProcessorCount;
     // mean of 2 launches after a warm-up: the GEMM of (tile, nwg), or the reduction
     auto time_ms = [&](int t, int nwg, bool reduce, float &ms) -> int {
-        int st = reduce ? reduce_planes(c, b) : enqueue(c, b, t, nwg, false, 0);
+        const bool qu = nwg < 0;
+        const int n = qu ? -nwg : nwg;
+        int st = reduce ? reduce_planes(c, b) : enqueue(c, b, t, n, false, 0, qu);
         if (st) return st;
         HIPCHK(hipEventRecord(c->ev[2], c->st));
         for (int r = 0; r < 2; ++r)
-            if ((st = reduce ? reduce_planes(c, b) : enqueue(c, b, t, nwg, false, 0))) return st;
+            if ((st = reduce ? reduce_planes(c, b) : enqueue(c, b, t, n, false, 0, qu))) return st;
         HIPCHK(hipEventRecord(c->ev[3], c->st));
         HIPCHK(hipEventSynchronize(c->ev[3]));
         HIPCHK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
@@ -417,7 +434,7 @@ int autotune(dc_ctx *c, BatchState &b) {
         const TileInfo ti = gemm_tile_info(t);
         if (fp32_only && ti.no_fp32) continue;
         if ((double)padded(b.pl.gg.R, ti.rt) * (double)padded(b.pl.gg.C, ti.ct) > 1.5 * wmin) continue;
-        for (int nwg : {0, ncu, 2 * ncu}) {
+        for (int nwg : {0, ncu, 2 * ncu, -ncu}) {
             float g = 0.f;
             if ((st = time_ms(t, nwg, false, g))) return st;
             if (g + red < best) {
@@ -435,7 +452,7 @@ int autotune(dc_ctx *c, BatchState &b) {
             if (kc < 256 || kc == k0) continue;
             if ((st = make_slices(c, b, kc))) return st;
             if ((st = time_ms(0, 0, true, red))) return st;
-            for (int nwg : {0, ncu, 2 * ncu}) {
+            for (int nwg : {0, ncu, 2 * ncu, -ncu}) {
                 float g = 0.f;
                 if ((st = time_ms(bt, nwg, false, g))) return st;
                 if (g + red < (bk == k0 ? 0.98f : 1.0f) * best) {
@@ -449,7 +466,8 @@ int autotune(dc_ctx *c, BatchState &b) {
         if (b.kchunk != bk && (st = make_slices(c, b, bk))) return st;
     }
     b.pl.tile = bt;
-    b.nwg = bn;
+    b.queue = bn < 0;
+    b.nwg = bn < 0 ? -bn : bn;
     b.tune_best_ms = best_gemm;
     return 0;
 }
@@ -533,12 +551,12 @@ int batch_chi2(dc_ctx *c, const dc_batch_desc *d, const double *target, const do
 
     if (!b.primed) {
         // records of this call, then the tile / grid-size choice, then the graph
-        if ((st = enqueue(c, b, b.pl.tile, 0, true, 0))) return st;
+        if ((st = enqueue(c, b, b.pl.tile, 0, true, 0, false))) return st;
         if ((st = autotune(c, b))) return st;
         b.primed = true;
         if (c->graphs) {
             HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-            st = enqueue(c, b, b.pl.tile, b.nwg, true, 2);
+            st = enqueue(c, b, b.pl.tile, b.nwg, true, 2, b.queue);
             hipGraph_t graph = nullptr;
             hipError_t e = hipStreamEndCapture(c->st, &graph);
             if (st) {
@@ -554,7 +572,7 @@ int batch_chi2(dc_ctx *c, const dc_batch_desc *d, const double *target, const do
     if (b.exec) {
         HIPCHK(hipGraphLaunch(b.exec, c->st));
     } else {
-        if ((st = enqueue(c, b, b.pl.tile, b.nwg, true, 2))) return st;
+        if ((st = enqueue(c, b, b.pl.tile, b.nwg, true, 2, b.queue))) return st;
     }
     HIPCHK(hipEventRecord(c->ev[5], c->st));
     std::vector<double> part((size_t)b.nblk * M);

@@ -38,6 +38,7 @@ struct GemmGeom {
     long long P;
     uint32_t dither;   // per-atom phase rotation salt (0: none), see dither_turn
     const int *order;  // launch order of the slices (null: slice0, slice0 + 1, ...)
+    int *queue;        // work-item counter, zero at launch (null: static assignment)
 };
 
 // Per-atom phase rotation of the phase GEMM: a pseudo-random turn fraction

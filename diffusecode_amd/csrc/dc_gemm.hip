@@ -574,12 +574,22 @@ phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice
     // XCD's L2 for the slice's atom records (headline FP16X3: HBM reads 225 ->
     // 72 MB per launch, GEMM -0.3%, measured A/B, identical results).
     int item0 = blockIdx.x;
-    if (!g.order && (int)gridDim.x == nitems) {
+    if (!g.queue && !g.order && (int)gridDim.x == nitems) {
         constexpr int NX = 8;
         const int q = nitems / NX, r = nitems % NX, x = blockIdx.x % NX, k = blockIdx.x / NX;
         item0 = (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
     }
-    for (int item = item0; item < nitems; item += gridDim.x) {
+    // with a work-item counter every work-group takes the next item when it
+    // is done with one (uniform across the work-group through LDS)
+    __shared__ int s_item;
+    auto next_item = [&](int cur) -> int {
+        if (!g.queue) return cur + (int)gridDim.x;
+        __syncthreads();
+        if (threadIdx.x == 0) s_item = atomicAdd(g.queue, 1);
+        __syncthreads();
+        return s_item;
+    };
+    for (int item = g.queue ? next_item(0) : item0; item < nitems; item = next_item(item)) {
         const int si = item / (nct * nrt);
         const int slice = slice0 + (g.order ? g.order[si] : si);
         const int kbeg = slice_beg[slice], kend = slice_beg[slice + 1];
