@@ -502,7 +502,8 @@ class Context:
     def batch_info(self):
         t, n, g, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
         self.check(lib().dc_batch_info(self.h, C.byref(t), C.byref(n), C.byref(g), C.byref(ms)))
-        return {"tile": t.value, "nwg": n.value, "graph": bool(g.value), "tune_ms": ms.value}
+        return {"tile": t.value, "nwg": abs(n.value), "queue": n.value < 0, "graph": bool(g.value),
+                "tune_ms": ms.value}
 
     @staticmethod
     def batch_desc(atom_beg, pos, types, occ, bdw, cfact, rten, eck, vi, inc):

@@ -491,7 +491,7 @@ int dc_set_graphs(dc_ctx *c, int on) {
 int dc_batch_info(const dc_ctx *c, int *tile, int *nwg, int *graph, double *tune_ms) {
     if (!c || !c->batch) return DC_ERR_STATE;
     if (tile) *tile = c->batch->pl.tile;
-    if (nwg) *nwg = c->batch->nwg;
+    if (nwg) *nwg = c->batch->queue ? -c->batch->nwg : c->batch->nwg;
     if (graph) *graph = c->batch->exec != nullptr;
     if (tune_ms) *tune_ms = c->batch->tune_best_ms;
     return DC_OK;

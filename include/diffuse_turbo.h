@@ -375,7 +375,9 @@ int dc_set_graphs(dc_ctx *ctx, int on);
  * dc_stats.adjoint_walk. */
 int dc_set_adjoint_walk(dc_ctx *ctx, int walk);
 /* Choice of the last batch shape: tile configuration, persistent work-groups
- * (0 = one per work item), whether a graph replays it, autotuned GEMM ms. */
+ * (0 = one per work item; negative = that many work-groups taking the work
+ * items, longest first, from a device counter), whether a graph replays it,
+ * autotuned GEMM ms. */
 int dc_batch_info(const dc_ctx *ctx, int *tile, int *nwg, int *graph, double *tune_ms);
 
 /* ------------------------------------------------------------------------
