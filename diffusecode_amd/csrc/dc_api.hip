@@ -1142,6 +1142,7 @@ int dc_four_run_dev(dc_ctx *c, const dc_run_desc *r, double *d_csf, double *d_ac
         ra.lform = 1;
         ra.cfact = ptr<double2>(c->cfact);
         ra.ncfact = r->ncfact;
+        ra.ntype = c->nscat;
         ra.acsf = acsf;
         ra.dsi = ptr<double>(c->dsi);
         ra.csf = ptr<double>(c->csf);

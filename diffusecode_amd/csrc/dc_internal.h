@@ -91,6 +91,7 @@ struct ReduceArgs {
     int lform;
     const double2 *cfact;   // (ncfact, nscat)
     int ncfact;
+    int ntype;              // cfact columns (slice types < ntype); 0 = not given
     const int *istl;        // optional bins (full-grid layout), null = compute
     const double *acsf;     // compact region, null = zero
     double *dsi, *csf, *wm, *wv;

@@ -100,10 +100,28 @@ __device__ __forceinline__ int pixel_bin(const Grid &g, const int *istl, long lo
 // (FP64), then (four_run, fourier_sup.f90:224-254)
 //   csf = F_l - acsf ; dsi += |csf|^2 ; Welford moments of F_l.
 // fresh: the first batch, whose dsi and moments start from zero.
-__global__ void reduce_lots_kernel(ReduceArgs a) {
-    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+//
+// One thread per pixel walks the batch's slices in order, RED_BATCH planes at
+// a time, and issues the loads of the next RED_BATCH planes before it sums the
+// current ones, so that a wave keeps loads in flight while it computes (with
+// loads issued only between sums, a many-lot batch ran at under half the
+// rate a plain read of its planes reaches).  CF_LDS: the pixel's form factor
+// of every type (1..RED_LDS_TYPES types, every slice type < a.ntype) is read
+// once into LDS, so that no global load in the sums waits behind the
+// prefetched planes; otherwise the sums load it from cfact.  Slices of a lot come in runs
+// of one type: a run's planes are summed, then that type's form factor is
+// applied once.  A lot without slices adds F_l = 0.
+constexpr int RED_BATCH = 8;
+constexpr int RED_BLOCK = 256;
+constexpr int RED_LDS_TYPES = 8;
+template <bool CF_LDS>
+__global__ __launch_bounds__(RED_BLOCK) void reduce_lots_kernel(ReduceArgs a) {
+    extern __shared__ double2 cf_lds[];   // CF_LDS: [type][thread]
+    const long long p = (long long)blockIdx.x * RED_BLOCK + threadIdx.x;
     if (p >= a.g.P) return;
     const int bin = a.lform ? pixel_bin(a.g, a.istl, p, a.ncfact) : 0;
+    if constexpr (CF_LDS)
+        for (int t = 0; t < a.ntype; ++t) cf_lds[t * RED_BLOCK + threadIdx.x] = a.cfact[(long long)t * a.ncfact + bin];
     double ar = 0.0, ai = 0.0;
     if (a.acsf) { ar = a.acsf[2 * p]; ai = a.acsf[2 * p + 1]; }
     const bool rd = !a.fresh;
@@ -111,43 +129,66 @@ __global__ void reduce_lots_kernel(ReduceArgs a) {
     double mr = a.wm && rd ? a.wm[2 * p] : 0.0, mi = a.wm && rd ? a.wm[2 * p + 1] : 0.0;
     double m2 = a.wv && rd ? a.wv[p] : 0.0;
     double cr = 0.0, ci = 0.0;
-    const long long po = plane_off(a.g, p);
-    int s = a.lot_slice[0];
-    for (int l = 0; l < a.nlots; ++l) {
-        const int send = a.lot_slice[l + 1];
-        double fr = 0.0, fi = 0.0;
-        while (s < send) {
-            // slices of a lot come in runs of one type: sum the run's planes,
-            // then apply that type's form factor once
-            const int t = a.slice_type ? a.slice_type[s] : 0;
-            int e = s + 1;
-            while (e < send && (a.slice_type ? a.slice_type[e] : 0) == t) ++e;
-            double sr = 0.0, si = 0.0;
-#pragma unroll 4
-            for (; s < e; ++s) {
-                const float2 v = a.partial[(long long)(s - a.slice0) * a.g.plane + po];
-                sr += (double)v.x;
-                si += (double)v.y;
+    const int s0 = a.lot_slice[0], s1 = a.lot_slice[a.nlots];
+    int l = 0;
+    double fr = 0.0, fi = 0.0, sr = 0.0, si = 0.0;
+    auto finish_lots = [&](int pos) {   // every lot ending at slice position pos
+        while (l < a.nlots && a.lot_slice[l + 1] == pos) {
+            if (a.wm) {
+                const double inv = 1.0 / (double)(a.lot0 + l + 1);
+                const double dr = fr - mr, di = fi - mi;
+                mr += dr * inv;
+                mi += di * inv;
+                m2 += dr * (fr - mr) + di * (fi - mi);
             }
-            if (a.lform) {
-                const double2 cf = a.cfact[(long long)t * a.ncfact + bin];
-                fr += cf.x * sr - cf.y * si;
-                fi += cf.x * si + cf.y * sr;
-            } else {
-                fr += sr;
-                fi += si;
+            cr = fr - ar;
+            ci = fi - ai;
+            dsi += cr * cr + ci * ci;
+            fr = fi = 0.0;
+            ++l;
+        }
+    };
+    finish_lots(s0);
+    if (s0 < s1) {
+        const long long pl = a.g.plane;
+        const float2 *base = a.partial + (long long)(s0 - a.slice0) * pl + plane_off(a.g, p);
+        // planes past the last slice repeat it: every load is unconditional
+        // and in bounds, and the sums stop at s1
+        float2 nxt[RED_BATCH];
+#pragma unroll
+        for (int u = 0; u < RED_BATCH; ++u) nxt[u] = base[(long long)min(u, s1 - 1 - s0) * pl];
+        for (int sb = s0; sb < s1; sb += RED_BATCH) {
+            float2 v[RED_BATCH];
+#pragma unroll
+            for (int u = 0; u < RED_BATCH; ++u) v[u] = nxt[u];
+            if (sb + RED_BATCH < s1) {
+#pragma unroll
+                for (int u = 0; u < RED_BATCH; ++u)
+                    nxt[u] = base[(long long)min(sb + RED_BATCH + u - s0, s1 - 1 - s0) * pl];
+            }
+#pragma unroll
+            for (int u = 0; u < RED_BATCH; ++u) {
+                const int s = sb + u;
+                if (s >= s1) break;
+                sr += (double)v[u].x;
+                si += (double)v[u].y;
+                const bool lot_end = a.lot_slice[l + 1] == s + 1;
+                const int t = a.slice_type ? a.slice_type[s] : 0;
+                if (lot_end || (a.slice_type ? a.slice_type[s + 1] : 0) != t) {
+                    if (a.lform) {
+                        const double2 cf = CF_LDS ? cf_lds[t * RED_BLOCK + threadIdx.x]
+                                                  : a.cfact[(long long)t * a.ncfact + bin];
+                        fr += cf.x * sr - cf.y * si;
+                        fi += cf.x * si + cf.y * sr;
+                    } else {
+                        fr += sr;
+                        fi += si;
+                    }
+                    sr = si = 0.0;
+                }
+                if (lot_end) finish_lots(s + 1);
             }
         }
-        if (a.wm) {
-            const double inv = 1.0 / (double)(a.lot0 + l + 1);
-            const double dr = fr - mr, di = fi - mi;
-            mr += dr * inv;
-            mi += di * inv;
-            m2 += dr * (fr - mr) + di * (fi - mi);
-        }
-        cr = fr - ar;
-        ci = fi - ai;
-        dsi += cr * cr + ci * ci;
     }
     a.dsi[p] = dsi;
     a.csf[2 * p] = cr;
@@ -321,7 +362,10 @@ hipError_t launch_prep(const PrepArgs &a, AtomRec *out, hipStream_t st) {
     return hipGetLastError();
 }
 hipError_t launch_reduce_lots(const ReduceArgs &a, hipStream_t st) {
-    reduce_lots_kernel<<<nblk(a.g.P, 256), 256, 0, st>>>(a);
+    if (a.lform && a.ntype >= 1 && a.ntype <= RED_LDS_TYPES)
+        reduce_lots_kernel<true><<<nblk(a.g.P, RED_BLOCK), RED_BLOCK, sizeof(double2) * RED_BLOCK * a.ntype, st>>>(a);
+    else
+        reduce_lots_kernel<false><<<nblk(a.g.P, RED_BLOCK), RED_BLOCK, 0, st>>>(a);
     return hipGetLastError();
 }
 hipError_t launch_reduce_sum(const ReduceArgs &a, double *out, int accumulate, hipStream_t st) {

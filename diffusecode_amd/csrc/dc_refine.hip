@@ -164,6 +164,7 @@ int model_forward(dc_ctx *c, const dc_model_desc *m, ModelPlan &mp) {
     ra.lform = 1;
     ra.cfact = ptr<double2>(c->cfact);
     ra.ncfact = m->ncfact;
+    ra.ntype = m->nscat;
     ra.dsi = ptr<double>(c->dsi);
     ra.csf = ptr<double>(c->m_f);
     ra.fresh = 1;
