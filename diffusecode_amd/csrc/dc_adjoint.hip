@@ -831,9 +831,20 @@ __global__ void chi2_resid_kernel(const double *icalc, const double *tgt, const 
     if (threadIdx.x == 0) blocksum[blockIdx.x] = red[0];
 }
 
-// One member per blockIdx.y; grid-stride over the member's pixels.
-__global__ void batch_reduce_kernel(BatchReduceArgs a) {
-    __shared__ double red[256];
+// One member per blockIdx.y; grid-stride over the member's pixels.  A
+// pixel's planes are loaded BR_BATCH at a time, all issued before they are
+// summed (planes past the member's last slice repeat it, so every load is
+// in bounds); CF_LDS: the pixel's form factor of each of the a.nscat
+// (<= BR_LDS_TYPES) types is read once into LDS, so that the sums and the Q
+// planes wait on no further global load.  Same arithmetic, same order, with
+// or without.
+constexpr int BR_BATCH = 8;
+constexpr int BR_BLOCK = 256;
+constexpr int BR_LDS_TYPES = 8;
+template <bool CF_LDS>
+__global__ __launch_bounds__(BR_BLOCK) void batch_reduce_kernel(BatchReduceArgs a) {
+    __shared__ double red[BR_BLOCK];
+    extern __shared__ double2 cf_lds[];   // CF_LDS: [type][thread]
     const int m = blockIdx.y;
     Grid g = a.g;
     for (int k = 0; k < 9; ++k) g.rten[k] = a.rten[9 * m + k];
@@ -848,24 +859,38 @@ __global__ void batch_reduce_kernel(BatchReduceArgs a) {
         int bin = istl_point(g, i, j, l);
         bin = bin >= a.ncfact ? a.ncfact - 1 : bin;
         const long long po = i * g.ps[0] + j * g.ps[1] + l * g.ps[2];
-        // the member's slices come in runs of one type: sum a run's planes
-        // (four loads in flight), then apply that type's form factor once
-        double fr = 0.0, fi = 0.0;
-        int s = s0;
-        while (s < s1) {
-            const int t = a.slice_type[s];
-            int e = s + 1;
-            while (e < s1 && a.slice_type[e] == t) ++e;
-            double sr = 0.0, si = 0.0;
-#pragma unroll 4
-            for (; s < e; ++s) {
-                const float2 v = a.partial[(long long)s * g.plane + po];
-                sr += (double)v.x;
-                si += (double)v.y;
+        if constexpr (CF_LDS)
+            for (int t = 0; t < a.nscat; ++t) cf_lds[t * BR_BLOCK + threadIdx.x] = cf[(long long)t * a.ncfact + bin];
+        auto cfv = [&](int t) {
+            if constexpr (CF_LDS) return cf_lds[t * BR_BLOCK + threadIdx.x];
+            else return cf[(long long)t * a.ncfact + bin];
+        };
+        // the member's slices come in runs of one type: sum a run's planes,
+        // then apply that type's form factor once
+        double fr = 0.0, fi = 0.0, sr = 0.0, si = 0.0;
+        for (int sb = s0; sb < s1; sb += BR_BATCH) {
+            // the batch's slice types first, then its planes: the sums wait
+            // for the types without waiting for the planes behind them
+            int tt[BR_BATCH + 1];
+#pragma unroll
+            for (int u = 0; u <= BR_BATCH; ++u) tt[u] = a.slice_type[min(sb + u, s1 - 1)];
+            float2 v[BR_BATCH];
+#pragma unroll
+            for (int u = 0; u < BR_BATCH; ++u) v[u] = a.partial[(long long)min(sb + u, s1 - 1) * g.plane + po];
+#pragma unroll
+            for (int u = 0; u < BR_BATCH; ++u) {
+                const int s = sb + u;
+                if (s >= s1) break;
+                sr += (double)v[u].x;
+                si += (double)v[u].y;
+                const int t = tt[u];
+                if (s + 1 == s1 || tt[u + 1] != t) {
+                    const double2 w = cfv(t);
+                    fr += w.x * sr - w.y * si;
+                    fi += w.x * si + w.y * sr;
+                    sr = si = 0.0;
+                }
             }
-            const double2 u = cf[(long long)t * a.ncfact + bin];
-            fr += u.x * sr - u.y * si;
-            fi += u.x * si + u.y * sr;
         }
         const long long mp = (long long)m * P + p;
         const double I = fr * fr + fi * fi;
@@ -878,7 +903,7 @@ __global__ void batch_reduce_kernel(BatchReduceArgs a) {
             const double rr = g2 * fr, ri = -g2 * fi;
             const long long o = (long long)j + (long long)a.C0 * l + (long long)a.C * i;
             for (int t = 0; t < a.nscat; ++t) {
-                const double2 u = cf[(long long)t * a.ncfact + bin];
+                const double2 u = cfv(t);
                 a.q[((long long)m * a.nscat + t) * a.plane + o] =
                     make_float2((float)(rr * u.x - ri * u.y), (float)(rr * u.y + ri * u.x));
             }
@@ -944,7 +969,11 @@ __global__ void fill_zero_kernel(double *p, long long n) {
 
 hipError_t launch_batch_reduce(const BatchReduceArgs &a, int nblocks, hipStream_t st) {
     if (a.nmembers <= 0) return hipSuccess;
-    batch_reduce_kernel<<<dim3(nblocks, a.nmembers), 256, 0, st>>>(a);
+    if (a.nscat >= 1 && a.nscat <= BR_LDS_TYPES)
+        batch_reduce_kernel<true>
+            <<<dim3(nblocks, a.nmembers), BR_BLOCK, sizeof(double2) * BR_BLOCK * a.nscat, st>>>(a);
+    else
+        batch_reduce_kernel<false><<<dim3(nblocks, a.nmembers), BR_BLOCK, 0, st>>>(a);
     return hipGetLastError();
 }
 
