@@ -111,7 +111,7 @@ __device__ __forceinline__ int pixel_bin(const Grid &g, const int *istl, long lo
 // prefetched planes; otherwise the sums load it from cfact.  Slices of a lot come in runs
 // of one type: a run's planes are summed, then that type's form factor is
 // applied once.  A lot without slices adds F_l = 0.
-constexpr int RED_BATCH = 8;
+constexpr int RED_BATCH = 4;
 constexpr int RED_BLOCK = 256;
 constexpr int RED_LDS_TYPES = 8;
 template <bool CF_LDS>
