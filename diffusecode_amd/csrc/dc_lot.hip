@@ -108,9 +108,11 @@ __device__ __forceinline__ int pixel_bin(const Grid &g, const int *istl, long lo
 // rate a plain read of its planes reaches).  CF_LDS: the pixel's form factor
 // of every type (1..RED_LDS_TYPES types, every slice type < a.ntype) is read
 // once into LDS, so that no global load in the sums waits behind the
-// prefetched planes; otherwise the sums load it from cfact.  Slices of a lot come in runs
-// of one type: a run's planes are summed, then that type's form factor is
-// applied once.  A lot without slices adds F_l = 0.
+// prefetched planes; otherwise the sums load it from cfact.  Slices of a lot
+// come in runs of one type: a run's planes are summed, then that type's form
+// factor is applied once.  A lot without slices adds F_l = 0.  Batches of 4
+// planes keep the kernel at 64 VGPRs (8 waves per SIMD); batches of 8 ran a
+// 64-lot batch no faster and the short walks of one-lot runs slower.
 constexpr int RED_BATCH = 4;
 constexpr int RED_BLOCK = 256;
 constexpr int RED_LDS_TYPES = 8;
