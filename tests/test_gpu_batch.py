@@ -62,11 +62,13 @@ def ctx():
     c.close()
 
 
-@pytest.mark.parametrize("prec,bar", [("fp32", 1e-6), ("bf16x3", 1e-5), ("fp16x3", 1e-5)])
-def test_batch_matches_oracle(ctx, prec, bar):
+def check_population(ctx, prec, bar, build):
+    """Score the population build(ctx) returns in precision prec and check
+    every member's intensities (max|dI|/max(I) <= bar), chi^2 and gradients
+    against the oracle."""
     ctx.set_precision(prec)
     try:
-        pop = make_population(ctx=ctx)
+        pop = build(ctx)
         T = targets(pop)
         W = 1.0 / np.maximum(T, 1e-3 * T.max(axis=0))
         chi, I, g = pop.score(T, W, want_icalc=True, want_grad=True)
@@ -85,6 +87,25 @@ def test_batch_matches_oracle(ctx, prec, bar):
         assert e <= bar
         assert abs(chi[m] - chi_ref) <= 1e-4 * chi_ref
         assert ge <= (1e-4 if prec == "fp32" else 1e-3)
+
+
+@pytest.mark.parametrize("prec,bar", [("fp32", 1e-6), ("bf16x3", 1e-5), ("fp16x3", 1e-5)])
+def test_batch_matches_oracle(ctx, prec, bar):
+    check_population(ctx, prec, bar, lambda c: make_population(ctx=c))
+
+
+@pytest.mark.parametrize("prec,bar", [("fp32", 1e-6), ("fp16x3", 1e-5)])
+def test_batch_many_types(ctx, prec, bar):
+    """Members with ten atom types: more than the population reduction holds
+    per pixel in LDS (eight), so its sums and Q planes read the form factors
+    from global memory."""
+    def build(c):
+        crs = [X.fcc_ordered(n=4, ntypes=10, mc_steps=0, seed=11 + k) for k in range(3)]
+        eck, vi, inc = FR.grid_block((-3.0, -3.0, 0.0), (3.0, 3.0, 0.0), (33, 33, 1))
+        pop = Population.from_crystals(crs, eck, vi, inc, ctx=c, precision=prec)
+        assert all(len(np.unique(mb["types"])) == 10 for mb in pop.members)
+        return pop
+    check_population(ctx, prec, bar, build)
 
 
 def test_graph_replays_equal_eager():

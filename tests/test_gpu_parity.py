@@ -209,6 +209,26 @@ def test_four_run_lots_aver(ctx, prec):
         assert err <= LOTS_BAR[prec]
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp16x3"])
+def test_four_run_lots_many_types(ctx, prec):
+    """Ten atom types: more than the lot reduction holds per pixel in LDS
+    (eight), so its type runs take their form factors from global memory.
+    Box lots with periodic wrap, <F> subtraction and the cross-lot variance
+    against the exact oracle, as in test_four_run_lots_aver."""
+    print(f"case four_run lots, 10 types, prec={prec}", flush=True)
+    c = X.fcc_ordered(n=8, ntypes=10, mc_steps=0, seed=9)
+    assert c.nscat == 10
+    eck, vi, inc = FR.grid_block((-2, -2, 0.5), (2, 2, 0.5), (41, 41, 1))
+    rng = np.random.default_rng(5)
+    lbeg = np.floor(rng.random((5, 3)) * 8).astype(np.int32) + 1
+    sel = (rng.random((8, 8, 8)) <= 0.5).astype(np.uint8)
+    gpu, ref = run_pair(ctx, c, prec, eck, vi, inc, ilots=2, ls=(3, 3, 3), lbeg=lbeg, fave=0.5, sel_cell=sel)
+    for k in ("dsi", "dvar", "acsf"):
+        err = float(np.abs(gpu[k] - ref[k]).max() / np.abs(ref[k]).max())
+        print(f"  {k} max rel {err:.3e}", flush=True)
+        assert err <= LOTS_BAR[prec]
+
+
 def test_four_run_ellipsoid_3d(ctx):
     """Ellipsoid lots on a small centred 3D block (Friedel + mapping choice)."""
     print("case four_run ellipsoid 3d", flush=True)
