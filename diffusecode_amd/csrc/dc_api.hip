@@ -316,7 +316,8 @@ void add_segments(SliceSet &s, const std::vector<std::pair<int, int>> &segs, int
 }
 
 int run_gemm(dc_ctx *c, const Plan &pl, int nslice, float2 *partial, const float *sq) {
-    const GemmGeom gg = launch_geom(c, pl.gg);
+    GemmGeom gg = launch_geom(c, pl.gg);
+    gg.pad_hash = c->rec_hash && !sq;
     if (nslice <= 0) return 0;
     int st = 0;
     hipEvent_t e0, e1;
@@ -562,6 +563,7 @@ int strucf_core(dc_ctx *c, const Grid &g0, long long ntot, const double *d_xat, 
     pa.src = d_xat;
     pa.src_ld = std::max<int64_t>(ld_xat, 1);
     pa.n = nxat;
+    pa.hash_salt = launch_geom(c, pl.gg).dither;   // undamped launches below
     HIPCHK(launch_prep(pa, rec, c->st));
     SliceSet ss;
     int off = 0;
@@ -577,7 +579,8 @@ int strucf_core(dc_ctx *c, const Grid &g0, long long ntot, const double *d_xat, 
     ra.ncfact = (int)ncfact;
     ra.istl = d_istl;
     const int per = std::max(1, (int)std::min<double>(4096.0, std::floor(plane_bytes / one_plane)));
-    const GemmGeom gg = launch_geom(c, pl.gg);
+    GemmGeom gg = launch_geom(c, pl.gg);
+    gg.pad_hash = 1;
     HIPCHK(hipEventRecord(c->ev[2], c->st));
     for (int s0 = 0; s0 < std::max(nslice, 1); s0 += per) {
         if (cancelled(c)) return set_err(c, DC_ERR_CANCEL, DC_ER_COMM, "cancelled");
@@ -870,7 +873,9 @@ int gemm_segments(dc_ctx *c, const Grid &g, const Plan &pl, GatherList &gl,
         }
     }
     if ((st = ensure(c, c->rec, sizeof(AtomRec) * std::max<long long>(n, 1)))) return st;
+    pa.hash_salt = launch_geom(c, gg).dither;   // run_gemm below launches undamped
     HIPCHK(launch_prep(pa, ptr<AtomRec>(c->rec), c->st));
+    c->rec_hash = true;
     int off = 0;
     ss = SliceSet();
     add_segments(ss, gl.segs, slice_len(c, kchunk_for(pl, gl.segs, pl.scheme)), off);

@@ -39,6 +39,7 @@ struct GemmGeom {
     uint32_t dither;   // per-atom phase rotation salt (0: none), see dither_turn
     const int *order;  // launch order of the slices (null: slice0, slice0 + 1, ...)
     int *queue;        // work-item counter, zero at launch (null: static assignment)
+    int pad_hash;      // split schemes, undamped: AtomRec::pad holds dither_hash32 of the record (prep)
 };
 
 // Per-atom phase rotation of the phase GEMM: a pseudo-random turn fraction
@@ -61,15 +62,17 @@ __host__ __device__ inline uint64_t dither_turn(uint32_t k, uint32_t salt) {
 // schemes: headline GEMM -1.9%).  The FP32 schemes keep the 64-bit hash:
 // their results stay as they were, one of their 1e-6 tests sitting within
 // 10% of the bar.
-__host__ __device__ inline uint64_t dither_turn32(uint32_t k, uint32_t salt) {
-    if (salt == 0) return 0;
+__host__ __device__ inline uint32_t dither_hash32(uint32_t k, uint32_t salt) {
     uint32_t h = k * 0x9E3779B1u + salt;
     h ^= h >> 16;
     h *= 0x85EBCA6Bu;
     h ^= h >> 13;
     h *= 0xC2B2AE35u;
     h ^= h >> 16;
-    return (uint64_t)h << 32;
+    return h;
+}
+__host__ __device__ inline uint64_t dither_turn32(uint32_t k, uint32_t salt) {
+    return salt == 0 ? 0 : (uint64_t)dither_hash32(k, salt) << 32;
 }
 
 // fraction of a FP64 value as 64-bit fixed point (exact for |t| < 2^11:

@@ -233,11 +233,22 @@ template <class T, bool SPLIT>
 __device__ __forceinline__ void build_tables(Tables<T> &tb, const int *__restrict__ a_k16,
                                              const int *__restrict__ b_k16, const AtomRec &ar, bool live,
                                              const Side &sa, const Side &sb, int pt, float wscale, uint32_t katom,
-                                             uint32_t salt) {
+                                             uint32_t salt, bool padh) {
     const int a = pt & (KS - 1), r = pt >> 4;
     constexpr int NR = T::NPT / KS;           // helper threads per atom
     static_assert(NR > 8, "the low tables take waves 0 and 1, the high rows need more threads");
-    const uint64_t psi = SPLIT ? dither_turn32(katom, salt) : dither_turn(katom, salt);
+    // padh: the record carries the hash (dither_hash32 of katom, written by
+    // the record prep for undamped launches), so the 512 threads of the
+    // work-group do not each evaluate it every stage; same bits either way
+    uint64_t psi;
+    if constexpr (SPLIT) {
+        if (padh)
+            psi = (uint64_t)__builtin_bit_cast(uint32_t, ar.pad) << 32;
+        else
+            psi = dither_turn32(katom, salt);
+    } else {
+        psi = dither_turn(katom, salt);
+    }
     const uint64_t psi2 = psi * 0xD6E8FEB86659FD93ull;
     if (r == 0) tb.dmp[a] = live ? ar.pad * 1.4426950408889634f : 0.0f;
     if (r < 8) {
@@ -564,6 +575,7 @@ phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice
                   int nitems) {
     __shared__ Smem<SCHEME, T> sm;
     const int nct = (g.C + T::CT - 1) / T::CT, nrt = (g.R + T::RT - 1) / T::RT;
+    const bool padh = !DAMP && g.pad_hash != 0;   // rotation hash read from the records
     // work items (column tile fastest, then row tile, then slice); a launch
     // with fewer work-groups than items walks them with a grid stride.  The
     // slices are taken in g.order when given (longest first, so that uneven
@@ -705,11 +717,11 @@ phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice
                 wunscale = ldexpf(1.0f, e);
             }
         }
-        build_tables<T, !fp32ops(SCHEME)>(sm.t[0], sm.a_k16, sm.b_k16, r0, kbeg + at < kend, sa, sb, pt, wscale, kbeg + at, g.dither);
+        build_tables<T, !fp32ops(SCHEME)>(sm.t[0], sm.a_k16, sm.b_k16, r0, kbeg + at < kend, sa, sb, pt, wscale, kbeg + at, g.dither, padh);
         __syncthreads();
         const bool awrap = g.R0 < g.R, bwrap = g.C0 < g.C;
         build_elems<SCHEME, T, DAMP>(sm.e[0], sm.t[0], sm.a_wpos, sm.b_wpos, sm.s_a, sm.s_b, wave, awrap, bwrap);
-        if (nst > 1) build_tables<T, !fp32ops(SCHEME)>(sm.t[1], sm.a_k16, sm.b_k16, r1, kbeg + KS + at < kend, sa, sb, pt, wscale, kbeg + KS + at, g.dither);
+        if (nst > 1) build_tables<T, !fp32ops(SCHEME)>(sm.t[1], sm.a_k16, sm.b_k16, r1, kbeg + KS + at < kend, sa, sb, pt, wscale, kbeg + KS + at, g.dither, padh);
         __syncthreads();
         for (int s = 0; s < nst; ++s) {
             const int cur = s & 1, nxt = cur ^ 1;
@@ -745,7 +757,7 @@ phase_gemm_kernel(const AtomRec *__restrict__ rec, const int *__restrict__ slice
                 if constexpr (PRIO) __builtin_amdgcn_s_setprio(0);
             }
             if (s + 2 < nst) build_tables<T, !fp32ops(SCHEME)>(sm.t[cur], sm.a_k16, sm.b_k16, rt, kbeg + (s + 2) * KS + at < kend, sa, sb, pt, wscale,
-                                              kbeg + (s + 2) * KS + at, g.dither);
+                                              kbeg + (s + 2) * KS + at, g.dither, padh);
             if (s + 1 < nst) build_elems<SCHEME, T, DAMP>(sm.e[nxt], sm.t[nxt], sm.a_wpos, sm.b_wpos, sm.s_a, sm.s_b, wave, awrap, bwrap);
             if (!mfma_first && live) {
                 if constexpr (D64) {

@@ -70,6 +70,10 @@ struct dc_ctx {
     DevBuf runs;
     // this device's dc_scratch array (dc_noalloc.hip), looked up at creation
     void *scratch = nullptr;
+    // rec's records carry the rotation hash in AtomRec::pad (PrepArgs::hash_salt):
+    // set by the Fourier run's prep, cleared by the refinement's, whose
+    // adjoints read pad as B
+    bool rec_hash = false;
 
     std::vector<DevBuf *> buffers() {
         return {&rec, &slice_beg, &slice_type, &lot_slice, &partial, &cfact, &istl, &idx, &oid, &offs, &xat, &w,

@@ -21,6 +21,8 @@ struct PrepArgs {
     const float *bdw;       // per-atom isotropic B in AtomRec::pad (null: 0)
     const double *occ_src;  // caller-order occupancy read through idx (overrides w)
     const double *bdw_src;  // caller-order B read through idx (overrides bdw)
+    uint32_t hash_salt;     // nonzero: AtomRec::pad = dither_hash32(record, salt) instead of B
+                            // (records of undamped GEMMs only; the adjoints read pad as B)
     long long n;
     int mode;               // PREP_CORNER / PREP_LMN
     int mapping;            // 0: rows = abscissa ; 1: rows = (abscissa, ordinate)

@@ -69,6 +69,7 @@ __global__ void prep_records_kernel(PrepArgs a, AtomRec *__restrict__ out) {
     }
     r.w = a.occ_src ? (float)a.occ_src[src] : (a.w ? a.w[k] : 1.0f);
     r.pad = a.bdw_src ? (float)a.bdw_src[src] : (a.bdw ? a.bdw[k] : 0.0f);
+    if (a.hash_salt) r.pad = __builtin_bit_cast(float, dither_hash32((uint32_t)k, a.hash_salt));
     out[k] = r;
 }
 

@@ -132,6 +132,7 @@ int upload_model(dc_ctx *c, const dc_model_desc *m, const ModelPlan &mp) {
     pa.bdw = ptr<float>(c->m_b);
     pa.n = ns;
     HIPCHK(launch_prep(pa, ptr<AtomRec>(c->rec), c->st));
+    c->rec_hash = false;
     return 0;
 }
 
@@ -391,6 +392,7 @@ int dc_model_jvp(dc_ctx *c, const dc_model_desc *m, const double *fcalc, int ndi
         pa.bdw_src = m->bdw ? ptr<double>(c->m_bd) : nullptr;
         pa.n = (long long)w.size();
         if (pa.n > 0) HIPCHK(launch_prep(pa, ptr<AtomRec>(c->rec), c->st));
+        c->rec_hash = false;
         if ((st = run_gemm(c, mp.pl, nslice, ptr<float2>(c->partial), mp.any_b ? ptr<float>(c->m_sq) : nullptr))) return st;
         JvpArgs ja{};
         ja.g = mp.g;
